@@ -1,0 +1,193 @@
+"""Benchmark of the X-engine hot path on one MI355X.
+
+Workload (one "step"): one integration of the int8 MFMA X-engine at
+NSTATION=512 x 2 pol, NFREQUENCY=64, NTIME=8192 on input already resident in
+HBM (seeded uniform int8 in [-128, 127]), output left in the device
+integration buffer in REGISTER_TILE order.  2.754e11 complex MACs per step.
+
+Prints one JSON line: whole-job TCMAC/s, the fraction of the int8 MFMA peak,
+the FP32-plain kernel on the same workload, the FP16 MFMA path on its own
+workload, the roofline of the dominant kernel (timed with HIP events on the
+stream it runs on), and the reference CPU X-engine timed on host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_INT8_TOPS = 5000.0     # MI355X dense int8 MFMA, 10^12 int ops/s
+PEAK_FP16_TFLOPS = 2500.0   # MI355X dense fp16 MFMA
+PEAK_FP32_TFLOPS = 157.3    # MI355X fp32 vector
+NS, NF, NT = 512, 64, 8192
+CPU_SAMPLE = (512, 1, 8192)  # one channel of the workload
+CPU_THREADS = 16             # the box's CPU share for one GPU
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(threads: int):
+    """Reference xgpuOmpXengine (oracle/_ref, compiled from the reference's own
+    omp_xengine.c) on `threads` host cores, run in a child process pinned to
+    those cores (the reference sizes its OpenMP team by omp_get_num_procs)."""
+    code = r"""
+import os, sys, time, json
+import numpy as np
+sys.path.insert(0, %r)
+import oracle
+threads = %d
+cpus = sorted(os.sched_getaffinity(0))[:threads]
+os.sched_setaffinity(0, cpus)
+ns, nf, nt = %r
+x = oracle.fill_uniform(nt * nf * ns * 4, 2, -128, 127)
+L = oracle.ref_lib("ns%%d_nf%%d_nt%%d" %% (ns, nf, nt))
+kind = "reference" if L is not None else "port"
+out = np.zeros(2 * oracle.tri_len(ns, nf), dtype=np.float32)
+def run():
+    t0 = time.perf_counter()
+    if L is not None:
+        L.xgpuOmpXengine(out.ctypes.data, x.ctypes.data)
+    else:
+        oracle.xengine_f32(x, ns, nf, nt, len(cpus))
+    return time.perf_counter() - t0
+run()
+ts = sorted(run() for _ in range(3))
+print(json.dumps({"sec": ts[1], "kind": kind, "cores": len(cpus)}))
+""" % (ROOT, threads, CPU_SAMPLE)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
+                       timeout=600)
+    if r.returncode != 0:
+        log(r.stderr[-2000:])
+        return None
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    ns, nf, nt = CPU_SAMPLE
+    cmac = nf * nt * ns * (ns + 1) // 2 * 4
+    return {
+        "value": cmac / res["sec"] / 1e12,
+        "unit": "TCMAC/s",
+        "cores": res["cores"],
+        "kind": res["kind"],
+        "sample": f"xgpuOmpXengine (float build) on NSTATION=512, NFREQUENCY=1, NTIME=8192 "
+                  f"(one channel of the workload, {cmac:.3e} CMAC), median of 3 after a warm-up, "
+                  f"{res['sec']:.2f} s per run",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this X-engine build runs on one GPU; --gpus {args.gpus} is not "
+              "supported (channels shard across GPUs with no communication; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import xgpu_amd as X
+
+    cmac = X.cmac_per_integration(NS, NF, NT)
+    log(f"[bench] int8 MFMA NS={NS} NF={NF} NT={NT}: {cmac:.4e} CMAC per step")
+    buf = X.DeviceBuffer(NT * NF * NS * 4)
+    buf.fill_uniform_int8(2, -128, 127)
+    eng = X.Engine(NS, NF, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+
+    for _ in range(args.warmup):
+        eng.clear()
+        eng.correlate(buf, NT)
+    X.core().xgpuAmdDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.clear()
+        eng.correlate(buf, NT)
+    X.core().xgpuAmdDeviceSynchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    value = cmac * args.steps / wall / 1e12
+    # kernel-only time with HIP events on the engine stream
+    ms_kernel = eng.time_correlate(buf, NT, warmup=1, reps=args.steps)
+    achieved_tops = cmac * 8 / (ms_kernel * 1e-3) / 1e12
+    log(f"[bench] int8: {ms_step:.3f} ms/step wall, {ms_kernel:.3f} ms kernel, {value:.1f} TCMAC/s")
+    eng.close()
+
+    # FP32 plain kernel on the same workload
+    eng32 = X.Engine(NS, NF, compute=X.XGPU_FLOAT32, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    ms32 = eng32.time_correlate(buf, NT, warmup=1, reps=3)
+    eng32.close()
+    t32 = cmac / (ms32 * 1e-3) / 1e12
+    log(f"[bench] fp32 plain: {ms32:.3f} ms, {t32:.2f} TCMAC/s")
+    buf.free()
+
+    # FP16 MFMA path on its own workload (NS=256, NF=128, NT=2048, fp16 input)
+    ns16, nf16, nt16 = 256, 128, 2048
+    b16 = X.DeviceBuffer(nt16 * nf16 * ns16 * 8)
+    b16.fill_uniform_int8(4, 0, 59)  # finite, positive fp16 bit patterns (exponent <= 14); timing only
+    e16 = X.Engine(ns16, nf16, compute=X.XGPU_FLOAT16, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    ms16 = e16.time_correlate(b16, nt16, warmup=1, reps=5)
+    e16.close()
+    b16.free()
+    cmac16 = X.cmac_per_integration(ns16, nf16, nt16)
+    t16 = cmac16 / (ms16 * 1e-3) / 1e12
+    log(f"[bench] fp16 MFMA: {ms16:.3f} ms, {t16:.1f} TCMAC/s")
+
+    cpu = None if args.no_cpu else cpu_baseline(CPU_THREADS)
+
+    line = {
+        "metric": "TCMAC/s & % int8 MFMA peak, 512st x 2pol int8 vs FP32 plain; p99 dump latency",
+        "value": round(value, 3),
+        "unit": "TCMAC/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i8",
+        "data": "synthetic",
+        "config": {
+            "workload": "int8 MFMA X-engine, one integration: NSTATION=512 x 2 pol, "
+                        "NFREQUENCY=64, NTIME=8192, input resident in HBM, output int32 "
+                        "REGISTER_TILE order in HBM",
+            "nstation": NS, "npol": 2, "nfrequency": NF, "ntime": NT,
+            "cmac_per_step": cmac, "input": "uniform int8 [-128,127], seed 2",
+        },
+        "pct_int8_mfma_peak": round(100.0 * cmac * 8 / (ms_step * 1e-3) / 1e12 / PEAK_INT8_TOPS, 2),
+        "fp32_plain": {"tcmac_s": round(t32, 3), "ms": round(ms32, 3),
+                       "pct_fp32_peak": round(100 * t32 * 8 / PEAK_FP32_TFLOPS, 1)},
+        "int8_over_fp32_plain": round(value / t32, 2),
+        "fp16_mfma": {"workload": "NSTATION=256 x 2 pol, NFREQUENCY=128, NTIME=2048, fp16 input",
+                      "tcmac_s": round(t16, 2), "ms": round(ms16, 4),
+                      "pct_fp16_peak": round(100 * t16 * 8 / PEAK_FP16_TFLOPS, 1)},
+        "roofline": {
+            "bound": "mfma",
+            "achieved": round(achieved_tops, 1),
+            "peak": PEAK_INT8_TOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved_tops / PEAK_INT8_TOPS, 4),
+            "traffic": None,
+            "kernel": "xg_xengine_mfma<int8>",
+            "ops_per_cmac": 8,
+            "kernel_ms": round(ms_kernel, 4),
+        },
+        "cpu_baseline": cpu,
+        "serving": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,142 @@
+/*
+ * xgpu_amd.h -- runtime-sized, device-pointer extension of the xgpu.h API.
+ *
+ * xgpu.h fixes every size when the library is built and moves data only from
+ * host buffers.  This header adds (no xGPU counterpart unless cited):
+ *   - engines sized at run time (any even station count, any channel count,
+ *     any block length), one per subarray, independent of each other;
+ *   - correlation of input already resident in device memory (the hot path);
+ *   - the three compute paths: exact int8 MFMA, FP16 MFMA, plain FP32;
+ *   - long integrations past the int32 bound, spilled to int64 / FP64;
+ *   - ragged batches of several engines in one persistent launch;
+ *   - a continuous-batching correlation server (xgpu_serve.h).
+ *
+ * All functions return XGPU_OK (0) or an xgpu.h error code; the message of
+ * the last failure on the calling thread is xgpuAmdLastError().
+ */
+#ifndef XGPU_AMD_H
+#define XGPU_AMD_H
+
+#include <stddef.h>
+#include "xgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Additional error codes (xgpu.h uses 0-5). */
+#define XGPU_INVALID_CONFIG  (6)   /* unsupported sizes or types */
+#define XGPU_NO_DEVICE       (7)   /* no HIP device visible */
+
+/* Dump flags for xgpuAmdDump. */
+#define XGPU_DUMP_DEVICE     (1)   /* dst is device memory (default: host) */
+#define XGPU_DUMP_WIDE       (2)   /* int64 (int8 path) / double (float paths) */
+#define XGPU_DUMP_TRIANGULAR (4)   /* interleaved TRIANGULAR_ORDER instead of the
+                                      engine's matrix_order (device reorder) */
+
+/* Copy directions for xgpuAmdMemcpy. */
+#define XGPU_H2D 1
+#define XGPU_D2H 2
+#define XGPU_D2D 3
+
+typedef struct XGPUAmdConfigStruct {
+  unsigned int npol;          /* must be 2 */
+  unsigned int nstation;      /* even, >= 2; REGISTER_TILE order needs a multiple of 4 */
+  unsigned int nfrequency;    /* >= 1 */
+  unsigned int ntime;         /* samples per xgpuAmdCorrelateHost call */
+  unsigned int ntimepipe;     /* samples per host->device transfer (divides ntime) */
+  unsigned int compute_type;  /* XGPU_INT8, XGPU_FLOAT32 or XGPU_FLOAT16 */
+  unsigned int matrix_order;  /* TRIANGULAR_ORDER, REAL_IMAG_TRIANGULAR_ORDER or
+                                 REGISTER_TILE_TRIANGULAR_ORDER */
+  int device;
+} XGPUAmdConfig;
+
+typedef struct XGPUAmdEngineStruct XGPUAmdEngine;
+
+/* Sizes a config implies, in XGPUInfo terms (like xgpuInfo, reference
+ * src/cuda_xengine.cu:113-142).  Input type: XGPU_INT8 for the int8 and FP32
+ * paths (2-byte complex), XGPU_FLOAT16 for the FP16 path (4-byte complex).
+ * Output element: int32 (int8 path) or float (FP32/FP16 paths), matLength
+ * complex values in split planes (REGISTER_TILE, REAL_IMAG) or interleaved
+ * (TRIANGULAR). */
+int xgpuAmdInfo(const XGPUAmdConfig *cfg, XGPUInfo *info);
+
+/* Create / destroy an engine: device integration buffer, streams, and the
+ * staging buffers of the host-buffer path. */
+int xgpuAmdCreate(XGPUAmdEngine **engine, const XGPUAmdConfig *cfg);
+void xgpuAmdDestroy(XGPUAmdEngine *engine);
+
+/* Start a new integration (reference xgpuClearDeviceIntegrationBuffer,
+ * src/cuda_xengine.cu:334-348).  O(1): the next correlation stores instead of
+ * accumulating. */
+int xgpuAmdClear(XGPUAmdEngine *engine);
+
+/* Correlate ntime samples of device-resident input laid out
+ * [t][f][s][p][re,im] (reference order, src/omp_xengine.c:57-60) into the
+ * integration, asynchronously on the engine's stream.  input_d must be
+ * 8-byte aligned.  Blocks longer than the int32 bound (int8 path: 65535
+ * samples) are split and spilled to int64 transparently. */
+int xgpuAmdCorrelate(XGPUAmdEngine *engine, const void *input_d, unsigned int ntime);
+
+/* Host-buffer path with the semantics of xgpuCudaXengine (reference
+ * src/cuda_xengine.cu:543-677): copies cfg.ntime samples from input_h in
+ * ntimepipe chunks, overlapping transfer and compute, then applies syncOp;
+ * SYNCOP_DUMP writes matLength values to output_h. */
+int xgpuAmdCorrelateHost(XGPUAmdEngine *engine, const void *input_h, int syncOp,
+                         void *output_h);
+
+/* Copy the current integration to dst (flags XGPU_DUMP_*); waits for the
+ * engine's queued work.  Without XGPU_DUMP_WIDE the values are int32 (wrapped
+ * exactly as int32 accumulation would wrap) or float. */
+int xgpuAmdDump(XGPUAmdEngine *engine, void *dst, int flags);
+
+/* Wait for all work queued on the engine. */
+int xgpuAmdSynchronize(XGPUAmdEngine *engine);
+
+/* The engine's HIP stream (hipStream_t). */
+void *xgpuAmdStream(XGPUAmdEngine *engine);
+
+/* Samples integrated since the last clear. */
+unsigned long long xgpuAmdSamples(const XGPUAmdEngine *engine);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *xgpuAmdLastError(void);
+
+/* ---- ragged batching ---------------------------------------------------- */
+typedef struct XGPUAmdBatchItemStruct {
+  XGPUAmdEngine *engine;    /* all items must use one compute type */
+  const void *input_d;      /* device input block */
+  unsigned int ntime;       /* samples in the block (<= int32 bound) */
+} XGPUAmdBatchItem;
+
+/* Correlate up to 16 blocks of different engines (station counts, channel
+ * counts, block lengths) in one persistent launch on `stream` (NULL: the first
+ * item's engine stream).  Each engine's integration advances as if its block
+ * had been correlated alone. */
+int xgpuAmdCorrelateBatch(const XGPUAmdBatchItem *items, int nitems, void *stream);
+
+/* ---- device memory and data helpers (no HIP headers needed) ------------- */
+int xgpuAmdMalloc(void **ptr, size_t bytes);
+int xgpuAmdFree(void *ptr);
+int xgpuAmdMemcpy(void *dst, const void *src, size_t bytes, int kind);
+int xgpuAmdDeviceSynchronize(void);
+/* Fill nbytes of device memory with uniform integers in [lo, hi] from the
+ * counter-based generator: byte i = lo + ((hi32(mix64(mix64(seed) + i)) *
+ * (hi-lo+1)) >> 32), mix64 = SplitMix64 finaliser. */
+int xgpuAmdFillUniformInt8(void *ptr_d, size_t nbytes, unsigned long long seed, int lo, int hi);
+
+/* ---- measurement -------------------------------------------------------- */
+/* Run xgpuAmdCorrelate(engine, input_d, ntime) warmup + reps times on the
+ * engine stream after an xgpuAmdClear each (one kernel per call), timing the
+ * reps with HIP events on that stream; *ms = mean device time per call. */
+int xgpuAmdTimeCorrelate(XGPUAmdEngine *engine, const void *input_d, unsigned int ntime,
+                         int warmup, int reps, float *ms);
+
+/* Kernel tuning of an engine: tile workgroups per CU (1 or 2; 0 = default). */
+int xgpuAmdSetOccupancy(XGPUAmdEngine *engine, int wg_per_cu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XGPU_AMD_H */

@@ -1,0 +1,250 @@
+"""GPU parity: the HIP kernels against the CPU oracle, through the C-ABI.
+
+Integer path: bit-exact against the exact int64 oracle.  FP32 plain path:
+bit-identical to the reference's float CPU X-engine while sums stay below
+2^24 (every partial sum is then an exactly representable integer), which
+covers the cfg1 workload; the reference check metric is also applied with
+1e-6.  FP16 path: against an FP64 oracle of the same fp16 inputs, relative
+RMS error <= 1e-6 and max abs error <= 1e-5 * max|V| (fp32 accumulation of
+exact fp16 products).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+import xgpu_amd as X
+from xgpu_amd import _lib
+from conftest import flush
+
+pytestmark = pytest.mark.gpu
+
+ORDERS = {
+    "regtile": X.REGISTER_TILE_TRIANGULAR_ORDER,
+    "tri": X.TRIANGULAR_ORDER,
+    "realimag": X.REAL_IMAG_TRIANGULAR_ORDER,
+}
+
+
+def to_tri(eng, raw, ns, nf, order):
+    """Engine dump -> interleaved triangular (host reorder by the oracle)."""
+    if order == X.TRIANGULAR_ORDER:
+        return raw
+    return oracle.reorder(order, raw, ns, nf)
+
+
+def run_i8(ns, nf, nt, x, order=X.REGISTER_TILE_TRIANGULAR_ORDER, calls=1):
+    flush(f"[i8] ns={ns} nf={nf} nt={nt} order={order} calls={calls}")
+    eng = X.Engine(ns, nf, nt, compute=X.XGPU_INT8, order=order)
+    buf = X.DeviceBuffer.from_array(x)
+    per = nt // calls
+    step = per * nf * ns * 4
+    for c in range(calls):
+        n = per if c < calls - 1 else nt - per * (calls - 1)
+        eng.correlate(buf.ptr + c * step, n)
+    raw = eng.dump()
+    wide = eng.dump(wide=True)
+    eng.close()
+    buf.free()
+    return raw, wide
+
+
+@pytest.mark.parametrize("ns,nf,nt,order", [
+    (16, 2, 64, "regtile"),
+    (64, 3, 128, "regtile"),
+    (64, 1, 100, "tri"),          # time tail inside a stage
+    (96, 2, 257, "realimag"),     # tile tail (192 inputs), time tail
+    (132, 2, 64, "regtile"),      # 264 inputs: 3 row blocks, partial last
+    (2, 1, 1, "tri"),             # smallest
+    (6, 2, 65, "tri"),            # nstation % 4 == 2
+    (256, 4, 1000, "regtile"),
+])
+def test_i8_matches_exact_oracle(ns, nf, nt, order):
+    x = oracle.fill_uniform(nt * nf * ns * 4, ns * 1000 + nt, -128, 127)
+    raw, wide = run_i8(ns, nf, nt, x, ORDERS[order])
+    ref = oracle.xengine_i64(x, ns, nf, nt)
+    assert np.array_equal(to_tri(None, raw, ns, nf, ORDERS[order]).astype(np.int64), ref)
+    assert np.array_equal(to_tri(None, wide, ns, nf, ORDERS[order]), ref)
+
+
+def test_i8_minus_128_everywhere():
+    ns, nf, nt = 64, 2, 4096
+    x = np.full(nt * nf * ns * 4, -128, dtype=np.int8)
+    raw, wide = run_i8(ns, nf, nt, x, X.TRIANGULAR_ORDER)
+    v = raw.reshape(-1, 2).astype(np.int64)
+    assert np.all(v[:, 0] == nt * 2 * 128 * 128)
+    assert np.all(v[:, 1] == 0)
+    # mixed extremes: x_i = -128 + 127i, x_j = 127 - 128i
+    x = np.zeros((nt, nf, ns, 2, 2), dtype=np.int8)
+    x[:, :, 0::2] = (-128, 127)
+    x[:, :, 1::2] = (127, -128)
+    raw, _ = run_i8(ns, nf, nt, x.reshape(-1), X.TRIANGULAR_ORDER)
+    assert np.array_equal(raw.astype(np.int64), oracle.xengine_i64(x, ns, nf, nt))
+
+
+def test_i8_accumulates_across_calls_and_clear():
+    ns, nf, nt = 64, 2, 640
+    x = oracle.fill_uniform(nt * nf * ns * 4, 77, -128, 127)
+    raw, _ = run_i8(ns, nf, nt, x, X.TRIANGULAR_ORDER, calls=5)
+    assert np.array_equal(raw.astype(np.int64), oracle.xengine_i64(x, ns, nf, nt))
+    flush("[i8] clear semantics")
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    buf = X.DeviceBuffer.from_array(x)
+    assert not eng.dump().any()          # fresh engine dumps zeros
+    eng.correlate(buf, nt)
+    eng.clear()
+    assert not eng.dump().any()          # clear is immediate
+    eng.correlate(buf, nt)
+    ref = oracle.xengine_i64(x, ns, nf, nt)
+    assert np.array_equal(oracle.reorder(3000, eng.dump(), ns, nf).astype(np.int64), ref)
+    eng.close()
+
+
+def test_i8_long_integration_spills_to_int64():
+    """Beyond 65535 samples the int32 buffer spills; the int64 dump is exact
+    and the int32 dump equals the exact value modulo 2^32."""
+    ns, nf, nt = 8, 1, 50000
+    x = np.full(nt * nf * ns * 4, -128, dtype=np.int8)
+    flush("[i8] spill ns=8 nt=3x50000")
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
+    buf = X.DeviceBuffer.from_array(x)
+    for _ in range(3):
+        eng.correlate(buf, nt)
+    assert eng.samples == 3 * nt
+    wide = eng.dump(wide=True).reshape(-1, 2)
+    narrow = eng.dump().reshape(-1, 2)
+    expect = 3 * nt * 2 * 128 * 128
+    assert expect > 2**31
+    assert np.all(wide[:, 0] == expect) and np.all(wide[:, 1] == 0)
+    assert np.all(narrow[:, 0] == np.int64(expect).astype(np.int64) % 2**32 - (2**32 if expect % 2**32 >= 2**31 else 0))
+    eng.close()
+    # random data, one call longer than the bound
+    ns, nf, nt = 4, 2, 70001
+    x = oracle.fill_uniform(nt * nf * ns * 4, 5, -128, 127)
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
+    buf = X.DeviceBuffer.from_array(x)
+    eng.correlate(buf, nt)
+    ref = oracle.xengine_i64(x, ns, nf, nt)
+    assert np.array_equal(eng.dump(wide=True), ref)
+    assert np.array_equal(eng.dump().astype(np.int64), ((ref + 2**31) % 2**32) - 2**31)
+    eng.close()
+
+
+def test_i8_device_reorder_dump():
+    ns, nf, nt = 48, 3, 96
+    x = oracle.fill_uniform(nt * nf * ns * 4, 3, -128, 127)
+    for order in (X.REGISTER_TILE_TRIANGULAR_ORDER, X.REAL_IMAG_TRIANGULAR_ORDER):
+        flush(f"[i8] device reorder order={order}")
+        eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=order)
+        buf = X.DeviceBuffer.from_array(x)
+        eng.correlate(buf, nt)
+        ref = oracle.xengine_i64(x, ns, nf, nt)
+        assert np.array_equal(eng.dump(triangular=True).astype(np.int64), ref)
+        assert np.array_equal(eng.dump(wide=True, triangular=True), ref)
+        eng.close()
+
+
+def test_i8_full_size_cfg2_sampled():
+    """NS=512 x 2 pol, NF=64, NT=8192: exact on sampled baselines of every
+    channel, plus checksums that cover every visibility."""
+    ns, nf, nt = 512, 64, 8192
+    flush("[i8] cfg2 full size")
+    buf = X.DeviceBuffer(nt * nf * ns * 4)
+    buf.fill_uniform_int8(2, -128, 127)
+    x = buf.download(np.int8)
+    assert np.array_equal(x[:1 << 20], oracle.fill_uniform(1 << 20, 2, -128, 127))
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    eng.correlate(buf, nt)
+    tri = oracle.reorder(3000, eng.dump(), ns, nf).reshape(nf, -1, 4, 2).astype(np.int64)
+    rng = np.random.default_rng(42)
+    fij = [(f, i, j) for f in range(nf)
+           for i, j in [(int(a), int(rng.integers(a + 1))) for a in rng.integers(0, ns, 6)]]
+    fij += [(f, ns - 1, ns - 1) for f in range(0, nf, 7)] + [(f, ns - 1, 0) for f in range(0, nf, 5)]
+    ref = oracle.xengine_i64_pairs(x, ns, nf, nt, fij)
+    for n, (f, i, j) in enumerate(fij):
+        assert np.array_equal(tri[f, i * (i + 1) // 2 + j], ref[n]), (f, i, j)
+    # every autocorrelation: sum_t |x|^2 per input, from the input alone
+    a = x.reshape(nt, nf, ns, 2, 2).astype(np.int64)
+    power = (a ** 2).sum(axis=(0, 4))  # [f, s, p]
+    autos = np.array([tri[:, i * (i + 1) // 2 + i, [0, 3], 0] for i in range(ns)])  # [s, f, p]
+    assert np.array_equal(autos.transpose(1, 0, 2), power)
+    # sum over all baselines of the full matrix = |sum_m x_m|^2 per (t, f)
+    eng.close()
+    buf.free()
+
+
+# ------------------------------------------------------------------ FP32 plain
+def test_f32_matches_float_oracle_small():
+    ns, nf, nt = 96, 3, 300
+    x = oracle.fill_uniform(nt * nf * ns * 4, 12, -127, 127)
+    flush("[f32] small")
+    eng = X.Engine(ns, nf, compute=X.XGPU_FLOAT32, order=X.TRIANGULAR_ORDER)
+    buf = X.DeviceBuffer.from_array(x)
+    eng.correlate(buf, nt)
+    got = eng.dump()
+    assert np.array_equal(got, oracle.xengine_f32(x, ns, nf, nt))
+    eng.close()
+
+
+def test_f32_cfg1_through_xgpu_h():
+    """cfg1 through the drop-in: xgpuInit -> xgpuCudaXengine(SYNCOP_DUMP) ->
+    xgpuReorderMatrix, against the reference float CPU X-engine."""
+    ns, nf, nt = 256, 10, 1024
+    flush("[f32] cfg1 via libxgpu.so")
+    lib = X.variant("libxgpu")
+    ctx = _lib.XGPUContext()
+    assert lib.xgpuInit(C.byref(ctx), 0) == 0
+    info = _lib.XGPUInfo()
+    lib.xgpuInfo(C.byref(info))
+    assert (info.nstation, info.nfrequency, info.ntime) == (ns, nf, nt)
+    x = oracle.fill_uniform(nt * nf * ns * 4, 1, -127, 127)
+    C.memmove(ctx.array_h, x.ctypes.data, x.nbytes)
+    assert lib.xgpuCudaXengine(C.byref(ctx), X.SYNCOP_DUMP) == 0
+    mat = np.ctypeslib.as_array((C.c_float * (2 * info.matLength)).from_address(ctx.matrix_h))
+    lib.xgpuReorderMatrix(ctx.matrix_h)
+    got = mat[: 2 * info.triLength].copy()
+    ref = oracle.xengine_f32(x, ns, nf, nt)
+    nbad, maxerr = oracle.check(got, ref, ns, nf, tol=1e-6)
+    assert nbad == 0, maxerr
+    assert np.array_equal(got, ref)
+    lib.xgpuFree(C.byref(ctx))
+    assert ctx.internal is None and ctx.array_h is None
+
+
+# ------------------------------------------------------------------ FP16 MFMA
+def test_f16_against_f64_oracle():
+    ns, nf, nt = 64, 2, 512
+    rng = np.random.default_rng(4)
+    xc = rng.normal(0, 16, (nt, nf, ns, 2, 2))
+    x16 = xc.astype(np.float16)
+    flush("[f16] small")
+    eng = X.Engine(ns, nf, compute=X.XGPU_FLOAT16, order=X.TRIANGULAR_ORDER)
+    buf = X.DeviceBuffer.from_array(x16)
+    eng.correlate(buf, nt)
+    got = eng.dump().astype(np.float64)
+    ref = oracle.xengine_f64_f16(x16, ns, nf, nt)
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 1e-5 * scale
+    assert np.sqrt(np.mean((got - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)) <= 1e-6
+    eng.close()
+
+
+# ------------------------------------------------------------------ batching
+def test_batch_ragged_equals_alone():
+    shapes = [(64, 3, 256), (96, 2, 512), (200, 1, 300), (16, 5, 64)]
+    engines, bufs, refs = [], [], []
+    for k, (ns, nf, nt) in enumerate(shapes):
+        x = oracle.fill_uniform(nt * nf * ns * 4, 50 + k, -128, 127)
+        engines.append(X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER))
+        bufs.append(X.DeviceBuffer.from_array(x))
+        refs.append(oracle.xengine_i64(x, ns, nf, nt))
+    flush("[i8] ragged batch of 4")
+    X.correlate_batch([(e, b, s[2]) for e, b, s in zip(engines, bufs, shapes)])
+    for e, r in zip(engines, refs):
+        assert np.array_equal(e.dump().astype(np.int64), r)
+    # second batch accumulates
+    X.correlate_batch([(e, b, s[2]) for e, b, s in zip(engines, bufs, shapes)])
+    for e, r in zip(engines, refs):
+        assert np.array_equal(e.dump(wide=True), 2 * r)
+        e.close()

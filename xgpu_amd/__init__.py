@@ -1,0 +1,180 @@
+"""xgpu_amd -- MI355X-native X-engine with the xGPU ``xgpu.h`` C API.
+
+The compute runs in hand-written gfx950 HIP kernels behind a C-ABI
+(``include/xgpu.h``, ``include/xgpu_amd.h``).  This package is the Python host
+side: ctypes bindings plus small helpers for device buffers and engines.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (  # noqa: F401  (re-exported constants)
+    REAL_IMAG_TRIANGULAR_ORDER, REGISTER_TILE_TRIANGULAR_ORDER, SYNCOP_DUMP, SYNCOP_NONE,
+    SYNCOP_SYNC_COMPUTE, SYNCOP_SYNC_TRANSFER, TRIANGULAR_ORDER, XGPU_DUMP_DEVICE,
+    XGPU_DUMP_TRIANGULAR, XGPU_DUMP_WIDE, XGPU_FLOAT16, XGPU_FLOAT32, XGPU_INT8, XGPUAmdConfig,
+    XGPUContext, XGPUInfo, XgpuError, XgpuLibraryMissing, check, core, variant,
+)
+
+__all__ = ["Engine", "DeviceBuffer", "info", "cmac_per_integration"]
+
+I32_SAMPLE_BOUND = 65535  # samples an int32 integration holds for any int8 input
+
+
+def cmac_per_integration(nstation: int, nfreq: int, ntime: int, npol: int = 2) -> int:
+    """Complex multiply-accumulates of one integration, NF*NT*NS(NS+1)/2*NPOL^2."""
+    return nfreq * ntime * nstation * (nstation + 1) // 2 * npol * npol
+
+
+def _config(nstation, nfrequency, ntime=0, ntimepipe=0, compute=XGPU_INT8,
+            order=REGISTER_TILE_TRIANGULAR_ORDER, device=0) -> XGPUAmdConfig:
+    c = XGPUAmdConfig()
+    c.npol, c.nstation, c.nfrequency = 2, nstation, nfrequency
+    c.ntime, c.ntimepipe = ntime, (ntimepipe or ntime)
+    c.compute_type, c.matrix_order, c.device = compute, order, device
+    return c
+
+
+def info(nstation, nfrequency, ntime=0, ntimepipe=0, compute=XGPU_INT8,
+         order=REGISTER_TILE_TRIANGULAR_ORDER) -> XGPUInfo:
+    out = XGPUInfo()
+    cfg = _config(nstation, nfrequency, ntime, ntimepipe, compute, order)
+    check(core().xgpuAmdInfo(C.byref(cfg), C.byref(out)))
+    return out
+
+
+class DeviceBuffer:
+    """A block of device memory owned by this object."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(core().xgpuAmdMalloc(C.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        check(core().xgpuAmdMemcpy(self.ptr + offset, a.ctypes.data, a.nbytes, _lib.XGPU_H2D))
+
+    def download(self, dtype, count: int | None = None, offset: int = 0) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = (self.nbytes - offset) // dt.itemsize if count is None else count
+        out = np.empty(n, dtype=dt)
+        check(core().xgpuAmdMemcpy(out.ctypes.data, self.ptr + offset, out.nbytes, _lib.XGPU_D2H))
+        return out
+
+    def fill_uniform_int8(self, seed: int, lo: int, hi: int, nbytes: int | None = None) -> None:
+        n = self.nbytes if nbytes is None else nbytes
+        check(core().xgpuAmdFillUniformInt8(self.ptr, n, seed, lo, hi))
+
+    def free(self) -> None:
+        if self.ptr:
+            core().xgpuAmdFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Engine:
+    """One subarray's X-engine (runtime-sized), see include/xgpu_amd.h."""
+
+    def __init__(self, nstation, nfrequency, ntime=0, ntimepipe=0, compute=XGPU_INT8,
+                 order=REGISTER_TILE_TRIANGULAR_ORDER, device=0):
+        self.cfg = _config(nstation, nfrequency, ntime, ntimepipe, compute, order, device)
+        self.info = XGPUInfo()
+        check(core().xgpuAmdInfo(C.byref(self.cfg), C.byref(self.info)))
+        h = C.c_void_p()
+        check(core().xgpuAmdCreate(C.byref(h), C.byref(self.cfg)))
+        self.handle = h.value
+        self.compute = compute
+        self.nstation, self.nfrequency = nstation, nfrequency
+
+    # sizes
+    @property
+    def mat_len(self) -> int:
+        return int(self.info.matLength)
+
+    @property
+    def tri_len(self) -> int:
+        return int(self.info.triLength)
+
+    @property
+    def out_dtype(self):
+        return np.int32 if self.compute == XGPU_INT8 else np.float32
+
+    @property
+    def wide_dtype(self):
+        return np.int64 if self.compute == XGPU_INT8 else np.float64
+
+    def clear(self) -> None:
+        check(core().xgpuAmdClear(self.handle))
+
+    def correlate(self, input_d, ntime: int) -> None:
+        ptr = input_d.ptr if isinstance(input_d, DeviceBuffer) else int(input_d)
+        check(core().xgpuAmdCorrelate(self.handle, ptr, ntime))
+
+    def correlate_host(self, input_h: np.ndarray, sync_op: int = SYNCOP_DUMP):
+        out = np.zeros(2 * self.mat_len, dtype=self.out_dtype)
+        a = np.ascontiguousarray(input_h)
+        check(core().xgpuAmdCorrelateHost(self.handle, a.ctypes.data, sync_op, out.ctypes.data))
+        return out
+
+    def dump(self, wide: bool = False, triangular: bool = False) -> np.ndarray:
+        """Integration as a flat array: split planes (register tile / real-imag)
+        or interleaved complex (triangular order or ``triangular=True``)."""
+        n = 2 * (self.tri_len if triangular else self.mat_len)
+        out = np.empty(n, dtype=self.wide_dtype if wide else self.out_dtype)
+        flags = (_lib.XGPU_DUMP_WIDE if wide else 0) | (_lib.XGPU_DUMP_TRIANGULAR if triangular else 0)
+        check(core().xgpuAmdDump(self.handle, out.ctypes.data, flags))
+        return out
+
+    def synchronize(self) -> None:
+        check(core().xgpuAmdSynchronize(self.handle))
+
+    @property
+    def samples(self) -> int:
+        return int(core().xgpuAmdSamples(self.handle))
+
+    def time_correlate(self, input_d, ntime: int, warmup: int = 1, reps: int = 5) -> float:
+        ms = C.c_float()
+        ptr = input_d.ptr if isinstance(input_d, DeviceBuffer) else int(input_d)
+        check(core().xgpuAmdTimeCorrelate(self.handle, ptr, ntime, warmup, reps, C.byref(ms)))
+        return float(ms.value)
+
+    def set_occupancy(self, wg_per_cu: int) -> None:
+        check(core().xgpuAmdSetOccupancy(self.handle, wg_per_cu))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            core().xgpuAmdDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def correlate_batch(items, stream=None) -> None:
+    """Ragged batch in one persistent launch: items = [(engine, input_d, ntime)]."""
+    arr = (_lib.XGPUAmdBatchItem * len(items))()
+    for k, (eng, buf, nt) in enumerate(items):
+        arr[k].engine = eng.handle
+        arr[k].input_d = buf.ptr if isinstance(buf, DeviceBuffer) else int(buf)
+        arr[k].ntime = nt
+    check(core().xgpuAmdCorrelateBatch(arr, len(items), stream))

@@ -1,0 +1,502 @@
+/*
+ * engine.c -- runtime-sized X-engine host runtime (plain C).
+ *
+ * Owns one subarray's device integration buffer and drives the gfx950 kernels
+ * through the thin C-ABI layer (xg_hip.h).  It is the MI355X counterpart of the
+ * internal context and pipeline of the reference host code
+ * (reference src/cuda_xengine.cu:30-65 context, :180-331 init, :334-348 clear,
+ * :543-677 the correlation pipeline and dump), with these differences:
+ *   - sizes are run-time values; several engines coexist (no globals);
+ *   - one kernel launch integrates a whole device-resident block; the host-
+ *     buffer path launches one kernel per transferred chunk, like the
+ *     reference, overlapping H2D copies with compute on two streams;
+ *   - clear is O(1): the next kernel stores instead of accumulating;
+ *   - int32 (or f32) integrations are spilled into an int64 (or f64) buffer
+ *     before they could overflow, so integrations of any length are exact.
+ */
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "engine_internal.h"
+#include "xg_hip.h"
+#include "xg_launch.h"
+#include "xgpu_amd.h"
+
+static _Thread_local char g_msg[512];
+
+int xg_fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_msg, sizeof g_msg, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+static int hipfail(const char *what) {
+  return xg_fail(XGPU_CUDA_ERROR, "%s: %s", what, xgh_last_error());
+}
+
+#define HIPCALL(expr, what)            \
+  do {                                 \
+    if ((expr) != 0) return hipfail(what); \
+  } while (0)
+
+const char *xgpuAmdLastError(void) { return g_msg; }
+
+/* Largest number of samples an int32 integration can hold for any int8
+ * input: |ar*br + ai*bi| <= 2*128*128 per sample, and both accumulators of
+ * the kernel (re and the reordered-plane im) stay within 2^31-1 for
+ * 65535*32768. */
+#define XG_I32_SAMPLE_BOUND 65535u
+/* Float paths: spill the f32 integration into f64 every 2^16 samples, keeping
+ * f32 partial sums short. */
+#define XG_F32_SPILL_SAMPLES 65536u
+
+int xgpuAmdInfo(const XGPUAmdConfig *c, XGPUInfo *info) {
+  if (!c || !info) return xg_fail(XGPU_INVALID_CONFIG, "NULL argument");
+  if (c->npol != 2) return xg_fail(XGPU_INVALID_CONFIG, "npol must be 2 (got %u)", c->npol);
+  if (c->nstation < 2 || (c->nstation & 1))
+    return xg_fail(XGPU_INVALID_CONFIG, "nstation must be even and >= 2 (got %u)", c->nstation);
+  if (c->nstation > 2 * 255 * XG_TILE / 2)
+    return xg_fail(XGPU_INVALID_CONFIG, "nstation too large (%u)", c->nstation);
+  if (c->nfrequency < 1) return xg_fail(XGPU_INVALID_CONFIG, "nfrequency must be >= 1");
+  if (c->compute_type != XGPU_INT8 && c->compute_type != XGPU_FLOAT32 &&
+      c->compute_type != XGPU_FLOAT16)
+    return xg_fail(XGPU_INVALID_CONFIG, "unknown compute_type %u", c->compute_type);
+  if (c->matrix_order != TRIANGULAR_ORDER && c->matrix_order != REAL_IMAG_TRIANGULAR_ORDER &&
+      c->matrix_order != REGISTER_TILE_TRIANGULAR_ORDER)
+    return xg_fail(XGPU_INVALID_CONFIG, "unknown matrix_order %u", c->matrix_order);
+  if (c->matrix_order == REGISTER_TILE_TRIANGULAR_ORDER && (c->nstation % 4))
+    return xg_fail(XGPU_INVALID_CONFIG,
+                   "REGISTER_TILE order needs nstation %% 4 == 0 (got %u)", c->nstation);
+  if (c->ntimepipe && c->ntime % c->ntimepipe)
+    return xg_fail(XGPU_INVALID_CONFIG, "ntime (%u) must be a multiple of ntimepipe (%u)",
+                   c->ntime, c->ntimepipe);
+  const unsigned long long ns = c->nstation, nf = c->nfrequency;
+  memset(info, 0, sizeof *info);
+  info->npol = 2;
+  info->nstation = c->nstation;
+  info->nbaseline = (unsigned)((ns + 1) * (ns / 2));
+  info->nfrequency = c->nfrequency;
+  info->ntime = c->ntime;
+  info->ntimepipe = c->ntimepipe ? c->ntimepipe : c->ntime;
+  info->input_type = (c->compute_type == XGPU_FLOAT16) ? XGPU_FLOAT16 : XGPU_INT8;
+  info->compute_type = c->compute_type;
+  info->vecLength = nf * c->ntime * ns * 2;
+  info->vecLengthPipe = nf * info->ntimepipe * ns * 2;
+  info->triLength = nf * ((ns + 1) * (ns / 2)) * 4;
+  if (c->matrix_order == REGISTER_TILE_TRIANGULAR_ORDER)
+    info->matLength = nf * ((ns / 2 + 1) * (ns / 4)) * 4 * 4;
+  else
+    info->matLength = info->triLength;
+  info->matrix_order = c->matrix_order;
+  info->shared_atomic_size = 4;
+  info->complex_block_size = 1;
+  return XGPU_OK;
+}
+
+static void job_fill(XgJob *J, const XGPUAmdEngine *e, const void *input, unsigned ntime,
+                     int accumulate, int chan_base) {
+  memset(J, 0, sizeof *J);
+  J->input = input;
+  J->out = e->integ;
+  J->tstride = e->tstride;
+  J->plane = (long long)e->info.matLength;
+  J->nstation = (int)e->cfg.nstation;
+  J->nfreq = (int)e->cfg.nfrequency;
+  J->ntime = (int)ntime;
+  J->nblk = (int)((2 * e->cfg.nstation + XG_TILE - 1) / XG_TILE);
+  J->tiles_per_chan = J->nblk * (J->nblk + 1) / 2;
+  J->order = (int)e->cfg.matrix_order;
+  J->accumulate = accumulate;
+  J->out_kind = e->out_kind;
+  J->chan_base = chan_base;
+}
+
+void xg_plan_launch(XgLaunch *L, int cus, int wg_per_cu) {
+  int maxx = 0;
+  for (int x = 0; x < XG_NXCD; ++x) {
+    L->xcd_start[x][0] = 0;
+    for (int j = 0; j < L->njobs; ++j) {
+      const XgJob *J = &L->jobs[j];
+      const int f0 = (((x - J->chan_base) % XG_NXCD) + XG_NXCD) % XG_NXCD;
+      const int nch = (f0 < J->nfreq) ? (J->nfreq - 1 - f0) / XG_NXCD + 1 : 0;
+      L->xcd_start[x][j + 1] = L->xcd_start[x][j] + nch * J->tiles_per_chan;
+    }
+    for (int j = L->njobs + 1; j <= XG_MAX_JOBS; ++j) L->xcd_start[x][j] = L->xcd_start[x][L->njobs];
+    if (L->xcd_start[x][L->njobs] > maxx) maxx = L->xcd_start[x][L->njobs];
+  }
+  int slots = (cus / XG_NXCD) * wg_per_cu;
+  if (slots < 1) slots = 1;
+  L->wg_per_xcd = maxx < slots ? maxx : slots;
+}
+
+int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
+  if (!out) return xg_fail(XGPU_INVALID_CONFIG, "NULL engine pointer");
+  *out = NULL;
+  XGPUInfo info;
+  int rc = xgpuAmdInfo(cfg, &info);
+  if (rc) return rc;
+  int ndev = 0;
+  if (xgh_device_count(&ndev) != 0 || ndev == 0)
+    return xg_fail(XGPU_NO_DEVICE, "no HIP device found (%s)", xgh_last_error());
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return xg_fail(XGPU_INVALID_CONFIG, "device %d out of range (%d devices)", cfg->device, ndev);
+  XGPUAmdEngine *e = (XGPUAmdEngine *)calloc(1, sizeof *e);
+  if (!e) return xg_fail(XGPU_OUT_OF_MEMORY, "out of host memory");
+  e->cfg = *cfg;
+  e->info = info;
+  e->family = (cfg->compute_type == XGPU_INT8)      ? XG_KERNEL_I8_MFMA
+              : (cfg->compute_type == XGPU_FLOAT16) ? XG_KERNEL_F16_MFMA
+                                                    : XG_KERNEL_F32_PLAIN;
+  e->out_kind = (e->family == XG_KERNEL_I8_MFMA) ? XG_OUT_I32 : XG_OUT_F32;
+  e->wide_kind = (e->family == XG_KERNEL_I8_MFMA) ? XG_OUT_I64 : XG_OUT_F64;
+  e->bound = (e->family == XG_KERNEL_I8_MFMA) ? XG_I32_SAMPLE_BOUND : XG_F32_SPILL_SAMPLES;
+  e->cplx_bytes = (e->family == XG_KERNEL_F16_MFMA) ? 4 : 2;
+  e->tstride = (long long)cfg->nfrequency * cfg->nstation * 2 * (long long)e->cplx_bytes;
+  e->integ_bytes = (size_t)info.matLength * 2 * 4;
+  e->cleared = 1;
+  e->wg_per_cu = 2;
+  if ((rc = xgh_set_device(cfg->device)) != 0) goto hip_error;
+  if ((rc = xgh_device_cu_count(cfg->device, &e->cus)) != 0) goto hip_error;
+  if ((rc = xgh_malloc(&e->integ, e->integ_bytes)) != 0) goto hip_error;
+  if ((rc = xgh_stream_create(&e->stream)) != 0) goto hip_error;
+  if ((rc = xgh_stream_create(&e->copy_stream)) != 0) goto hip_error;
+  for (int i = 0; i < 2; ++i)
+    if ((rc = xgh_event_create(&e->ev_kernel[i], 0)) != 0) goto hip_error;
+  if ((rc = xgh_event_create(&e->ev_t0, 1)) != 0) goto hip_error;
+  if ((rc = xgh_event_create(&e->ev_t1, 1)) != 0) goto hip_error;
+  if ((rc = xgh_event_create(&e->ev_aux, 0)) != 0) goto hip_error;
+  *out = e;
+  return XGPU_OK;
+hip_error:
+  rc = hipfail("xgpuAmdCreate");
+  xgpuAmdDestroy(e);
+  return rc;
+}
+
+void xgpuAmdDestroy(XGPUAmdEngine *e) {
+  if (!e) return;
+  xgh_set_device(e->cfg.device);
+  if (e->stream) xgh_stream_sync(e->stream);
+  if (e->copy_stream) xgh_stream_sync(e->copy_stream);
+  for (int i = 0; i < 2; ++i) {
+    xgh_free(e->dev_in[i]);
+    xgh_event_destroy(e->ev_kernel[i]);
+  }
+  for (int p = 0; p < e->npipe_events; ++p) xgh_event_destroy(e->ev_chunk[p]);
+  free(e->ev_chunk);
+  xgh_event_destroy(e->ev_t0);
+  xgh_event_destroy(e->ev_t1);
+  xgh_event_destroy(e->ev_aux);
+  xgh_free(e->integ);
+  xgh_free(e->wide);
+  xgh_free(e->scratch);
+  xgh_free(e->scratch2);
+  xgh_stream_destroy(e->stream);
+  xgh_stream_destroy(e->copy_stream);
+  free(e);
+}
+
+int xgpuAmdClear(XGPUAmdEngine *e) {
+  if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
+  e->cleared = 1;
+  e->spilled = 0;
+  e->samples = 0;
+  e->samples_in_integ = 0;
+  return XGPU_OK;
+}
+
+unsigned long long xgpuAmdSamples(const XGPUAmdEngine *e) { return e ? e->samples : 0; }
+void *xgpuAmdStream(XGPUAmdEngine *e) { return e ? e->stream : NULL; }
+
+int xgpuAmdSetOccupancy(XGPUAmdEngine *e, int wg_per_cu) {
+  if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
+  if (wg_per_cu < 0 || wg_per_cu > 2) return xg_fail(XGPU_INVALID_CONFIG, "wg_per_cu must be 0..2");
+  e->wg_per_cu = wg_per_cu ? wg_per_cu : 2;
+  return XGPU_OK;
+}
+
+/* wide += integ; integ = 0 (the int32 / f32 part restarts from zero). */
+int xg_spill(XGPUAmdEngine *e) {
+  const size_t n = e->info.matLength * 2;
+  if (!e->wide) HIPCALL(xgh_malloc(&e->wide, n * 8), "spill buffer");
+  if (!e->spilled) HIPCALL(xgh_launch_fill_zero(e->wide, n * 8, e->stream), "spill clear");
+  if (e->cleared) {
+    /* nothing integrated since the clear */
+  } else {
+    HIPCALL(xgh_launch_spill(e->wide_kind, e->wide, e->integ, n, e->stream), "spill");
+  }
+  e->spilled = 1;
+  e->cleared = 1; /* integ now logically zero: next kernel stores */
+  e->samples_in_integ = 0;
+  return XGPU_OK;
+}
+
+int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime) {
+  XgLaunch L;
+  memset(&L, 0, sizeof L);
+  L.njobs = 1;
+  job_fill(&L.jobs[0], e, input_d, ntime, !e->cleared, 0);
+  xg_plan_launch(&L, e->cus, e->wg_per_cu);
+  HIPCALL(xgh_launch_xengine(e->family, &L, e->stream), "xengine kernel");
+  e->cleared = 0;
+  e->samples_in_integ += ntime;
+  e->samples += ntime;
+  return XGPU_OK;
+}
+
+int xgpuAmdCorrelate(XGPUAmdEngine *e, const void *input_d, unsigned int ntime) {
+  if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
+  if (!input_d && ntime) return xg_fail(XGPU_INVALID_CONFIG, "NULL input");
+  if (((uintptr_t)input_d) & 7) return xg_fail(XGPU_INVALID_CONFIG, "input must be 8-byte aligned");
+  HIPCALL(xgh_set_device(e->cfg.device), "set device");
+  const char *p = (const char *)input_d;
+  while (ntime > 0) {
+    unsigned room = e->bound - e->samples_in_integ;
+    if (room == 0) {
+      int rc = xg_spill(e);
+      if (rc) return rc;
+      continue;
+    }
+    unsigned n = ntime < room ? ntime : room;
+    int rc = xg_launch_single(e, p, n);
+    if (rc) return rc;
+    p += (size_t)n * (size_t)e->tstride;
+    ntime -= n;
+  }
+  return XGPU_OK;
+}
+
+static int ensure_scratch(XGPUAmdEngine *e, void **buf, size_t bytes) {
+  if (*buf) return XGPU_OK;
+  HIPCALL(xgh_malloc(buf, bytes), "dump scratch");
+  return XGPU_OK;
+}
+
+int xgpuAmdDump(XGPUAmdEngine *e, void *dst, int flags) {
+  if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
+  if (!dst) return xg_fail(XGPU_INVALID_CONFIG, "NULL dump destination");
+  HIPCALL(xgh_set_device(e->cfg.device), "set device");
+  const int wide = (flags & XGPU_DUMP_WIDE) != 0;
+  const int on_dev = (flags & XGPU_DUMP_DEVICE) != 0;
+  const int tri = (flags & XGPU_DUMP_TRIANGULAR) && e->cfg.matrix_order != TRIANGULAR_ORDER;
+  const size_t n = e->info.matLength * 2; /* scalars in the integration */
+  const size_t esz = wide ? 8 : 4;
+  const void *val = NULL;
+  int rc;
+  /* scratch holds up to n wide scalars */
+  if (e->cleared && !e->spilled) {
+    if ((rc = ensure_scratch(e, &e->scratch, n * 8))) return rc;
+    HIPCALL(xgh_launch_fill_zero(e->scratch, n * esz, e->stream), "dump zero");
+    val = e->scratch;
+  } else if (!wide && !e->spilled) {
+    val = e->integ;
+  } else {
+    if ((rc = ensure_scratch(e, &e->scratch, n * 8))) return rc;
+    if (e->spilled && !e->cleared) {
+      HIPCALL(xgh_launch_combine(e->wide_kind, !wide, e->scratch, e->wide, e->integ, n, e->stream),
+              "dump combine");
+    } else if (e->spilled) { /* only the spilled part is live */
+      if (wide)
+        HIPCALL(xgh_memcpy_d2d_async(e->scratch, e->wide, n * 8, e->stream), "dump copy");
+      else {
+        HIPCALL(xgh_launch_fill_zero(e->scratch, n * 4, e->stream), "dump zero");
+        /* scratch(narrow) = wide + 0 */
+        if ((rc = ensure_scratch(e, &e->scratch2, n * 8))) return rc;
+        HIPCALL(xgh_launch_fill_zero(e->scratch2, n * 4, e->stream), "dump zero");
+        HIPCALL(xgh_launch_combine(e->wide_kind, 1, e->scratch, e->wide, e->scratch2, n, e->stream),
+                "dump combine");
+      }
+    } else { /* wide requested, nothing spilled: widen integ */
+      HIPCALL(xgh_launch_fill_zero(e->scratch, n * 8, e->stream), "dump zero");
+      HIPCALL(xgh_launch_spill(e->wide_kind, e->scratch, e->integ, n, e->stream), "dump widen");
+    }
+    val = e->scratch;
+  }
+  if (tri) {
+    if ((rc = ensure_scratch(e, &e->scratch2, n * 8))) return rc;
+    HIPCALL(xgh_launch_reorder((int)e->cfg.matrix_order, (int)esz, e->scratch2, val,
+                               (int)e->cfg.nstation, (int)e->cfg.nfrequency,
+                               (long long)e->info.matLength, e->stream),
+            "dump reorder");
+    val = e->scratch2;
+  }
+  const size_t bytes = (tri ? e->info.triLength * 2 : n) * esz;
+  if (on_dev) {
+    HIPCALL(xgh_memcpy_d2d_async(dst, val, bytes, e->stream), "dump D2D");
+    HIPCALL(xgh_stream_sync(e->stream), "dump sync");
+  } else {
+    HIPCALL(xgh_memcpy_d2h_async(dst, val, bytes, e->stream), "dump D2H");
+    HIPCALL(xgh_stream_sync(e->stream), "dump sync");
+  }
+  return XGPU_OK;
+}
+
+int xgpuAmdSynchronize(XGPUAmdEngine *e) {
+  if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
+  HIPCALL(xgh_set_device(e->cfg.device), "set device");
+  HIPCALL(xgh_stream_sync(e->copy_stream), "sync copy stream");
+  HIPCALL(xgh_stream_sync(e->stream), "sync compute stream");
+  return XGPU_OK;
+}
+
+/* Host-buffer path (reference src/cuda_xengine.cu:543-677).  Two device
+ * input buffers alternate between calls; within a call, chunk p's transfer on
+ * the copy stream is followed by its kernel on the compute stream, so the
+ * transfer of chunk p+1 overlaps the kernel of chunk p. */
+int xgpuAmdCorrelateHost(XGPUAmdEngine *e, const void *input_h, int syncOp, void *output_h) {
+  if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
+  if (!input_h) return xg_fail(XGPU_HOST_BUFFER_NOT_SET, "NULL host input");
+  if (syncOp == SYNCOP_DUMP && !output_h) return xg_fail(XGPU_HOST_BUFFER_NOT_SET, "NULL host output");
+  if (e->cfg.ntime == 0) return XGPU_OK;
+  HIPCALL(xgh_set_device(e->cfg.device), "set device");
+  const unsigned ntp = e->info.ntimepipe;
+  const int npipe = (int)(e->cfg.ntime / ntp);
+  const size_t chunk = (size_t)ntp * (size_t)e->tstride;
+  const int buf = e->parity;
+  e->parity ^= 1;
+  if (!e->dev_in[buf]) HIPCALL(xgh_malloc(&e->dev_in[buf], chunk * npipe), "input buffer");
+  if (e->npipe_events < npipe) {
+    void **ev = (void **)realloc(e->ev_chunk, sizeof(void *) * npipe);
+    if (!ev) return xg_fail(XGPU_OUT_OF_MEMORY, "out of host memory");
+    e->ev_chunk = ev;
+    for (int p = e->npipe_events; p < npipe; ++p) {
+      HIPCALL(xgh_event_create(&e->ev_chunk[p], 0), "event");
+      e->npipe_events = p + 1;
+    }
+  }
+  /* the buffer is free once the kernels of the call that used it finished */
+  if (e->ev_kernel_used[buf]) HIPCALL(xgh_stream_wait_event(e->copy_stream, e->ev_kernel[buf]), "wait");
+  char *dbase = (char *)e->dev_in[buf];
+  const char *hbase = (const char *)input_h;
+  for (int p = 0; p < npipe; ++p) {
+    HIPCALL(xgh_memcpy_h2d_async(dbase + p * chunk, hbase + p * chunk, chunk, e->copy_stream), "H2D");
+    HIPCALL(xgh_event_record(e->ev_chunk[p], e->copy_stream), "record");
+    HIPCALL(xgh_stream_wait_event(e->stream, e->ev_chunk[p]), "wait");
+    int rc = xgpuAmdCorrelate(e, dbase + p * chunk, ntp);
+    if (rc) return rc;
+  }
+  HIPCALL(xgh_event_record(e->ev_kernel[buf], e->stream), "record");
+  e->ev_kernel_used[buf] = 1;
+  switch (syncOp) {
+    case SYNCOP_DUMP:
+      return xgpuAmdDump(e, output_h, 0);
+    case SYNCOP_SYNC_COMPUTE:
+      HIPCALL(xgh_stream_sync(e->stream), "sync compute");
+      break;
+    case SYNCOP_SYNC_TRANSFER:
+      HIPCALL(xgh_stream_sync(e->copy_stream), "sync transfer");
+      break;
+    default:
+      break;
+  }
+  return XGPU_OK;
+}
+
+int xgpuAmdCorrelateBatch(const XGPUAmdBatchItem *items, int nitems, void *stream) {
+  if (!items || nitems < 1) return xg_fail(XGPU_INVALID_CONFIG, "empty batch");
+  if (nitems > XG_MAX_JOBS) return xg_fail(XGPU_INVALID_CONFIG, "batch larger than %d", XG_MAX_JOBS);
+  XGPUAmdEngine *e0 = items[0].engine;
+  if (!e0) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine in batch");
+  for (int i = 0; i < nitems; ++i) {
+    XGPUAmdEngine *e = items[i].engine;
+    if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine in batch");
+    if (e->family != e0->family || e->cfg.device != e0->cfg.device)
+      return xg_fail(XGPU_INVALID_CONFIG, "batch items must share compute type and device");
+    if (items[i].ntime > e->bound)
+      return xg_fail(XGPU_INVALID_CONFIG, "batch block of %u samples exceeds the bound %u",
+                     items[i].ntime, e->bound);
+    if (((uintptr_t)items[i].input_d) & 7)
+      return xg_fail(XGPU_INVALID_CONFIG, "input must be 8-byte aligned");
+    for (int k = 0; k < i; ++k)
+      if (items[k].engine == e) return xg_fail(XGPU_INVALID_CONFIG, "engine twice in one batch");
+  }
+  HIPCALL(xgh_set_device(e0->cfg.device), "set device");
+  void *s = stream ? stream : e0->stream;
+  XgLaunch L;
+  memset(&L, 0, sizeof L);
+  int chan = 0;
+  for (int i = 0; i < nitems; ++i) {
+    XGPUAmdEngine *e = items[i].engine;
+    if (e->samples_in_integ + items[i].ntime > e->bound) {
+      int rc = xg_spill(e);
+      if (rc) return rc;
+    }
+    if (e->stream != s) { /* order after the engine's own queued work */
+      HIPCALL(xgh_event_record(e->ev_aux, e->stream), "record");
+      HIPCALL(xgh_stream_wait_event(s, e->ev_aux), "wait");
+    }
+    job_fill(&L.jobs[i], e, items[i].input_d, items[i].ntime, !e->cleared, chan);
+    chan += (int)e->cfg.nfrequency;
+  }
+  L.njobs = nitems;
+  xg_plan_launch(&L, e0->cus, e0->wg_per_cu);
+  HIPCALL(xgh_launch_xengine(e0->family, &L, s), "batched xengine kernel");
+  HIPCALL(xgh_event_record(e0->ev_aux, s), "record");
+  for (int i = 0; i < nitems; ++i) {
+    XGPUAmdEngine *e = items[i].engine;
+    if (e->stream != s) HIPCALL(xgh_stream_wait_event(e->stream, e0->ev_aux), "wait");
+    e->cleared = 0;
+    e->samples_in_integ += items[i].ntime;
+    e->samples += items[i].ntime;
+  }
+  return XGPU_OK;
+}
+
+int xgpuAmdTimeCorrelate(XGPUAmdEngine *e, const void *input_d, unsigned int ntime, int warmup,
+                         int reps, float *ms) {
+  if (!e || !ms || reps < 1) return xg_fail(XGPU_INVALID_CONFIG, "bad arguments");
+  int rc;
+  for (int i = 0; i < warmup; ++i) {
+    xgpuAmdClear(e);
+    if ((rc = xgpuAmdCorrelate(e, input_d, ntime))) return rc;
+  }
+  HIPCALL(xgh_event_record(e->ev_t0, e->stream), "record");
+  for (int i = 0; i < reps; ++i) {
+    xgpuAmdClear(e);
+    if ((rc = xgpuAmdCorrelate(e, input_d, ntime))) return rc;
+  }
+  HIPCALL(xgh_event_record(e->ev_t1, e->stream), "record");
+  HIPCALL(xgh_event_sync(e->ev_t1), "event sync");
+  float total = 0.f;
+  HIPCALL(xgh_event_elapsed_ms(&total, e->ev_t0, e->ev_t1), "elapsed");
+  *ms = total / (float)reps;
+  return XGPU_OK;
+}
+
+/* ---- device memory helpers ---------------------------------------------- */
+int xgpuAmdMalloc(void **ptr, size_t bytes) {
+  if (!ptr) return xg_fail(XGPU_INVALID_CONFIG, "NULL pointer");
+  HIPCALL(xgh_malloc(ptr, bytes), "xgpuAmdMalloc");
+  return XGPU_OK;
+}
+int xgpuAmdFree(void *ptr) {
+  HIPCALL(xgh_free(ptr), "xgpuAmdFree");
+  return XGPU_OK;
+}
+int xgpuAmdMemcpy(void *dst, const void *src, size_t bytes, int kind) {
+  switch (kind) {
+    case XGPU_H2D: HIPCALL(xgh_memcpy_h2d(dst, src, bytes), "xgpuAmdMemcpy"); break;
+    case XGPU_D2H: HIPCALL(xgh_memcpy_d2h(dst, src, bytes), "xgpuAmdMemcpy"); break;
+    case XGPU_D2D:
+      HIPCALL(xgh_memcpy_d2d_async(dst, src, bytes, NULL), "xgpuAmdMemcpy");
+      HIPCALL(xgh_device_sync(), "xgpuAmdMemcpy");
+      break;
+    default: return xg_fail(XGPU_INVALID_CONFIG, "bad copy kind %d", kind);
+  }
+  return XGPU_OK;
+}
+int xgpuAmdDeviceSynchronize(void) {
+  HIPCALL(xgh_device_sync(), "xgpuAmdDeviceSynchronize");
+  return XGPU_OK;
+}
+int xgpuAmdFillUniformInt8(void *ptr_d, size_t nbytes, unsigned long long seed, int lo, int hi) {
+  if (hi < lo || lo < -128 || hi > 127) return xg_fail(XGPU_INVALID_CONFIG, "bad range");
+  HIPCALL(xgh_launch_fill_uniform_i8(ptr_d, nbytes, seed, lo, hi, NULL), "fill");
+  HIPCALL(xgh_device_sync(), "fill sync");
+  return XGPU_OK;
+}

@@ -1,0 +1,50 @@
+/*
+ * engine_internal.h -- private engine state shared by the C host runtime
+ * files (engine.c, serve.c).  Not installed.
+ */
+#ifndef XG_ENGINE_INTERNAL_H
+#define XG_ENGINE_INTERNAL_H
+
+#include <stddef.h>
+#include "xg_launch.h"
+#include "xgpu_amd.h"
+
+struct XGPUAmdEngineStruct {
+  XGPUAmdConfig cfg;
+  XGPUInfo info;
+  int family;        /* XG_KERNEL_* */
+  int out_kind;      /* XG_OUT_I32 or XG_OUT_F32 */
+  int wide_kind;     /* XG_OUT_I64 or XG_OUT_F64 */
+  unsigned bound;    /* samples an integration buffer takes before a spill */
+  size_t cplx_bytes; /* 2 (int8) or 4 (fp16) */
+  long long tstride; /* input bytes per time sample */
+  size_t integ_bytes;
+  int cus;
+  int wg_per_cu;
+
+  void *integ;       /* int32 / f32 integration, 2*matLength scalars */
+  void *wide;        /* int64 / f64 spill, allocated on first spill */
+  void *scratch;     /* dump staging */
+  void *scratch2;
+
+  int cleared;       /* integ is logically zero: next kernel stores */
+  int spilled;       /* wide holds part of the integration */
+  unsigned samples_in_integ;
+  unsigned long long samples;
+
+  void *stream;      /* compute */
+  void *copy_stream; /* host -> device transfers */
+  void *ev_kernel[2];
+  int ev_kernel_used[2];
+  void **ev_chunk;
+  int npipe_events;
+  void *ev_t0, *ev_t1, *ev_aux;
+  void *dev_in[2];
+  int parity;
+};
+
+int xg_fail(int code, const char *fmt, ...);
+void xg_plan_launch(XgLaunch *L, int cus, int wg_per_cu);
+int xg_spill(XGPUAmdEngine *e);
+
+#endif /* XG_ENGINE_INTERNAL_H */

@@ -1,0 +1,735 @@
+// xg_kernels.hip -- gfx950 X-engine kernels.
+//
+// Replaces the CUDA kernel shared2x2 and its LDS loaders
+// (reference src/kernel.cuh:238-591, src/shared_transfer_4.cuh,
+// src/shared_transfer_4_dp4a.cuh, src/shared_transfer_8.cuh).  The
+// arithmetic is the same per-channel Hermitian rank-K update
+//   V[f][i][j][p1][p2] = sum_t x[t][f][i][p1] * conj(x[t][f][j][p2]),  i >= j
+// (reference src/omp_xengine.c:24-28,55-84), re = ar*br + ai*bi,
+// im = ai*br - ar*bi, but the implementation is MI355X-native:
+//
+//  * One persistent launch covers a list of jobs (subarray blocks) and, for
+//    each, every channel's lower-triangle of 128x128-input tiles.  Workgroups
+//    stay resident and walk their XCD's share of the tile list.
+//  * Accumulators stay in registers for the whole block of ntime samples
+//    (the reference re-reads and re-writes global memory every NTIME_PIPE).
+//  * Input is time-major ([t][f][s][p][c]); the corner turn to the
+//    time-contiguous MFMA operand layout is done in registers with v_perm
+//    while staging global -> LDS, never as a separate HBM pass.
+//  * LDS rows are 128 bytes (one stage of one input), with a 16-byte-chunk
+//    XOR swizzle that makes both the staging writes and the operand reads
+//    bank-conflict free.
+//
+// Integer path (int8 MFMA, int32 accumulation, exact):
+//   Re:  acc_re += A  . B   with A = (ar,ai) pairs of the row input,
+//                                 B = (br,bi) pairs of the column input
+//   Im:  acc_im += A' . B   with A' = (ai, ~ar)   (planes reordered)
+//   ~ar = -ar - 1 for every int8 including -128, so
+//   A'.B = sum(ai*br - ar*bi) - sum(bi), and the exact conjugate term is
+//   acc_im + sum_t bi(column).  No int8 value is ever negated, so -128 is
+//   exact.  The column sums come from v_dot4 on the B operand.
+//   |per-sample term| <= 2*128*128, so int32 cannot overflow for blocks of
+//   up to 65535 samples; the host splits longer blocks.
+//
+// FP16 path: A' = (ai, -ar) by a sign flip (exact in floating point),
+// FP16 MFMA with FP32 accumulation.
+//
+// FP32 plain path: SIMT FP32 FMA, exact int8 -> float conversion, 8x8
+// complex register tile per thread.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "xg_hip.h"
+#include "xg_launch.h"
+
+int xgh_report(hipError_t e, const char *what);
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+
+#define XG_ROWB 128                         // LDS bytes per input per stage
+#define XG_SIDE_BYTES (XG_TILE * XG_ROWB)   // 16 KiB: one side of a tile
+#define XG_STAGE_BYTES (2 * XG_SIDE_BYTES)  // rows + columns
+#define XG_LDS_BYTES (2 * XG_STAGE_BYTES)   // double buffered: 64 KiB
+
+// ---------------------------------------------------------------------------
+// Tile schedule
+// ---------------------------------------------------------------------------
+struct TileRef {
+  int job, f, rb, cb;
+};
+
+// k-th tile of XCD `xcd`'s share (see xg_launch.h).  Returns false past the end.
+__device__ __forceinline__ bool xg_tile(const XgLaunch &L, int xcd, int k, TileRef &t) {
+  const int *start = L.xcd_start[xcd];
+  if (k >= start[L.njobs]) return false;
+  int j = 0;
+  while (k >= start[j + 1]) ++j;
+  const XgJob &J = L.jobs[j];
+  const int local = k - start[j];
+  const int ci = local / J.tiles_per_chan;
+  const int ti = local - ci * J.tiles_per_chan;
+  const int f0 = (((xcd - J.chan_base) % XG_NXCD) + XG_NXCD) % XG_NXCD;
+  int rb = (int)((sqrtf(8.0f * (float)ti + 1.0f) - 1.0f) * 0.5f);
+  while (rb * (rb + 1) / 2 > ti) --rb;
+  while ((rb + 1) * (rb + 2) / 2 <= ti) ++rb;
+  t.job = j;
+  t.f = f0 + XG_NXCD * ci;
+  t.rb = rb;
+  t.cb = ti - rb * (rb + 1) / 2;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// LDS addressing: row = input within the tile side, chunk = 16-byte group of
+// samples.  slot = chunk ^ ((row + row/8) & 7) gives distinct 16-byte bank
+// groups both for 8 consecutive rows at one chunk (operand reads) and for the
+// rows 4g+q / 2g+q written by 8 consecutive staging lanes.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lds_off(int side, int row, int chunk) {
+  const int slot = chunk ^ ((row + (row >> 3)) & 7);
+  return side * XG_SIDE_BYTES + row * XG_ROWB + (slot << 4);
+}
+
+// ---------------------------------------------------------------------------
+// Visibility store: input indices m (row) and n (column) within channel f.
+// Handles the triangle cut, the three output orders and store/accumulate.
+// Orders follow reference src/kernel.cuh:49-117 and src/cpu_util.c:57-132.
+// ---------------------------------------------------------------------------
+template <typename V>
+__device__ __forceinline__ void put(void *base, long long idx, V v, int acc) {
+  V *p = (V *)base + idx;
+  if (acc) *p = *p + v; else *p = v;
+}
+
+__device__ __forceinline__ void put_i32(void *base, long long idx, int v, int acc) {
+  unsigned *p = (unsigned *)base + idx;
+  if (acc) *p = *p + (unsigned)v; else *p = (unsigned)v;
+}
+
+// Output index of the real part and the distance to the imaginary part.
+__device__ __forceinline__ bool vis_index(const XgJob &J, int f, int m, int n,
+                                          long long &ire, long long &dim, bool &zero_only) {
+  const int ninput = 2 * J.nstation;
+  zero_only = false;
+  if (m >= ninput || n >= ninput) return false;
+  const int i = m >> 1, p1 = m & 1, j = n >> 1, p2 = n & 1;
+  if (J.order == XG_ORDER_REGTILE) {
+    const long long P = J.nstation >> 1;
+    const long long RT = P * (P + 1) / 2;
+    const int R = i >> 1, rx = i & 1, C = j >> 1, ry = j & 1;
+    if (i < j) {
+      // Quadrant "12" of a diagonal register tile is never written by the
+      // reference (kernel.cuh:65-71) and stays zero after a clear.
+      if (R != C || J.accumulate) return false;
+      zero_only = true;
+    }
+    const long long l = ((long long)f * 4 + (2 * ry + rx)) * RT + (long long)R * (R + 1) / 2 + C;
+    ire = (l * 2 + p1) * 2 + p2;
+    dim = J.plane;
+    return true;
+  }
+  if (i < j) return false;
+  const long long NB = (long long)J.nstation * (J.nstation + 1) / 2;
+  const long long k = (long long)f * NB + (long long)i * (i + 1) / 2 + j;
+  if (J.order == XG_ORDER_REALIMAG) {
+    ire = k * 4 + p1 * 2 + p2;
+    dim = J.plane;
+  } else {  // TRIANGULAR: interleaved complex
+    ire = ((k * 2 + p1) * 2 + p2) * 2;
+    dim = 1;
+  }
+  return true;
+}
+
+__device__ __forceinline__ void store_int(const XgJob &J, int f, int m, int n, int re, int im) {
+  long long ire, dim;
+  bool zero;
+  if (!vis_index(J, f, m, n, ire, dim, zero)) return;
+  if (zero) { re = 0; im = 0; }
+  if (J.out_kind == XG_OUT_I64) {
+    put<long long>(J.out, ire, (long long)re, J.accumulate);
+    put<long long>(J.out, ire + dim, (long long)im, J.accumulate);
+  } else {
+    put_i32(J.out, ire, re, J.accumulate);
+    put_i32(J.out, ire + dim, im, J.accumulate);
+  }
+}
+
+__device__ __forceinline__ void store_flt(const XgJob &J, int f, int m, int n, float re, float im) {
+  long long ire, dim;
+  bool zero;
+  if (!vis_index(J, f, m, n, ire, dim, zero)) return;
+  if (zero) { re = 0.f; im = 0.f; }
+  if (J.out_kind == XG_OUT_F64) {
+    put<double>(J.out, ire, (double)re, J.accumulate);
+    put<double>(J.out, ire + dim, (double)im, J.accumulate);
+  } else {
+    put<float>(J.out, ire, re, J.accumulate);
+    put<float>(J.out, ire + dim, im, J.accumulate);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA kernels (int8 and fp16 share the skeleton)
+// ---------------------------------------------------------------------------
+template <bool F16>
+struct MfmaTraits;
+
+// int8: a complex sample is 2 bytes; a 16-byte chunk holds 8 samples; one
+// stage is 64 samples.  Staging task: 4 inputs x 8 samples (8-byte loads).
+template <>
+struct MfmaTraits<false> {
+  static constexpr int kCplxBytes = 2;
+  static constexpr int kStageSamples = 64;
+  typedef v16i Acc;
+};
+
+// fp16: a complex sample is 4 bytes; a chunk holds 4 samples; one stage is 32
+// samples.  Staging task: 2 inputs x 8 samples (8-byte loads).
+template <>
+struct MfmaTraits<true> {
+  static constexpr int kCplxBytes = 4;
+  static constexpr int kStageSamples = 32;
+  typedef v16f Acc;
+};
+
+template <bool F16>
+__device__ __forceinline__ void stage_load(uint2 (&d)[8], const unsigned char *src, bool valid,
+                                           int t0, int ntime, long long tstride) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int t = t0 + k;
+    if (valid && t < ntime)
+      d[k] = *(const uint2 *)(src + (long long)t * tstride);
+    else
+      d[k] = make_uint2(0u, 0u);
+  }
+}
+
+// Corner turn of one staging task into LDS.
+//   int8: d[k] = samples t0+k of inputs 4g..4g+3 (2 bytes each).  Input q's 8
+//         samples become one 16-byte chunk (chunk index h) of row 4g+q.
+//   fp16: d[k] = samples t0+k of inputs 2g, 2g+1 (4 bytes each).  Input q's
+//         8 samples become chunks 2h and 2h+1 of row 2g+q.
+template <bool F16>
+__device__ __forceinline__ void stage_store(unsigned char *lds, int side, int g, int h,
+                                            const uint2 (&d)[8]) {
+  if (!F16) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const unsigned sel = (q & 1) ? 0x07060302u : 0x05040100u;
+      v4i w;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const unsigned lo = (q >> 1) ? d[2 * s].y : d[2 * s].x;
+        const unsigned hi = (q >> 1) ? d[2 * s + 1].y : d[2 * s + 1].x;
+        w[s] = (int)__builtin_amdgcn_perm(hi, lo, sel);
+      }
+      *(v4i *)(lds + lds_off(side, 4 * g + q, h)) = w;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        v4i w;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) w[s] = (int)(q ? d[4 * c + s].y : d[4 * c + s].x);
+        *(v4i *)(lds + lds_off(side, 2 * g + q, 2 * h + c)) = w;
+      }
+    }
+  }
+}
+
+template <bool F16>
+__device__ __forceinline__ void mfma_tile(const XgJob &J, const TileRef &T, unsigned char *lds) {
+  typedef MfmaTraits<F16> Tr;
+  typedef typename Tr::Acc Acc;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int ninput = 2 * J.nstation;
+  const bool diag = (T.rb == T.cb);
+
+  // staging assignment
+  const int g = F16 ? (tid & 63) : (tid & 31);
+  const int h = F16 ? (tid >> 6) : (tid >> 5);
+  const int per_task = F16 ? 2 : 4;  // inputs per staging task
+  const unsigned char *in_f =
+      (const unsigned char *)J.input + (long long)T.f * J.nstation * 2 * Tr::kCplxBytes;
+  const int inp0 = T.rb * XG_TILE + per_task * g;
+  const int inp1 = T.cb * XG_TILE + per_task * g;
+  const bool v0 = inp0 < ninput;
+  const bool v1 = (!diag) && inp1 < ninput;
+  const unsigned char *src0 = in_f + (long long)inp0 * Tr::kCplxBytes;
+  const unsigned char *src1 = in_f + (long long)inp1 * Tr::kCplxBytes;
+  const int nstage = (J.ntime + Tr::kStageSamples - 1) / Tr::kStageSamples;
+
+  const int wr = wave >> 1, wc = wave & 1;
+  const bool active = !(diag && wr < wc);
+  const int half = lane >> 5, r32 = lane & 31;
+  const int colside = diag ? 0 : 1;
+
+  Acc accR[2][2], accI[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        accR[a][b][e] = 0;
+        accI[a][b][e] = 0;
+      }
+  int csum[2] = {0, 0};
+
+  uint2 d0[8], d1[8];
+  stage_load<F16>(d0, src0, v0, 8 * h, J.ntime, J.tstride);
+  stage_load<F16>(d1, src1, v1, 8 * h, J.ntime, J.tstride);
+  stage_store<F16>(lds, 0, g, h, d0);
+  if (!diag) stage_store<F16>(lds, 1, g, h, d1);
+  __syncthreads();
+
+  for (int s = 0; s < nstage; ++s) {
+    const int buf = s & 1;
+    const bool more = (s + 1) < nstage;
+    if (more) {
+      const int t0 = (s + 1) * Tr::kStageSamples + 8 * h;
+      stage_load<F16>(d0, src0, v0, t0, J.ntime, J.tstride);
+      stage_load<F16>(d1, src1, v1, t0, J.ntime, J.tstride);
+    }
+    if (active) {
+      const unsigned char *base = lds + buf * XG_STAGE_BYTES;
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const int chunk = 2 * ks + half;
+        v4i A[2], Ap[2], B[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+          A[a] = *(const v4i *)(base + lds_off(0, wr * 64 + a * 32 + r32, chunk));
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          B[b] = *(const v4i *)(base + lds_off(colside, wc * 64 + b * 32 + r32, chunk));
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const unsigned x = (unsigned)A[a][e];
+            if (!F16)  // (ar,ai) -> (ai,~ar) per complex byte pair
+              Ap[a][e] = (int)(__builtin_amdgcn_perm(x, x, 0x02030001u) ^ 0xFF00FF00u);
+            else       // (re,im) -> (im,-re) per complex half pair
+              Ap[a][e] = (int)(((x >> 16) | (x << 16)) ^ 0x80000000u);
+          }
+        if (!F16) {
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              csum[b] = __builtin_amdgcn_sdot4(B[b][e], 0x01000100, csum[b], false);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            if (!F16) {
+              accR[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[a], B[b], accR[a][b], 0, 0, 0);
+              accI[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(Ap[a], B[b], accI[a][b], 0, 0, 0);
+            } else {
+              accR[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                  __builtin_bit_cast(v8h, A[a]), __builtin_bit_cast(v8h, B[b]), accR[a][b], 0, 0, 0);
+              accI[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                  __builtin_bit_cast(v8h, Ap[a]), __builtin_bit_cast(v8h, B[b]), accI[a][b], 0, 0, 0);
+            }
+          }
+      }
+    }
+    if (more) {
+      unsigned char *nb = lds + (buf ^ 1) * XG_STAGE_BYTES;
+      stage_store<F16>(nb, 0, g, h, d0);
+      if (!diag) stage_store<F16>(nb, 1, g, h, d1);
+    }
+    __syncthreads();
+  }
+
+  if (!active) return;
+  if (!F16) {
+    // lanes l and l^32 hold the two K-halves of column l&31
+    csum[0] += __shfl_xor(csum[0], 32);
+    csum[1] += __shfl_xor(csum[1], 32);
+  }
+  // 32x32 accumulator layout: element e of lane l is row 8*(e/4)+4*(l/32)+e%4,
+  // column l%32.
+  const int m0 = T.rb * XG_TILE + wr * 64;
+  const int n0 = T.cb * XG_TILE + wc * 64;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int m = m0 + a * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+        const int n = n0 + b * 32 + r32;
+        if (!F16) {
+          const int im = (int)((unsigned)accI[a][b][e] + (unsigned)csum[b]);
+          store_int(J, T.f, m, n, accR[a][b][e], im);
+        } else {
+          store_flt(J, T.f, m, n, accR[a][b][e], accI[a][b][e]);
+        }
+      }
+}
+
+template <bool F16>
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(2, 2)))
+void xg_xengine_mfma(XgLaunch L) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[XG_LDS_BYTES];
+  const int xcd = blockIdx.x % XG_NXCD;
+  const int wl = blockIdx.x / XG_NXCD;
+  TileRef T;
+  for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
+    mfma_tile<F16>(L.jobs[T.job], T, lds);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 plain kernel: exact int8 -> float, SIMT FMA.  Thread (ty,tx) of a
+// 16x16 layout owns rows ty+16a and columns tx+16b (a,b < 8) of the
+// 128x128-input tile.  Stage: 16 samples, float2 per input per sample.
+// ---------------------------------------------------------------------------
+#define XG_F32_TK 16
+#define XG_F32_SIDE (XG_F32_TK * XG_TILE)  // float2 elements per side per stage
+
+__device__ __forceinline__ void f32_tile(const XgJob &J, const TileRef &T, float2 *lds) {
+  const int tid = threadIdx.x;
+  const int ty = tid >> 4, tx = tid & 15;
+  const int ninput = 2 * J.nstation;
+  const bool diag = (T.rb == T.cb);
+  const int g = tid & 31, h = tid >> 5;  // staging: inputs 4g..4g+3, samples 2h,2h+1
+  const unsigned char *in_f = (const unsigned char *)J.input + (long long)T.f * J.nstation * 4;
+  const int inp0 = T.rb * XG_TILE + 4 * g, inp1 = T.cb * XG_TILE + 4 * g;
+  const bool v0 = inp0 < ninput, v1 = (!diag) && inp1 < ninput;
+  const unsigned char *src0 = in_f + (long long)inp0 * 2;
+  const unsigned char *src1 = in_f + (long long)inp1 * 2;
+  const int nstage = (J.ntime + XG_F32_TK - 1) / XG_F32_TK;
+
+  float accR[8][8], accI[8][8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      accR[a][b] = 0.f;
+      accI[a][b] = 0.f;
+    }
+
+  uint2 d0[2], d1[2];
+  auto load = [&](int s) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int t = s * XG_F32_TK + 2 * h + k;
+      const bool ok = t < J.ntime;
+      d0[k] = (v0 && ok) ? *(const uint2 *)(src0 + (long long)t * J.tstride) : make_uint2(0u, 0u);
+      d1[k] = (v1 && ok) ? *(const uint2 *)(src1 + (long long)t * J.tstride) : make_uint2(0u, 0u);
+    }
+  };
+  auto cvt_store = [&](float2 *side, const uint2 (&d)[2]) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      float4 lo, hi;
+      const unsigned x = d[k].x, y = d[k].y;
+      lo.x = (float)(signed char)(x);
+      lo.y = (float)(signed char)(x >> 8);
+      lo.z = (float)(signed char)(x >> 16);
+      lo.w = (float)(signed char)(x >> 24);
+      hi.x = (float)(signed char)(y);
+      hi.y = (float)(signed char)(y >> 8);
+      hi.z = (float)(signed char)(y >> 16);
+      hi.w = (float)(signed char)(y >> 24);
+      float4 *row = (float4 *)(side + (2 * h + k) * XG_TILE + 4 * g);
+      row[0] = lo;
+      row[1] = hi;
+    }
+  };
+
+  load(0);
+  cvt_store(lds, d0);
+  if (!diag) cvt_store(lds + XG_F32_SIDE, d1);
+  __syncthreads();
+  for (int s = 0; s < nstage; ++s) {
+    const int buf = s & 1;
+    const bool more = (s + 1) < nstage;
+    if (more) load(s + 1);
+    const float2 *R = lds + buf * 2 * XG_F32_SIDE;
+    const float2 *C = diag ? R : R + XG_F32_SIDE;
+#pragma unroll 2
+    for (int t = 0; t < XG_F32_TK; ++t) {
+      float2 r[8], c[8];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) r[a] = R[t * XG_TILE + ty + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) c[b] = C[t * XG_TILE + tx + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          accR[a][b] = fmaf(r[a].x, c[b].x, accR[a][b]);
+          accR[a][b] = fmaf(r[a].y, c[b].y, accR[a][b]);
+          accI[a][b] = fmaf(r[a].y, c[b].x, accI[a][b]);
+          accI[a][b] = fmaf(-r[a].x, c[b].y, accI[a][b]);
+        }
+    }
+    if (more) {
+      float2 *nb = lds + (buf ^ 1) * 2 * XG_F32_SIDE;
+      cvt_store(nb, d0);
+      if (!diag) cvt_store(nb + XG_F32_SIDE, d1);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      store_flt(J, T.f, T.rb * XG_TILE + ty + 16 * a, T.cb * XG_TILE + tx + 16 * b, accR[a][b],
+                accI[a][b]);
+}
+
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 2)))
+void xg_xengine_f32(XgLaunch L) {
+  __shared__ __attribute__((aligned(16))) float2 lds[4 * XG_F32_SIDE];
+  const int xcd = blockIdx.x % XG_NXCD;
+  const int wl = blockIdx.x / XG_NXCD;
+  TileRef T;
+  for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
+    f32_tile(L.jobs[T.job], T, lds);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Support kernels
+// ---------------------------------------------------------------------------
+__global__ void xg_fill_zero_k(uint4 *p, size_t n16, unsigned *tail, size_t ntail) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntail) tail[t] = 0u;
+}
+
+template <typename D, typename S>
+__global__ void xg_spill_k(D *dst, const S *src, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] += (D)src[i];
+}
+
+template <typename W, typename S, typename O>
+__global__ void xg_combine_k(O *dst, const W *spill, const S *cur, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = (O)(spill[i] + (W)cur[i]);
+}
+
+// Counter-based generator: byte i = lo + ((hi32(splitmix64(key + i)) * span) >> 32)
+__device__ __forceinline__ uint64_t xg_mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__global__ void xg_fill_uniform_k(uint32_t *p, size_t nwords, uint64_t key, int lo, uint64_t span,
+                                  unsigned char *tailp, size_t ntail, size_t tail0) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += stride) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const uint64_t r = xg_mix64(key + 4 * w + b);
+      const int x = lo + (int)(((r >> 32) * span) >> 32);
+      v |= ((uint32_t)(x & 0xff)) << (8 * b);
+    }
+    p[w] = v;
+  }
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntail) {
+    const uint64_t r = xg_mix64(key + tail0 + t);
+    tailp[t] = (unsigned char)((lo + (int)(((r >> 32) * span) >> 32)) & 0xff);
+  }
+}
+
+// Reorder split-plane REGISTER_TILE / REAL_IMAG into interleaved TRIANGULAR
+// (reference src/cpu_util.c:57-132, run on the device).  One thread per
+// triangular visibility (f, i>=j, p1, p2).
+template <typename E>
+__global__ void xg_reorder_k(int src_order, E *dst, const E *src, int nstation, int nfreq,
+                             long long plane) {
+  const long long NB = (long long)nstation * (nstation + 1) / 2;
+  const long long total = (long long)nfreq * NB * 4;
+  const long long P = nstation >> 1;
+  const long long RT = P * (P + 1) / 2;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
+    const int pp = (int)(x & 3);
+    const long long kk = x >> 2;
+    const int f = (int)(kk / NB);
+    const long long b = kk - (long long)f * NB;
+    int i = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+    while ((long long)i * (i + 1) / 2 > b) --i;
+    while ((long long)(i + 1) * (i + 2) / 2 <= b) ++i;
+    const int j = (int)(b - (long long)i * (i + 1) / 2);
+    long long s;
+    if (src_order == XG_ORDER_REGTILE) {
+      const int R = i >> 1, rx = i & 1, C = j >> 1, ry = j & 1;
+      const long long l = ((long long)f * 4 + (2 * ry + rx)) * RT + (long long)R * (R + 1) / 2 + C;
+      s = l * 4 + pp;
+    } else {
+      s = kk * 4 + pp;
+    }
+    dst[2 * x] = src[s];
+    dst[2 * x + 1] = src[s + plane];
+  }
+}
+
+// dp4a input corner turn (reference src/cpu_util.c:235-255) on the device:
+// out byte ((((t/4*NF+f)*NS+s)*2+p)*2+c)*4 + t%4 = in byte (((t*NF+f)*NS+s)*2+p)*2+c.
+// One thread per output 32-bit word (4 consecutive samples of one component).
+__global__ void xg_swizzle_k(uint32_t *dst, const unsigned char *src, int nstation, int nfreq,
+                             int ntime) {
+  const long long row = (long long)nfreq * nstation * 4;  // bytes per time sample
+  const long long total = (long long)(ntime / 4) * row;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += stride) {
+    const long long t4 = w / row;
+    const long long e = w - t4 * row;
+    uint32_t v = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) v |= (uint32_t)src[(t4 * 4 + b) * row + e] << (8 * b);
+    dst[w] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(size_t n, unsigned block) {
+  size_t g = (n + block - 1) / block;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+extern "C" int xgh_launch_xengine(int family, const XgLaunch *L, void *stream) {
+  const unsigned grid = (unsigned)(XG_NXCD * L->wg_per_xcd);
+  if (grid == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  switch (family) {
+    case XG_KERNEL_I8_MFMA:
+      hipLaunchKernelGGL(xg_xengine_mfma<false>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      break;
+    case XG_KERNEL_F16_MFMA:
+      hipLaunchKernelGGL(xg_xengine_mfma<true>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      break;
+    case XG_KERNEL_F32_PLAIN:
+      hipLaunchKernelGGL(xg_xengine_f32, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      break;
+    default:
+      return xgh_report(hipErrorInvalidValue, "xgh_launch_xengine(family)");
+  }
+  return xgh_report(hipGetLastError(), "xg_xengine launch");
+}
+
+extern "C" int xgh_launch_fill_zero(void *ptr, size_t bytes, void *stream) {
+  if (!bytes) return 0;
+  const size_t n16 = bytes / 16;
+  const size_t ntail = (bytes - n16 * 16) / 4;
+  unsigned *tail = (unsigned *)((char *)ptr + n16 * 16);
+  hipLaunchKernelGGL(xg_fill_zero_k, dim3(grid_for(n16 > ntail ? n16 : ntail, 256)), dim3(256), 0,
+                     (hipStream_t)stream, (uint4 *)ptr, n16, tail, ntail);
+  return xgh_report(hipGetLastError(), "xg_fill_zero launch");
+}
+
+extern "C" int xgh_launch_spill(int kind, void *dst, const void *src, size_t n, void *stream) {
+  if (!n) return 0;
+  if (kind == XG_OUT_I64)
+    hipLaunchKernelGGL((xg_spill_k<long long, int>), dim3(grid_for(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (long long *)dst, (const int *)src, n);
+  else if (kind == XG_OUT_F64)
+    hipLaunchKernelGGL((xg_spill_k<double, float>), dim3(grid_for(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (double *)dst, (const float *)src, n);
+  else
+    return xgh_report(hipErrorInvalidValue, "xgh_launch_spill(kind)");
+  return xgh_report(hipGetLastError(), "xg_spill launch");
+}
+
+extern "C" int xgh_launch_combine(int kind, int narrow, void *dst, const void *spill,
+                                  const void *cur, size_t n, void *stream) {
+  if (!n) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 g(grid_for(n, 256)), b(256);
+  if (kind == XG_OUT_I64) {
+    if (narrow)  // low 32 bits: what wrapping int32 accumulation would hold
+      hipLaunchKernelGGL((xg_combine_k<unsigned long long, int, unsigned>), g, b, 0, s,
+                         (unsigned *)dst, (const unsigned long long *)spill, (const int *)cur, n);
+    else
+      hipLaunchKernelGGL((xg_combine_k<long long, int, long long>), g, b, 0, s, (long long *)dst,
+                         (const long long *)spill, (const int *)cur, n);
+  } else if (kind == XG_OUT_F64) {
+    if (narrow)
+      hipLaunchKernelGGL((xg_combine_k<double, float, float>), g, b, 0, s, (float *)dst,
+                         (const double *)spill, (const float *)cur, n);
+    else
+      hipLaunchKernelGGL((xg_combine_k<double, float, double>), g, b, 0, s, (double *)dst,
+                         (const double *)spill, (const float *)cur, n);
+  } else {
+    return xgh_report(hipErrorInvalidValue, "xgh_launch_combine(kind)");
+  }
+  return xgh_report(hipGetLastError(), "xg_combine launch");
+}
+
+extern "C" int xgh_launch_fill_uniform_i8(void *ptr, size_t n, uint64_t seed, int lo, int hi,
+                                          void *stream) {
+  if (!n) return 0;
+  if (hi < lo || lo < -128 || hi > 127)
+    return xgh_report(hipErrorInvalidValue, "xgh_launch_fill_uniform_i8(range)");
+  const size_t nwords = n / 4, ntail = n - nwords * 4;
+  // key = mix64(seed); the same definition is used by the host generator
+  uint64_t x = seed + 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  const uint64_t key = x ^ (x >> 31);
+  hipLaunchKernelGGL(xg_fill_uniform_k, dim3(grid_for(nwords > ntail ? nwords : ntail, 256)),
+                     dim3(256), 0, (hipStream_t)stream, (uint32_t *)ptr, nwords, key, lo,
+                     (uint64_t)(hi - lo + 1), (unsigned char *)ptr + nwords * 4, ntail, nwords * 4);
+  return xgh_report(hipGetLastError(), "xg_fill_uniform launch");
+}
+
+extern "C" int xgh_launch_reorder(int src_order, int elem_bytes, void *dst, const void *src,
+                                  int nstation, int nfreq, long long plane, void *stream) {
+  const long long total = (long long)nfreq * nstation * (nstation + 1) / 2 * 4;
+  if (total <= 0) return 0;
+  if (src_order != XG_ORDER_REGTILE && src_order != XG_ORDER_REALIMAG)
+    return xgh_report(hipErrorInvalidValue, "xgh_launch_reorder(order)");
+  const dim3 g(grid_for((size_t)total, 256)), b(256);
+  if (elem_bytes == 4)
+    hipLaunchKernelGGL((xg_reorder_k<unsigned>), g, b, 0, (hipStream_t)stream, src_order,
+                       (unsigned *)dst, (const unsigned *)src, nstation, nfreq, plane);
+  else if (elem_bytes == 8)
+    hipLaunchKernelGGL((xg_reorder_k<unsigned long long>), g, b, 0, (hipStream_t)stream, src_order,
+                       (unsigned long long *)dst, (const unsigned long long *)src, nstation, nfreq,
+                       plane);
+  else
+    return xgh_report(hipErrorInvalidValue, "xgh_launch_reorder(elem)");
+  return xgh_report(hipGetLastError(), "xg_reorder launch");
+}
+
+extern "C" int xgh_launch_swizzle(void *dst, const void *src, int nstation, int nfreq, int ntime,
+                                  void *stream) {
+  if (ntime % 4) return xgh_report(hipErrorInvalidValue, "xgh_launch_swizzle(ntime%4)");
+  const long long total = (long long)(ntime / 4) * nfreq * nstation * 4;
+  if (total <= 0) return 0;
+  hipLaunchKernelGGL(xg_swizzle_k, dim3(grid_for((size_t)total, 256)), dim3(256), 0,
+                     (hipStream_t)stream, (uint32_t *)dst, (const unsigned char *)src, nstation,
+                     nfreq, ntime);
+  return xgh_report(hipGetLastError(), "xg_swizzle launch");
+}
