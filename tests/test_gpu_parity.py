@@ -117,7 +117,7 @@ def test_i8_long_integration_spills_to_int64():
     expect = 3 * nt * 2 * 128 * 128
     assert expect > 2**31
     assert np.all(wide[:, 0] == expect) and np.all(wide[:, 1] == 0)
-    assert np.all(narrow[:, 0] == np.int64(expect).astype(np.int64) % 2**32 - (2**32 if expect % 2**32 >= 2**31 else 0))
+    assert np.all(narrow[:, 0] == ((expect + 2**31) % 2**32) - 2**31)
     eng.close()
     # random data, one call longer than the bound
     ns, nf, nt = 4, 2, 70001
@@ -165,11 +165,14 @@ def test_i8_full_size_cfg2_sampled():
     for n, (f, i, j) in enumerate(fij):
         assert np.array_equal(tri[f, i * (i + 1) // 2 + j], ref[n]), (f, i, j)
     # every autocorrelation: sum_t |x|^2 per input, from the input alone
-    a = x.reshape(nt, nf, ns, 2, 2).astype(np.int64)
-    power = (a ** 2).sum(axis=(0, 4))  # [f, s, p]
+    xt = x.reshape(nt, nf * ns * 2 * 2)
+    power = np.zeros(nf * ns * 2 * 2, dtype=np.int64)
+    for t0 in range(0, nt, 512):
+        c = xt[t0:t0 + 512].astype(np.int32)
+        power += (c * c).sum(axis=0, dtype=np.int64)
+    power = power.reshape(nf, ns, 2, 2).sum(axis=3)  # [f, s, p]
     autos = np.array([tri[:, i * (i + 1) // 2 + i, [0, 3], 0] for i in range(ns)])  # [s, f, p]
     assert np.array_equal(autos.transpose(1, 0, 2), power)
-    # sum over all baselines of the full matrix = |sum_m x_m|^2 per (t, f)
     eng.close()
     buf.free()
 
