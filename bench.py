@@ -84,12 +84,37 @@ print(json.dumps({"sec": ts[1], "kind": kind, "cores": len(cpus)}))
     }
 
 
+def profile_only(X, which, steps, warmup):
+    """Run one path's workload warmup+steps times (one kernel per step)."""
+    if which == "fp16":
+        ns, nf, nt, comp, nbytes = 256, 128, 2048, X.XGPU_FLOAT16, 256 * 128 * 2048 * 8
+    else:
+        ns, nf, nt = NS, NF, NT
+        comp = X.XGPU_INT8 if which == "int8" else X.XGPU_FLOAT32
+        nbytes = ns * nf * nt * 4
+    buf = X.DeviceBuffer(nbytes)
+    if which == "fp16":
+        buf.fill_uniform_int8(4, 0, 59)
+    else:
+        buf.fill_uniform_int8(2, -128, 127)
+    eng = X.Engine(ns, nf, compute=comp, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    for _ in range(warmup + steps):
+        eng.clear()
+        eng.correlate(buf, nt)
+    eng.synchronize()
+    log(f"[bench] {which}: {warmup + steps} integrations of NS={ns} NF={nf} NT={nt} done")
+    eng.close()
+    buf.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["int8", "fp32", "fp16"], default=None,
+                    help="run just this path's workload (for profiling); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this X-engine build runs on one GPU; --gpus {args.gpus} is not "
@@ -99,6 +124,9 @@ def main():
 
     import xgpu_amd as X
 
+    if args.only:
+        profile_only(X, args.only, args.steps, args.warmup)
+        return
     cmac = X.cmac_per_integration(NS, NF, NT)
     log(f"[bench] int8 MFMA NS={NS} NF={NF} NT={NT}: {cmac:.4e} CMAC per step")
     buf = X.DeviceBuffer(NT * NF * NS * 4)
