@@ -132,8 +132,14 @@ int xgpuAmdFillUniformInt8(void *ptr_d, size_t nbytes, unsigned long long seed, 
 int xgpuAmdTimeCorrelate(XGPUAmdEngine *engine, const void *input_d, unsigned int ntime,
                          int warmup, int reps, float *ms);
 
-/* Kernel tuning of an engine: tile workgroups per CU (1 or 2; 0 = default). */
-int xgpuAmdSetOccupancy(XGPUAmdEngine *engine, int wg_per_cu);
+/* Kernel tuning of an engine (0 fields = default). */
+typedef struct XGPUAmdTuningStruct {
+  int wg_per_cu;   /* persistent workgroups per CU: 1 or 2 */
+  int prefetch;    /* MFMA kernels: 1 = touch the input 2 stages ahead into L2, 2 = off */
+  int reserved[6];
+} XGPUAmdTuning;
+int xgpuAmdSetTuning(XGPUAmdEngine *engine, const XGPUAmdTuning *tuning);
+int xgpuAmdGetTuning(const XGPUAmdEngine *engine, XGPUAmdTuning *tuning);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,8 @@ from ._lib import (  # noqa: F401  (re-exported constants)
     XGPUContext, XGPUInfo, XgpuError, XgpuLibraryMissing, check, core, variant,
 )
 
-__all__ = ["Engine", "DeviceBuffer", "info", "cmac_per_integration"]
+__all__ = ["Engine", "DeviceBuffer", "HostBuffer", "Server", "info", "cmac_per_integration",
+           "correlate_batch"]
 
 I32_SAMPLE_BOUND = 65535  # samples an int32 integration holds for any int8 input
 
@@ -155,8 +156,16 @@ class Engine:
         check(core().xgpuAmdTimeCorrelate(self.handle, ptr, ntime, warmup, reps, C.byref(ms)))
         return float(ms.value)
 
-    def set_occupancy(self, wg_per_cu: int) -> None:
-        check(core().xgpuAmdSetOccupancy(self.handle, wg_per_cu))
+    def set_tuning(self, wg_per_cu: int = 0, prefetch: int = 0) -> None:
+        """wg_per_cu: 1 or 2 (0 default); prefetch: 1 on, 2 off (0 default)."""
+        t = _lib.XGPUAmdTuning()
+        t.wg_per_cu, t.prefetch = wg_per_cu, prefetch
+        check(core().xgpuAmdSetTuning(self.handle, C.byref(t)))
+
+    def get_tuning(self) -> dict:
+        t = _lib.XGPUAmdTuning()
+        check(core().xgpuAmdGetTuning(self.handle, C.byref(t)))
+        return {"wg_per_cu": t.wg_per_cu, "prefetch": t.prefetch}
 
     def close(self) -> None:
         if getattr(self, "handle", None):
@@ -178,3 +187,86 @@ def correlate_batch(items, stream=None) -> None:
         arr[k].input_d = buf.ptr if isinstance(buf, DeviceBuffer) else int(buf)
         arr[k].ntime = nt
     check(core().xgpuAmdCorrelateBatch(arr, len(items), stream))
+
+
+class HostBuffer:
+    """Pinned host memory viewed as a numpy array (fast asynchronous copies)."""
+
+    def __init__(self, nbytes: int, dtype=np.int8):
+        p = C.c_void_p()
+        check(core().xgpuAmdHostAlloc(C.byref(p), max(int(nbytes), 1)))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+        dt = np.dtype(dtype)
+        self.array = np.ctypeslib.as_array((C.c_byte * self.nbytes).from_address(self.ptr)).view(dt)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            core().xgpuAmdHostFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching correlation server (include/xgpu_serve.h)."""
+
+    def __init__(self, device=0, staging_bytes=0, max_batch=0, max_inflight=0):
+        cfg = _lib.XGPUServerConfig(device, staging_bytes, max_batch, max_inflight)
+        h = C.c_void_p()
+        check(core().xgpuServerCreate(C.byref(h), C.byref(cfg)))
+        self.handle = h.value
+        self.rings = []
+
+    def open_stream(self, nstation, nfrequency, blocks_per_dump, compute=XGPU_INT8, dump_slots=4):
+        """Returns (stream id, dump ring: HostBuffer of dump_slots dumps)."""
+        ns2 = nstation * (nstation + 1) // 2
+        dump_bytes = nfrequency * ns2 * 4 * 2 * 8
+        ring = HostBuffer(dump_bytes * dump_slots, np.int64 if compute == XGPU_INT8 else np.float64)
+        cfg = _lib.XGPUStreamConfig(nstation, nfrequency, compute, blocks_per_dump)
+        sid = C.c_int()
+        check(core().xgpuServerOpenStream(self.handle, C.byref(cfg), ring.ptr, dump_slots,
+                                          C.byref(sid)))
+        assert core().xgpuServerDumpBytes(self.handle, sid.value) == dump_bytes
+        self.rings.append(ring)
+        return sid.value, ring
+
+    def submit(self, sid: int, block, ntime: int) -> None:
+        ptr = block.ptr if isinstance(block, HostBuffer) else int(block)
+        check(core().xgpuServerSubmit(self.handle, sid, ptr, ntime))
+
+    def drain(self) -> None:
+        check(core().xgpuServerDrain(self.handle))
+
+    def blocks_done(self, sid):
+        return int(core().xgpuServerBlocksDone(self.handle, sid))
+
+    def dumps_done(self, sid):
+        return int(core().xgpuServerDumpsDone(self.handle, sid))
+
+    def stats(self) -> dict:
+        st = _lib.XGPUServerStats()
+        check(core().xgpuServerStats(self.handle, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in _lib.XGPUServerStats._fields_}
+
+    def reset_stats(self) -> None:
+        check(core().xgpuServerResetStats(self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            core().xgpuServerDestroy(self.handle)
+            self.handle = None
+        for r in self.rings:
+            r.free()
+        self.rings = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -74,6 +74,29 @@ class XGPUAmdBatchItem(C.Structure):
     _fields_ = [("engine", C.c_void_p), ("input_d", C.c_void_p), ("ntime", C.c_uint)]
 
 
+class XGPUAmdTuning(C.Structure):
+    _fields_ = [("wg_per_cu", C.c_int), ("prefetch", C.c_int), ("reserved", C.c_int * 6)]
+
+
+class XGPUServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("staging_bytes", C.c_size_t), ("max_batch", C.c_int),
+                ("max_inflight", C.c_int)]
+
+
+class XGPUStreamConfig(C.Structure):
+    _fields_ = [("nstation", C.c_uint), ("nfrequency", C.c_uint), ("compute_type", C.c_uint),
+                ("blocks_per_dump", C.c_uint)]
+
+
+class XGPUServerStats(C.Structure):
+    _fields_ = [("blocks_submitted", C.c_ulonglong), ("blocks_done", C.c_ulonglong),
+                ("dumps_done", C.c_ulonglong), ("launches", C.c_ulonglong),
+                ("cmac_done", C.c_ulonglong), ("busy_s", C.c_double),
+                ("tcmac_per_s", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("max_ms", C.c_double), ("mean_batch", C.c_double),
+                ("staging_peak_bytes", C.c_size_t)]
+
+
 _core = None
 _variants: dict = {}
 
@@ -120,7 +143,20 @@ def core() -> C.CDLL:
         _sig(lib, "xgpuAmdDeviceSynchronize", I)
         _sig(lib, "xgpuAmdFillUniformInt8", I, P, SZ, ULL, I, I)
         _sig(lib, "xgpuAmdTimeCorrelate", I, P, P, U, I, I, C.POINTER(C.c_float))
-        _sig(lib, "xgpuAmdSetOccupancy", I, P, I)
+        _sig(lib, "xgpuAmdSetTuning", I, P, C.POINTER(XGPUAmdTuning))
+        _sig(lib, "xgpuAmdGetTuning", I, P, C.POINTER(XGPUAmdTuning))
+        _sig(lib, "xgpuAmdHostAlloc", I, C.POINTER(P), SZ)
+        _sig(lib, "xgpuAmdHostFree", I, P)
+        _sig(lib, "xgpuServerCreate", I, C.POINTER(P), C.POINTER(XGPUServerConfig))
+        _sig(lib, "xgpuServerDestroy", I, P)
+        _sig(lib, "xgpuServerOpenStream", I, P, C.POINTER(XGPUStreamConfig), P, U, C.POINTER(I))
+        _sig(lib, "xgpuServerDumpBytes", SZ, P, I)
+        _sig(lib, "xgpuServerSubmit", I, P, I, P, U)
+        _sig(lib, "xgpuServerDrain", I, P)
+        _sig(lib, "xgpuServerBlocksDone", ULL, P, I)
+        _sig(lib, "xgpuServerDumpsDone", ULL, P, I)
+        _sig(lib, "xgpuServerStats", I, P, C.POINTER(XGPUServerStats))
+        _sig(lib, "xgpuServerResetStats", I, P)
         _core = lib
     return _core
 

@@ -131,7 +131,7 @@ void xg_plan_launch(XgLaunch *L, int cus, int wg_per_cu) {
   }
   int slots = (cus / XG_NXCD) * wg_per_cu;
   if (slots < 1) slots = 1;
-  L->wg_per_xcd = maxx < slots ? maxx : slots;
+  L->wg_per_xcd = (short)(maxx < slots ? maxx : slots);
 }
 
 int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
@@ -160,6 +160,7 @@ int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
   e->integ_bytes = (size_t)info.matLength * 2 * 4;
   e->cleared = 1;
   e->wg_per_cu = 2;
+  e->prefetch = XG_DEFAULT_PREFETCH;
   if ((rc = xgh_set_device(cfg->device)) != 0) goto hip_error;
   if ((rc = xgh_device_cu_count(cfg->device, &e->cus)) != 0) goto hip_error;
   if ((rc = xgh_malloc(&e->integ, e->integ_bytes)) != 0) goto hip_error;
@@ -213,10 +214,21 @@ int xgpuAmdClear(XGPUAmdEngine *e) {
 unsigned long long xgpuAmdSamples(const XGPUAmdEngine *e) { return e ? e->samples : 0; }
 void *xgpuAmdStream(XGPUAmdEngine *e) { return e ? e->stream : NULL; }
 
-int xgpuAmdSetOccupancy(XGPUAmdEngine *e, int wg_per_cu) {
-  if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
-  if (wg_per_cu < 0 || wg_per_cu > 2) return xg_fail(XGPU_INVALID_CONFIG, "wg_per_cu must be 0..2");
-  e->wg_per_cu = wg_per_cu ? wg_per_cu : 2;
+int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
+  if (!e || !t) return xg_fail(XGPU_NOT_INITIALIZED, "NULL argument");
+  if (t->wg_per_cu < 0 || t->wg_per_cu > 2)
+    return xg_fail(XGPU_INVALID_CONFIG, "wg_per_cu must be 0..2");
+  if (t->prefetch < 0 || t->prefetch > 2) return xg_fail(XGPU_INVALID_CONFIG, "prefetch must be 0..2");
+  e->wg_per_cu = t->wg_per_cu ? t->wg_per_cu : 2;
+  e->prefetch = t->prefetch == 0 ? XG_DEFAULT_PREFETCH : (t->prefetch == 1);
+  return XGPU_OK;
+}
+
+int xgpuAmdGetTuning(const XGPUAmdEngine *e, XGPUAmdTuning *t) {
+  if (!e || !t) return xg_fail(XGPU_NOT_INITIALIZED, "NULL argument");
+  memset(t, 0, sizeof *t);
+  t->wg_per_cu = e->wg_per_cu;
+  t->prefetch = e->prefetch ? 1 : 2;
   return XGPU_OK;
 }
 
@@ -242,6 +254,7 @@ int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime) {
   L.njobs = 1;
   job_fill(&L.jobs[0], e, input_d, ntime, !e->cleared, 0);
   xg_plan_launch(&L, e->cus, e->wg_per_cu);
+  L.prefetch = (short)e->prefetch;
   HIPCALL(xgh_launch_xengine(e->family, &L, e->stream), "xengine kernel");
   e->cleared = 0;
   e->samples_in_integ += ntime;
@@ -277,18 +290,15 @@ static int ensure_scratch(XGPUAmdEngine *e, void **buf, size_t bytes) {
   return XGPU_OK;
 }
 
-int xgpuAmdDump(XGPUAmdEngine *e, void *dst, int flags) {
-  if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
-  if (!dst) return xg_fail(XGPU_INVALID_CONFIG, "NULL dump destination");
-  HIPCALL(xgh_set_device(e->cfg.device), "set device");
+/* Stage the integration (flags XGPU_DUMP_*) into device memory and return a
+ * pointer to it and its size; everything is queued on the engine stream. */
+static int dump_stage(XGPUAmdEngine *e, int flags, const void **out, size_t *bytes) {
   const int wide = (flags & XGPU_DUMP_WIDE) != 0;
-  const int on_dev = (flags & XGPU_DUMP_DEVICE) != 0;
   const int tri = (flags & XGPU_DUMP_TRIANGULAR) && e->cfg.matrix_order != TRIANGULAR_ORDER;
   const size_t n = e->info.matLength * 2; /* scalars in the integration */
   const size_t esz = wide ? 8 : 4;
   const void *val = NULL;
   int rc;
-  /* scratch holds up to n wide scalars */
   if (e->cleared && !e->spilled) {
     if ((rc = ensure_scratch(e, &e->scratch, n * 8))) return rc;
     HIPCALL(xgh_launch_fill_zero(e->scratch, n * esz, e->stream), "dump zero");
@@ -301,11 +311,9 @@ int xgpuAmdDump(XGPUAmdEngine *e, void *dst, int flags) {
       HIPCALL(xgh_launch_combine(e->wide_kind, !wide, e->scratch, e->wide, e->integ, n, e->stream),
               "dump combine");
     } else if (e->spilled) { /* only the spilled part is live */
-      if (wide)
+      if (wide) {
         HIPCALL(xgh_memcpy_d2d_async(e->scratch, e->wide, n * 8, e->stream), "dump copy");
-      else {
-        HIPCALL(xgh_launch_fill_zero(e->scratch, n * 4, e->stream), "dump zero");
-        /* scratch(narrow) = wide + 0 */
+      } else {
         if ((rc = ensure_scratch(e, &e->scratch2, n * 8))) return rc;
         HIPCALL(xgh_launch_fill_zero(e->scratch2, n * 4, e->stream), "dump zero");
         HIPCALL(xgh_launch_combine(e->wide_kind, 1, e->scratch, e->wide, e->scratch2, n, e->stream),
@@ -325,14 +333,35 @@ int xgpuAmdDump(XGPUAmdEngine *e, void *dst, int flags) {
             "dump reorder");
     val = e->scratch2;
   }
-  const size_t bytes = (tri ? e->info.triLength * 2 : n) * esz;
-  if (on_dev) {
+  *out = val;
+  *bytes = (tri ? e->info.triLength * 2 : n) * esz;
+  return XGPU_OK;
+}
+
+int xg_dump_async(XGPUAmdEngine *e, void *dst, int flags) {
+  const void *val = NULL;
+  size_t bytes = 0;
+  int rc = dump_stage(e, flags, &val, &bytes);
+  if (rc) return rc;
+  if (flags & XGPU_DUMP_DEVICE)
     HIPCALL(xgh_memcpy_d2d_async(dst, val, bytes, e->stream), "dump D2D");
-    HIPCALL(xgh_stream_sync(e->stream), "dump sync");
-  } else {
+  else
     HIPCALL(xgh_memcpy_d2h_async(dst, val, bytes, e->stream), "dump D2H");
-    HIPCALL(xgh_stream_sync(e->stream), "dump sync");
-  }
+  return XGPU_OK;
+}
+
+size_t xg_dump_bytes(const XGPUAmdEngine *e, int flags) {
+  const int tri = (flags & XGPU_DUMP_TRIANGULAR) && e->cfg.matrix_order != TRIANGULAR_ORDER;
+  return (tri ? e->info.triLength * 2 : e->info.matLength * 2) * ((flags & XGPU_DUMP_WIDE) ? 8 : 4);
+}
+
+int xgpuAmdDump(XGPUAmdEngine *e, void *dst, int flags) {
+  if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
+  if (!dst) return xg_fail(XGPU_INVALID_CONFIG, "NULL dump destination");
+  HIPCALL(xgh_set_device(e->cfg.device), "set device");
+  int rc = xg_dump_async(e, dst, flags);
+  if (rc) return rc;
+  HIPCALL(xgh_stream_sync(e->stream), "dump sync");
   return XGPU_OK;
 }
 
@@ -435,6 +464,7 @@ int xgpuAmdCorrelateBatch(const XGPUAmdBatchItem *items, int nitems, void *strea
   }
   L.njobs = nitems;
   xg_plan_launch(&L, e0->cus, e0->wg_per_cu);
+  L.prefetch = (short)e0->prefetch;
   HIPCALL(xgh_launch_xengine(e0->family, &L, s), "batched xengine kernel");
   HIPCALL(xgh_event_record(e0->ev_aux, s), "record");
   for (int i = 0; i < nitems; ++i) {

@@ -21,6 +21,7 @@ struct XGPUAmdEngineStruct {
   size_t integ_bytes;
   int cus;
   int wg_per_cu;
+  int prefetch;      /* MFMA kernels: L2 prefetch 2 stages ahead */
 
   void *integ;       /* int32 / f32 integration, 2*matLength scalars */
   void *wide;        /* int64 / f64 spill, allocated on first spill */
@@ -43,8 +44,15 @@ struct XGPUAmdEngineStruct {
   int parity;
 };
 
+/* default of the L2 prefetch knob of the MFMA kernels */
+#define XG_DEFAULT_PREFETCH 0
+
 int xg_fail(int code, const char *fmt, ...);
 void xg_plan_launch(XgLaunch *L, int cus, int wg_per_cu);
 int xg_spill(XGPUAmdEngine *e);
+int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime);
+/* queue a dump (flags XGPU_DUMP_*) on the engine stream without waiting */
+int xg_dump_async(XGPUAmdEngine *e, void *dst, int flags);
+size_t xg_dump_bytes(const XGPUAmdEngine *e, int flags);
 
 #endif /* XG_ENGINE_INTERNAL_H */

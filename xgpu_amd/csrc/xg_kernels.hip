@@ -244,8 +244,11 @@ __device__ __forceinline__ void stage_store(unsigned char *lds, int side, int g,
   }
 }
 
-template <bool F16>
-__device__ __forceinline__ void mfma_tile(const XgJob &J, const TileRef &T, unsigned char *lds) {
+// PF: also touch the lines of stage s+2 with one dword load per 128-byte line
+// per thread (consumed a stage later into a sink), so the real loads of stage
+// s+1 find their lines in L2.  Returns the sink.
+template <bool F16, bool PF>
+__device__ __forceinline__ unsigned mfma_tile(const XgJob &J, const TileRef &T, unsigned char *lds) {
   typedef MfmaTraits<F16> Tr;
   typedef typename Tr::Acc Acc;
   const int tid = threadIdx.x;
@@ -267,6 +270,19 @@ __device__ __forceinline__ void mfma_tile(const XgJob &J, const TileRef &T, unsi
   const unsigned char *src1 = in_f + (long long)inp1 * Tr::kCplxBytes;
   const int nstage = (J.ntime + Tr::kStageSamples - 1) / Tr::kStageSamples;
 
+  // prefetch assignment: thread -> (side, sample, 128-byte segment) of a stage
+  const int pf_side = tid >> 7, pf_u = tid & 127;
+  const int pf_sample = pf_u / Tr::kCplxBytes, pf_seg = pf_u % Tr::kCplxBytes;
+  const int pf_inp = (pf_side ? T.cb : T.rb) * XG_TILE + pf_seg * (128 / Tr::kCplxBytes);
+  const bool pf_ok = PF && pf_inp < ninput && !(pf_side && diag);
+  const unsigned char *pf_src = in_f + (long long)pf_inp * Tr::kCplxBytes;
+  unsigned pf_prev = 0, pf_sink = 0;
+  auto prefetch = [&](int st) {
+    const int t = st * Tr::kStageSamples + pf_sample;
+    pf_sink ^= pf_prev;
+    pf_prev = (pf_ok && t < J.ntime) ? *(const unsigned *)(pf_src + (long long)t * J.tstride) : 0u;
+  };
+
   const int wr = wave >> 1, wc = wave & 1;
   const bool active = !(diag && wr < wc);
   const int half = lane >> 5, r32 = lane & 31;
@@ -287,6 +303,7 @@ __device__ __forceinline__ void mfma_tile(const XgJob &J, const TileRef &T, unsi
   uint2 d0[8], d1[8];
   stage_load<F16>(d0, src0, v0, 8 * h, J.ntime, J.tstride);
   stage_load<F16>(d1, src1, v1, 8 * h, J.ntime, J.tstride);
+  if (PF) prefetch(1);
   stage_store<F16>(lds, 0, g, h, d0);
   if (!diag) stage_store<F16>(lds, 1, g, h, d1);
   __syncthreads();
@@ -299,6 +316,7 @@ __device__ __forceinline__ void mfma_tile(const XgJob &J, const TileRef &T, unsi
       stage_load<F16>(d0, src0, v0, t0, J.ntime, J.tstride);
       stage_load<F16>(d1, src1, v1, t0, J.ntime, J.tstride);
     }
+    if (PF && s + 2 < nstage) prefetch(s + 2);
     if (active) {
       const unsigned char *base = lds + buf * XG_STAGE_BYTES;
 #pragma unroll
@@ -352,7 +370,8 @@ __device__ __forceinline__ void mfma_tile(const XgJob &J, const TileRef &T, unsi
     __syncthreads();
   }
 
-  if (!active) return;
+  pf_sink ^= pf_prev;
+  if (!active) return pf_sink;
   if (!F16) {
     // lanes l and l^32 hold the two K-halves of column l&31
     csum[0] += __shfl_xor(csum[0], 32);
@@ -377,19 +396,23 @@ __device__ __forceinline__ void mfma_tile(const XgJob &J, const TileRef &T, unsi
           store_flt(J, T.f, m, n, accR[a][b][e], accI[a][b][e]);
         }
       }
+  return pf_sink;
 }
 
-template <bool F16>
+template <bool F16, bool PF>
 __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(2, 2)))
 void xg_xengine_mfma(XgLaunch L) {
   __shared__ __attribute__((aligned(16))) unsigned char lds[XG_LDS_BYTES];
   const int xcd = blockIdx.x % XG_NXCD;
   const int wl = blockIdx.x / XG_NXCD;
   TileRef T;
+  unsigned sink = 0;
   for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
-    mfma_tile<F16>(L.jobs[T.job], T, lds);
+    sink ^= mfma_tile<F16, PF>(L.jobs[T.job], T, lds);
     __syncthreads();
   }
+  // njobs is never negative: keeps the prefetch loads alive without a store
+  if (PF && L.njobs < 0) *(unsigned *)L.jobs[0].out = sink;
 }
 
 // ---------------------------------------------------------------------------
@@ -625,10 +648,16 @@ extern "C" int xgh_launch_xengine(int family, const XgLaunch *L, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   switch (family) {
     case XG_KERNEL_I8_MFMA:
-      hipLaunchKernelGGL(xg_xengine_mfma<false>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      if (L->prefetch)
+        hipLaunchKernelGGL((xg_xengine_mfma<false, true>), dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      else
+        hipLaunchKernelGGL((xg_xengine_mfma<false, false>), dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       break;
     case XG_KERNEL_F16_MFMA:
-      hipLaunchKernelGGL(xg_xengine_mfma<true>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      if (L->prefetch)
+        hipLaunchKernelGGL((xg_xengine_mfma<true, true>), dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      else
+        hipLaunchKernelGGL((xg_xengine_mfma<true, false>), dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       break;
     case XG_KERNEL_F32_PLAIN:
       hipLaunchKernelGGL(xg_xengine_f32, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);

@@ -61,7 +61,8 @@ typedef struct XgJob {
 typedef struct XgLaunch {
   XgJob jobs[XG_MAX_JOBS];
   int njobs;
-  int wg_per_xcd;                               /* grid = 8 * wg_per_xcd */
+  short wg_per_xcd;                             /* grid = 8 * wg_per_xcd */
+  short prefetch;                               /* MFMA kernels: L2 prefetch 2 stages ahead */
   int xcd_start[XG_NXCD][XG_MAX_JOBS + 1];      /* per-XCD prefix of tile counts */
 } XgLaunch;
 
