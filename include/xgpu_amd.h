@@ -136,10 +136,24 @@ int xgpuAmdTimeCorrelate(XGPUAmdEngine *engine, const void *input_d, unsigned in
 typedef struct XGPUAmdTuningStruct {
   int wg_per_cu;   /* persistent workgroups per CU: 1 or 2 */
   int prefetch;    /* MFMA kernels: 1 = touch the input 2 stages ahead into L2, 2 = off */
-  int reserved[6];
+  int graph;       /* host-buffer path: 1 = capture each call's transfers and kernels in
+                      a hipGraph once and replay it, 2 = issue them one by one */
+  int reserved[5];
 } XGPUAmdTuning;
 int xgpuAmdSetTuning(XGPUAmdEngine *engine, const XGPUAmdTuning *tuning);
 int xgpuAmdGetTuning(const XGPUAmdEngine *engine, XGPUAmdTuning *tuning);
+
+/* Autotuner: time every candidate tuning (workgroups per CU x L2 prefetch) on
+ * ntime samples of input_d with HIP events (warmup + reps calls each), keep the
+ * fastest on the engine and return it with its time.  When cache_path is not
+ * NULL the result is appended to that text tuning cache, keyed by device name,
+ * station count, channel count, block length and compute type, and an engine
+ * created later with the same key picks it up (xgpuAmdLoadTuning). */
+int xgpuAmdAutotune(XGPUAmdEngine *engine, const void *input_d, unsigned int ntime, int warmup,
+                    int reps, const char *cache_path, XGPUAmdTuning *best, float *best_ms);
+/* Apply the tuning cached for this engine's key and ntime; returns
+ * XGPU_INVALID_CONFIG (and leaves the engine unchanged) when there is none. */
+int xgpuAmdLoadTuning(XGPUAmdEngine *engine, const char *cache_path, unsigned int ntime);
 
 #ifdef __cplusplus
 }

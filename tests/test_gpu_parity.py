@@ -251,3 +251,70 @@ def test_batch_ragged_equals_alone():
     for e, r in zip(engines, refs):
         assert np.array_equal(e.dump(wide=True), 2 * r)
         e.close()
+
+
+# ------------------------------------------------------------------ host path + graphs
+def _host_case(ns=64, nf=4, nt=512, ntp=64, seed=9):
+    xs = [oracle.fill_uniform(nt * nf * ns * 4, seed + k, -128, 127) for k in range(3)]
+    refs = [oracle.xengine_i64(x, ns, nf, nt) for x in xs]
+    return ns, nf, nt, ntp, xs, refs
+
+
+@pytest.mark.parametrize("graph", [2, 1])
+def test_host_path_pipeline_and_graph_replays(graph):
+    """xgpuAmdCorrelateHost (the xgpuCudaXengine pipeline): chunked transfers and
+    kernels; with graphs on, each call is captured once and replayed.  Every
+    replay is checked against the oracle, with eager work and syncs between."""
+    ns, nf, nt, ntp, xs, refs = _host_case()
+    flush(f"[host] graph={graph}")
+    eng = X.Engine(ns, nf, nt, ntp, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
+    eng.set_tuning(graph=graph)
+    hb = [X.HostBuffer(x.nbytes) for x in xs]
+    for h, x in zip(hb, xs):
+        h.array[:] = x
+    out = np.zeros(2 * eng.mat_len, dtype=np.int32)
+    lib = X.core()
+    for rep in range(3):
+        eng.clear()
+        for k in range(3):  # three calls accumulate
+            sync = X.SYNCOP_DUMP if k == 2 else X.SYNCOP_SYNC_TRANSFER
+            X.check(lib.xgpuAmdCorrelateHost(eng.handle, hb[k].ptr, sync, out.ctypes.data))
+        assert np.array_equal(out.astype(np.int64), refs[0] + refs[1] + refs[2]), rep
+        # eager device-path work on the same engine between replays
+        eng.clear()
+        buf = X.DeviceBuffer.from_array(xs[rep])
+        eng.correlate(buf, nt)
+        assert np.array_equal(eng.dump().astype(np.int64), refs[rep])
+        buf.free()
+    assert eng.get_tuning()["graph"] == graph
+    eng.close()
+    for h in hb:
+        h.free()
+
+
+def test_autotune_keeps_results_exact(tmp_path):
+    ns, nf, nt = 128, 8, 1024
+    x = oracle.fill_uniform(nt * nf * ns * 4, 31, -128, 127)
+    ref = oracle.xengine_i64(x, ns, nf, nt)
+    buf = X.DeviceBuffer.from_array(x)
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
+    cache = str(tmp_path / "tuning.txt")
+    flush("[autotune] ns=128 nf=8 nt=1024")
+    best, ms = eng.autotune(buf, nt, warmup=1, reps=2, cache_path=cache)
+    assert ms > 0 and best["wg_per_cu"] in (1, 2) and best["prefetch"] in (1, 2)
+    eng.correlate(buf, nt)
+    assert np.array_equal(eng.dump().astype(np.int64), ref)
+    for wg in (1, 2):           # every candidate is exact
+        for pf in (1, 2):
+            eng.set_tuning(wg, pf)
+            eng.clear()
+            eng.correlate(buf, nt)
+            assert np.array_equal(eng.dump().astype(np.int64), ref), (wg, pf)
+    eng2 = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
+    assert eng2.load_tuning(cache, nt)
+    assert eng2.get_tuning()["wg_per_cu"] == best["wg_per_cu"]
+    assert eng2.get_tuning()["prefetch"] == best["prefetch"]
+    assert not eng2.load_tuning(cache, nt + 1)
+    eng.close()
+    eng2.close()
+    buf.free()

@@ -156,16 +156,30 @@ class Engine:
         check(core().xgpuAmdTimeCorrelate(self.handle, ptr, ntime, warmup, reps, C.byref(ms)))
         return float(ms.value)
 
-    def set_tuning(self, wg_per_cu: int = 0, prefetch: int = 0) -> None:
-        """wg_per_cu: 1 or 2 (0 default); prefetch: 1 on, 2 off (0 default)."""
+    def set_tuning(self, wg_per_cu: int = 0, prefetch: int = 0, graph: int = 0) -> None:
+        """wg_per_cu: 1 or 2; prefetch / graph: 1 on, 2 off (0 = default)."""
         t = _lib.XGPUAmdTuning()
-        t.wg_per_cu, t.prefetch = wg_per_cu, prefetch
+        t.wg_per_cu, t.prefetch, t.graph = wg_per_cu, prefetch, graph
         check(core().xgpuAmdSetTuning(self.handle, C.byref(t)))
 
     def get_tuning(self) -> dict:
         t = _lib.XGPUAmdTuning()
         check(core().xgpuAmdGetTuning(self.handle, C.byref(t)))
-        return {"wg_per_cu": t.wg_per_cu, "prefetch": t.prefetch}
+        return {"wg_per_cu": t.wg_per_cu, "prefetch": t.prefetch, "graph": t.graph}
+
+    def autotune(self, input_d, ntime: int, warmup: int = 1, reps: int = 3,
+                 cache_path: str | None = None):
+        """Time every kernel tuning on this input; keep and return the fastest."""
+        t = _lib.XGPUAmdTuning()
+        ms = C.c_float()
+        ptr = input_d.ptr if isinstance(input_d, DeviceBuffer) else int(input_d)
+        check(core().xgpuAmdAutotune(self.handle, ptr, ntime, warmup, reps,
+                                     cache_path.encode() if cache_path else None,
+                                     C.byref(t), C.byref(ms)))
+        return {"wg_per_cu": t.wg_per_cu, "prefetch": t.prefetch}, float(ms.value)
+
+    def load_tuning(self, cache_path: str, ntime: int) -> bool:
+        return core().xgpuAmdLoadTuning(self.handle, cache_path.encode(), ntime) == 0
 
     def close(self) -> None:
         if getattr(self, "handle", None):

@@ -75,7 +75,8 @@ class XGPUAmdBatchItem(C.Structure):
 
 
 class XGPUAmdTuning(C.Structure):
-    _fields_ = [("wg_per_cu", C.c_int), ("prefetch", C.c_int), ("reserved", C.c_int * 6)]
+    _fields_ = [("wg_per_cu", C.c_int), ("prefetch", C.c_int), ("graph", C.c_int),
+                ("reserved", C.c_int * 5)]
 
 
 class XGPUServerConfig(C.Structure):
@@ -145,6 +146,9 @@ def core() -> C.CDLL:
         _sig(lib, "xgpuAmdTimeCorrelate", I, P, P, U, I, I, C.POINTER(C.c_float))
         _sig(lib, "xgpuAmdSetTuning", I, P, C.POINTER(XGPUAmdTuning))
         _sig(lib, "xgpuAmdGetTuning", I, P, C.POINTER(XGPUAmdTuning))
+        _sig(lib, "xgpuAmdAutotune", I, P, P, U, I, I, C.c_char_p, C.POINTER(XGPUAmdTuning),
+             C.POINTER(C.c_float))
+        _sig(lib, "xgpuAmdLoadTuning", I, P, C.c_char_p, U)
         _sig(lib, "xgpuAmdHostAlloc", I, C.POINTER(P), SZ)
         _sig(lib, "xgpuAmdHostFree", I, P)
         _sig(lib, "xgpuServerCreate", I, C.POINTER(P), C.POINTER(XGPUServerConfig))

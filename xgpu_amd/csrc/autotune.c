@@ -1,0 +1,101 @@
+/*
+ * autotune.c -- kernel-parameter autotuner and tuning cache (no xGPU
+ * counterpart).
+ *
+ * Candidates are the engine's kernel knobs (persistent workgroups per CU,
+ * L2 prefetch of the MFMA kernels).  Each is timed with HIP events on the
+ * engine stream over `reps` single-kernel integrations of the caller's
+ * device input; the fastest is kept.  The cache is a text file, one line per
+ * result:
+ *   <device name>|<nstation>|<nfrequency>|<ntime>|<compute>|<wg_per_cu>|<prefetch>|<ms>
+ * Later lines win, so re-tuning simply appends.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "engine_internal.h"
+#include "xg_hip.h"
+#include "xgpu_amd.h"
+
+static void key_of(const XGPUAmdEngine *e, unsigned ntime, char *buf, size_t len) {
+  char name[128] = "unknown";
+  xgh_device_name(e->cfg.device, name, sizeof name);
+  for (char *p = name; *p; ++p)
+    if (*p == '|' || *p == '\n') *p = '_';
+  snprintf(buf, len, "%s|%u|%u|%u|%u", name, e->cfg.nstation, e->cfg.nfrequency, ntime,
+           e->cfg.compute_type);
+}
+
+int xgpuAmdLoadTuning(XGPUAmdEngine *e, const char *path, unsigned ntime) {
+  if (!e || !path) return xg_fail(XGPU_INVALID_CONFIG, "NULL argument");
+  FILE *f = fopen(path, "r");
+  if (!f) return xg_fail(XGPU_INVALID_CONFIG, "no tuning cache at %s", path);
+  char key[256], line[512];
+  key_of(e, ntime, key, sizeof key);
+  const size_t klen = strlen(key);
+  int found = 0;
+  XGPUAmdTuning t;
+  memset(&t, 0, sizeof t);
+  while (fgets(line, sizeof line, f)) {
+    if (strncmp(line, key, klen) == 0 && line[klen] == '|') {
+      int wg = 0, pf = 0;
+      if (sscanf(line + klen + 1, "%d|%d|", &wg, &pf) == 2) {
+        t.wg_per_cu = wg;
+        t.prefetch = pf;
+        found = 1;
+      }
+    }
+  }
+  fclose(f);
+  if (!found) return xg_fail(XGPU_INVALID_CONFIG, "no cached tuning for %s", key);
+  XGPUAmdTuning cur;
+  xgpuAmdGetTuning(e, &cur);
+  t.graph = cur.graph;
+  return xgpuAmdSetTuning(e, &t);
+}
+
+int xgpuAmdAutotune(XGPUAmdEngine *e, const void *input_d, unsigned ntime, int warmup, int reps,
+                    const char *cache_path, XGPUAmdTuning *best, float *best_ms) {
+  if (!e || !input_d || ntime == 0 || reps < 1)
+    return xg_fail(XGPU_INVALID_CONFIG, "bad autotune arguments");
+  XGPUAmdTuning keep;
+  xgpuAmdGetTuning(e, &keep);
+  XGPUAmdTuning win = keep;
+  float win_ms = 0.f;
+  int have = 0;
+  const int npf = (e->family == XG_KERNEL_F32_PLAIN) ? 1 : 2;
+  for (int wg = 1; wg <= 2; ++wg)
+    for (int pf = 1; pf <= npf; ++pf) {
+      XGPUAmdTuning t = keep;
+      t.wg_per_cu = wg;
+      t.prefetch = (e->family == XG_KERNEL_F32_PLAIN) ? 2 : pf;
+      int rc = xgpuAmdSetTuning(e, &t);
+      if (rc) return rc;
+      float ms = 0.f;
+      rc = xgpuAmdTimeCorrelate(e, input_d, ntime, warmup, reps, &ms);
+      if (rc) {
+        xgpuAmdSetTuning(e, &keep);
+        return rc;
+      }
+      if (!have || ms < win_ms) {
+        have = 1;
+        win_ms = ms;
+        win = t;
+      }
+    }
+  int rc = xgpuAmdSetTuning(e, &win);
+  if (rc) return rc;
+  xgpuAmdClear(e);
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;
+  if (cache_path) {
+    FILE *f = fopen(cache_path, "a");
+    if (!f) return xg_fail(XGPU_INVALID_CONFIG, "cannot write tuning cache %s", cache_path);
+    char key[256];
+    key_of(e, ntime, key, sizeof key);
+    fprintf(f, "%s|%d|%d|%.4f\n", key, win.wg_per_cu, win.prefetch, win_ms);
+    fclose(f);
+  }
+  return XGPU_OK;
+}

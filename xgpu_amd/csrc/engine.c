@@ -171,6 +171,7 @@ int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
   if ((rc = xgh_event_create(&e->ev_t0, 1)) != 0) goto hip_error;
   if ((rc = xgh_event_create(&e->ev_t1, 1)) != 0) goto hip_error;
   if ((rc = xgh_event_create(&e->ev_aux, 0)) != 0) goto hip_error;
+  if ((rc = xgh_event_create(&e->ev_join, 0)) != 0) goto hip_error;
   *out = e;
   return XGPU_OK;
 hip_error:
@@ -193,6 +194,8 @@ void xgpuAmdDestroy(XGPUAmdEngine *e) {
   xgh_event_destroy(e->ev_t0);
   xgh_event_destroy(e->ev_t1);
   xgh_event_destroy(e->ev_aux);
+  xgh_event_destroy(e->ev_join);
+  for (int g = 0; g < e->ngraphs; ++g) xgh_graph_destroy(e->graphs[g].exec);
   xgh_free(e->integ);
   xgh_free(e->wide);
   xgh_free(e->scratch);
@@ -219,6 +222,8 @@ int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
   if (t->wg_per_cu < 0 || t->wg_per_cu > 2)
     return xg_fail(XGPU_INVALID_CONFIG, "wg_per_cu must be 0..2");
   if (t->prefetch < 0 || t->prefetch > 2) return xg_fail(XGPU_INVALID_CONFIG, "prefetch must be 0..2");
+  if (t->graph < 0 || t->graph > 2) return xg_fail(XGPU_INVALID_CONFIG, "graph must be 0..2");
+  e->use_graph = (t->graph == 1);
   e->wg_per_cu = t->wg_per_cu ? t->wg_per_cu : 2;
   e->prefetch = t->prefetch == 0 ? XG_DEFAULT_PREFETCH : (t->prefetch == 1);
   return XGPU_OK;
@@ -229,6 +234,7 @@ int xgpuAmdGetTuning(const XGPUAmdEngine *e, XGPUAmdTuning *t) {
   memset(t, 0, sizeof *t);
   t->wg_per_cu = e->wg_per_cu;
   t->prefetch = e->prefetch ? 1 : 2;
+  t->graph = e->use_graph ? 1 : 2;
   return XGPU_OK;
 }
 
@@ -377,6 +383,28 @@ int xgpuAmdSynchronize(XGPUAmdEngine *e) {
  * input buffers alternate between calls; within a call, chunk p's transfer on
  * the copy stream is followed by its kernel on the compute stream, so the
  * transfer of chunk p+1 overlaps the kernel of chunk p. */
+/* Issue one host-path call: per chunk, H2D on the copy stream, then the
+ * kernel of that chunk on the compute stream. */
+static int host_issue(XGPUAmdEngine *e, const void *input_h, char *dbase, int npipe, unsigned ntp,
+                      size_t chunk) {
+  const char *hbase = (const char *)input_h;
+  for (int p = 0; p < npipe; ++p) {
+    HIPCALL(xgh_memcpy_h2d_async(dbase + p * chunk, hbase + p * chunk, chunk, e->copy_stream), "H2D");
+    HIPCALL(xgh_event_record(e->ev_chunk[p], e->copy_stream), "record");
+    HIPCALL(xgh_stream_wait_event(e->stream, e->ev_chunk[p]), "wait");
+    int rc = xgpuAmdCorrelate(e, dbase + p * chunk, ntp);
+    if (rc) return rc;
+  }
+  return XGPU_OK;
+}
+
+/* Host-buffer path (reference src/cuda_xengine.cu:543-677).  Two device
+ * input buffers alternate between calls; within a call, chunk p's transfer on
+ * the copy stream is followed by its kernel on the compute stream, so the
+ * transfer of chunk p+1 overlaps the kernel of chunk p.  With the graph knob
+ * on, the call is captured once per (input pointer, clear state, buffer) into
+ * a two-stream hipGraph and replayed on the compute stream; calls that would
+ * cross a spill are issued eagerly. */
 int xgpuAmdCorrelateHost(XGPUAmdEngine *e, const void *input_h, int syncOp, void *output_h) {
   if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
   if (!input_h) return xg_fail(XGPU_HOST_BUFFER_NOT_SET, "NULL host input");
@@ -398,16 +426,67 @@ int xgpuAmdCorrelateHost(XGPUAmdEngine *e, const void *input_h, int syncOp, void
       e->npipe_events = p + 1;
     }
   }
-  /* the buffer is free once the kernels of the call that used it finished */
-  if (e->ev_kernel_used[buf]) HIPCALL(xgh_stream_wait_event(e->copy_stream, e->ev_kernel[buf]), "wait");
   char *dbase = (char *)e->dev_in[buf];
-  const char *hbase = (const char *)input_h;
-  for (int p = 0; p < npipe; ++p) {
-    HIPCALL(xgh_memcpy_h2d_async(dbase + p * chunk, hbase + p * chunk, chunk, e->copy_stream), "H2D");
-    HIPCALL(xgh_event_record(e->ev_chunk[p], e->copy_stream), "record");
-    HIPCALL(xgh_stream_wait_event(e->stream, e->ev_chunk[p]), "wait");
-    int rc = xgpuAmdCorrelate(e, dbase + p * chunk, ntp);
-    if (rc) return rc;
+  int rc;
+  const int graphable = e->use_graph && e->samples_in_integ + e->cfg.ntime <= e->bound &&
+                        (e->cfg.ntime <= e->bound);
+  if (graphable) {
+    int g = -1;
+    for (int i = 0; i < e->ngraphs; ++i)
+      if (e->graphs[i].input_h == input_h && e->graphs[i].cleared == e->cleared &&
+          e->graphs[i].buf == buf)
+        g = i;
+    /* the graph orders after everything queued on the compute stream, and the
+     * copy stream's earlier transfers */
+    HIPCALL(xgh_event_record(e->ev_aux, e->copy_stream), "record");
+    HIPCALL(xgh_stream_wait_event(e->stream, e->ev_aux), "wait");
+    if (g < 0) {
+      if (e->ngraphs == (int)(sizeof e->graphs / sizeof e->graphs[0])) {
+        xgh_graph_destroy(e->graphs[0].exec);
+        memmove(&e->graphs[0], &e->graphs[1], sizeof e->graphs[0] * (e->ngraphs - 1));
+        e->ngraphs--;
+      }
+      const int was_cleared = e->cleared;
+      const unsigned s_in = e->samples_in_integ;
+      const unsigned long long s_all = e->samples;
+      HIPCALL(xgh_graph_begin(e->stream), "begin capture");
+      rc = xgh_event_record(e->ev_join, e->stream) || xgh_stream_wait_event(e->copy_stream, e->ev_join)
+               ? XGPU_CUDA_ERROR
+               : host_issue(e, input_h, dbase, npipe, ntp, chunk);
+      if (rc == XGPU_OK && (xgh_event_record(e->ev_join, e->copy_stream) ||
+                            xgh_stream_wait_event(e->stream, e->ev_join)))
+        rc = XGPU_CUDA_ERROR;
+      void *exec = NULL;
+      const int rc_end = xgh_graph_end(e->stream, &exec);
+      /* capture only recorded the work: restore the state it advanced */
+      e->cleared = was_cleared;
+      e->samples_in_integ = s_in;
+      e->samples = s_all;
+      if (rc == XGPU_OK && rc_end == 0) {
+        g = e->ngraphs++;
+        e->graphs[g].input_h = input_h;
+        e->graphs[g].cleared = was_cleared;
+        e->graphs[g].buf = buf;
+        e->graphs[g].exec = exec;
+      } else {
+        /* not capturable (e.g. a pageable source): issue this call eagerly */
+        xgh_graph_destroy(exec);
+        e->use_graph = 0;
+      }
+    }
+    if (g >= 0) {
+      HIPCALL(xgh_graph_launch(e->graphs[g].exec, e->stream), "graph launch");
+      e->cleared = 0;
+      e->samples_in_integ += e->cfg.ntime;
+      e->samples += e->cfg.ntime;
+    } else if ((rc = host_issue(e, input_h, dbase, npipe, ntp, chunk))) {
+      return rc;
+    }
+  } else {
+    /* the buffer is free once the kernels of the call that used it finished */
+    if (e->ev_kernel_used[buf])
+      HIPCALL(xgh_stream_wait_event(e->copy_stream, e->ev_kernel[buf]), "wait");
+    if ((rc = host_issue(e, input_h, dbase, npipe, ntp, chunk))) return rc;
   }
   HIPCALL(xgh_event_record(e->ev_kernel[buf], e->stream), "record");
   e->ev_kernel_used[buf] = 1;
@@ -417,8 +496,8 @@ int xgpuAmdCorrelateHost(XGPUAmdEngine *e, const void *input_h, int syncOp, void
     case SYNCOP_SYNC_COMPUTE:
       HIPCALL(xgh_stream_sync(e->stream), "sync compute");
       break;
-    case SYNCOP_SYNC_TRANSFER:
-      HIPCALL(xgh_stream_sync(e->copy_stream), "sync transfer");
+    case SYNCOP_SYNC_TRANSFER: /* graph replays carry their transfers on the compute stream */
+      HIPCALL(xgh_stream_sync(graphable ? e->stream : e->copy_stream), "sync transfer");
       break;
     default:
       break;

@@ -39,9 +39,18 @@ struct XGPUAmdEngineStruct {
   int ev_kernel_used[2];
   void **ev_chunk;
   int npipe_events;
-  void *ev_t0, *ev_t1, *ev_aux;
+  void *ev_t0, *ev_t1, *ev_aux, *ev_join;
   void *dev_in[2];
   int parity;
+
+  /* host-buffer path graphs, keyed by (input pointer, cleared state, buffer) */
+  int use_graph;
+  struct {
+    const void *input_h;
+    int cleared, buf;
+    void *exec;
+  } graphs[8];
+  int ngraphs;
 };
 
 /* default of the L2 prefetch knob of the MFMA kernels */
