@@ -158,7 +158,15 @@ def serving_bench(X, seed=5, nstreams=8, dumps_per_stream=2, load=0.7):
 
 
 def profile_only(X, which, steps, warmup):
-    """Run one path's workload warmup+steps times (one kernel per step)."""
+    """Run one path's workload warmup+steps times (one kernel per step).
+    "copy": a plain 1 GiB device-to-device copy per step (counter calibration)."""
+    if which == "copy":
+        a, b = X.DeviceBuffer(1 << 30), X.DeviceBuffer(1 << 30)
+        a.fill_uniform_int8(1, -128, 127)
+        for _ in range(warmup + steps):
+            X.check(X.core().xgpuAmdMemcpy(b.ptr, a.ptr, 1 << 30, 3))
+        log(f"[bench] copy: {warmup + steps} x 1 GiB device copies done")
+        return
     if which == "fp16":
         ns, nf, nt, comp, nbytes = 256, 128, 2048, X.XGPU_FLOAT16, 256 * 128 * 2048 * 8
     else:
@@ -187,7 +195,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--only", choices=["int8", "fp32", "fp16"], default=None,
+    ap.add_argument("--only", choices=["int8", "fp32", "fp16", "copy"], default=None,
                     help="run just this path's workload (for profiling); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:

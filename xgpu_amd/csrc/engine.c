@@ -592,7 +592,10 @@ int xgpuAmdMemcpy(void *dst, const void *src, size_t bytes, int kind) {
     case XGPU_H2D: HIPCALL(xgh_memcpy_h2d(dst, src, bytes), "xgpuAmdMemcpy"); break;
     case XGPU_D2H: HIPCALL(xgh_memcpy_d2h(dst, src, bytes), "xgpuAmdMemcpy"); break;
     case XGPU_D2D:
-      HIPCALL(xgh_memcpy_d2d_async(dst, src, bytes, NULL), "xgpuAmdMemcpy");
+      if (((((uintptr_t)dst | (uintptr_t)src) & 15) == 0) && (bytes & 3) == 0)
+        HIPCALL(xgh_launch_copy(dst, src, bytes, NULL), "xgpuAmdMemcpy");
+      else
+        HIPCALL(xgh_memcpy_d2d_async(dst, src, bytes, NULL), "xgpuAmdMemcpy");
       HIPCALL(xgh_device_sync(), "xgpuAmdMemcpy");
       break;
     default: return xg_fail(XGPU_INVALID_CONFIG, "bad copy kind %d", kind);

@@ -539,6 +539,16 @@ __global__ void xg_fill_zero_k(uint4 *p, size_t n16, unsigned *tail, size_t ntai
   if (t < ntail) tail[t] = 0u;
 }
 
+// Plain streaming copy (16-byte loads and stores, grid-stride).
+__global__ void xg_copy_k(uint4 *dst, const uint4 *src, size_t n16, unsigned *dtail,
+                          const unsigned *stail, size_t ntail) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    dst[i] = src[i];
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntail) dtail[t] = stail[t];
+}
+
 template <typename D, typename S>
 __global__ void xg_spill_k(D *dst, const S *src, size_t n) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -676,6 +686,18 @@ extern "C" int xgh_launch_fill_zero(void *ptr, size_t bytes, void *stream) {
   hipLaunchKernelGGL(xg_fill_zero_k, dim3(grid_for(n16 > ntail ? n16 : ntail, 256)), dim3(256), 0,
                      (hipStream_t)stream, (uint4 *)ptr, n16, tail, ntail);
   return xgh_report(hipGetLastError(), "xg_fill_zero launch");
+}
+
+extern "C" int xgh_launch_copy(void *dst, const void *src, size_t bytes, void *stream) {
+  if (!bytes) return 0;
+  if ((((uintptr_t)dst | (uintptr_t)src) & 15) || (bytes & 3))
+    return xgh_report(hipErrorInvalidValue, "xgh_launch_copy(alignment)");
+  const size_t n16 = bytes / 16, ntail = (bytes - n16 * 16) / 4;
+  hipLaunchKernelGGL(xg_copy_k, dim3(grid_for(n16 > ntail ? n16 : ntail, 256)), dim3(256), 0,
+                     (hipStream_t)stream, (uint4 *)dst, (const uint4 *)src, n16,
+                     (unsigned *)((char *)dst + n16 * 16),
+                     (const unsigned *)((const char *)src + n16 * 16), ntail);
+  return xgh_report(hipGetLastError(), "xg_copy launch");
 }
 
 extern "C" int xgh_launch_spill(int kind, void *dst, const void *src, size_t n, void *stream) {
