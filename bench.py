@@ -157,7 +157,7 @@ def serving_bench(X, seed=5, nstreams=8, dumps_per_stream=2, load=0.7):
     }
 
 
-def profile_only(X, which, steps, warmup):
+def profile_only(X, which, steps, warmup, kernel=0, stage=0):
     """Run one path's workload warmup+steps times (one kernel per step).
     "copy": a plain 1 GiB device-to-device copy per step (counter calibration)."""
     if which == "copy":
@@ -179,6 +179,7 @@ def profile_only(X, which, steps, warmup):
     else:
         buf.fill_uniform_int8(2, -128, 127)
     eng = X.Engine(ns, nf, compute=comp, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    eng.set_tuning(kernel=kernel, stage=stage)
     for _ in range(warmup + steps):
         eng.clear()
         eng.correlate(buf, nt)
@@ -197,6 +198,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--only", choices=["int8", "fp32", "fp16", "copy"], default=None,
                     help="run just this path's workload (for profiling); prints no JSON line")
+    ap.add_argument("--kernel", type=int, default=0, choices=[0, 1, 2],
+                    help="int8 kernel: 1 complex-plane, 2 real-Gram (0 = default)")
+    ap.add_argument("--stage", type=int, default=0, choices=[0, 64, 128],
+                    help="real-Gram kernel samples per LDS stage (0 = default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this X-engine build runs on one GPU; --gpus {args.gpus} is not "
@@ -207,13 +212,17 @@ def main():
     import xgpu_amd as X
 
     if args.only:
-        profile_only(X, args.only, args.steps, args.warmup)
+        profile_only(X, args.only, args.steps, args.warmup, args.kernel, args.stage)
         return
     cmac = X.cmac_per_integration(NS, NF, NT)
     log(f"[bench] int8 MFMA NS={NS} NF={NF} NT={NT}: {cmac:.4e} CMAC per step")
     buf = X.DeviceBuffer(NT * NF * NS * 4)
     buf.fill_uniform_int8(2, -128, 127)
     eng = X.Engine(NS, NF, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    eng.set_tuning(kernel=args.kernel, stage=args.stage)
+    tun = eng.get_tuning()
+    kname = (f"xg_xengine_rg<{tun['stage']}>" if tun["kernel"] == 2
+             else "xg_xengine_mfma<int8>")
 
     for _ in range(args.warmup):
         eng.clear()
@@ -230,7 +239,18 @@ def main():
     # kernel-only time with HIP events on the engine stream
     ms_kernel = eng.time_correlate(buf, NT, warmup=1, reps=args.steps)
     achieved_tops = cmac * 8 / (ms_kernel * 1e-3) / 1e12
-    log(f"[bench] int8: {ms_step:.3f} ms/step wall, {ms_kernel:.3f} ms kernel, {value:.1f} TCMAC/s")
+    log(f"[bench] int8 {kname}: {ms_step:.3f} ms/step wall, {ms_kernel:.3f} ms kernel, "
+        f"{value:.1f} TCMAC/s")
+    # the other int8 kernels on the same input, kernel time only
+    variants = {}
+    for k, st in ((1, 0), (2, 64), (2, 128)):
+        if (k, st) == (tun["kernel"], tun["stage"]) or (k == 1 and tun["kernel"] == 1):
+            continue
+        eng.set_tuning(kernel=k, stage=st)
+        ms_v = eng.time_correlate(buf, NT, warmup=1, reps=5)
+        name = f"xg_xengine_rg<{st}>" if k == 2 else "xg_xengine_mfma<int8>"
+        variants[name] = {"ms": round(ms_v, 4), "tcmac_s": round(cmac / (ms_v * 1e-3) / 1e12, 1)}
+        log(f"[bench] int8 {name}: {ms_v:.3f} ms kernel")
     eng.close()
 
     # FP32 plain kernel on the same workload
@@ -298,10 +318,11 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved_tops / PEAK_INT8_TOPS, 4),
             "traffic": None,
-            "kernel": "xg_xengine_mfma<int8>",
+            "kernel": kname,
             "ops_per_cmac": 8,
             "kernel_ms": round(ms_kernel, 4),
         },
+        "int8_kernel_variants": variants,
         "cpu_baseline": cpu,
         "serving": serving,
     }

@@ -138,7 +138,10 @@ typedef struct XGPUAmdTuningStruct {
   int prefetch;    /* MFMA kernels: 1 = touch the input 2 stages ahead into L2, 2 = off */
   int graph;       /* host-buffer path: 1 = capture each call's transfers and kernels in
                       a hipGraph once and replay it, 2 = issue them one by one */
-  int reserved[5];
+  int kernel;      /* int8: 1 = complex-plane kernel (any nstation), 2 = real-Gram kernel
+                      where it applies (nstation % 4 == 0, 16-byte aligned input; default) */
+  int stage;       /* real-Gram kernel: samples per LDS stage, 64 or 128 (0 = 64) */
+  int reserved[3];
 } XGPUAmdTuning;
 int xgpuAmdSetTuning(XGPUAmdEngine *engine, const XGPUAmdTuning *tuning);
 int xgpuAmdGetTuning(const XGPUAmdEngine *engine, XGPUAmdTuning *tuning);

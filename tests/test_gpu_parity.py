@@ -34,9 +34,11 @@ def to_tri(eng, raw, ns, nf, order):
     return oracle.reorder(order, raw, ns, nf)
 
 
-def run_i8(ns, nf, nt, x, order=X.REGISTER_TILE_TRIANGULAR_ORDER, calls=1):
-    flush(f"[i8] ns={ns} nf={nf} nt={nt} order={order} calls={calls}")
+def run_i8(ns, nf, nt, x, order=X.REGISTER_TILE_TRIANGULAR_ORDER, calls=1, **tuning):
+    flush(f"[i8] ns={ns} nf={nf} nt={nt} order={order} calls={calls} {tuning}")
     eng = X.Engine(ns, nf, nt, compute=X.XGPU_INT8, order=order)
+    if tuning:
+        eng.set_tuning(**tuning)
     buf = X.DeviceBuffer.from_array(x)
     per = nt // calls
     step = per * nf * ns * 4
@@ -60,18 +62,22 @@ def run_i8(ns, nf, nt, x, order=X.REGISTER_TILE_TRIANGULAR_ORDER, calls=1):
     (6, 2, 65, "tri"),            # nstation % 4 == 2
     (256, 4, 1000, "regtile"),
 ])
-def test_i8_matches_exact_oracle(ns, nf, nt, order):
+@pytest.mark.parametrize("kernel,stage", [(1, 0), (2, 64), (2, 128)])
+def test_i8_matches_exact_oracle(ns, nf, nt, order, kernel, stage):
+    """kernel 1: complex-plane int8 MFMA kernel; kernel 2: real-Gram kernel
+    (used where nstation % 4 == 0, else the engine takes kernel 1)."""
     x = oracle.fill_uniform(nt * nf * ns * 4, ns * 1000 + nt, -128, 127)
-    raw, wide = run_i8(ns, nf, nt, x, ORDERS[order])
+    raw, wide = run_i8(ns, nf, nt, x, ORDERS[order], kernel=kernel, stage=stage)
     ref = oracle.xengine_i64(x, ns, nf, nt)
     assert np.array_equal(to_tri(None, raw, ns, nf, ORDERS[order]).astype(np.int64), ref)
     assert np.array_equal(to_tri(None, wide, ns, nf, ORDERS[order]), ref)
 
 
-def test_i8_minus_128_everywhere():
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_i8_minus_128_everywhere(kernel):
     ns, nf, nt = 64, 2, 4096
     x = np.full(nt * nf * ns * 4, -128, dtype=np.int8)
-    raw, wide = run_i8(ns, nf, nt, x, X.TRIANGULAR_ORDER)
+    raw, wide = run_i8(ns, nf, nt, x, X.TRIANGULAR_ORDER, kernel=kernel)
     v = raw.reshape(-1, 2).astype(np.int64)
     assert np.all(v[:, 0] == nt * 2 * 128 * 128)
     assert np.all(v[:, 1] == 0)
@@ -79,8 +85,35 @@ def test_i8_minus_128_everywhere():
     x = np.zeros((nt, nf, ns, 2, 2), dtype=np.int8)
     x[:, :, 0::2] = (-128, 127)
     x[:, :, 1::2] = (127, -128)
-    raw, _ = run_i8(ns, nf, nt, x.reshape(-1), X.TRIANGULAR_ORDER)
+    raw, _ = run_i8(ns, nf, nt, x.reshape(-1), X.TRIANGULAR_ORDER, kernel=kernel)
     assert np.array_equal(raw.astype(np.int64), oracle.xengine_i64(x, ns, nf, nt))
+
+
+def test_i8_int32_bound_block_exact():
+    """One block of exactly 65535 samples of -128 + -128i: every accumulator
+    reaches its int32 extreme (2 * 65535 * 128^2 = 2^31 - 2^15 per
+    autocorrelation) without a spill, in both int8 kernels."""
+    ns, nf, nt = 4, 1, 65535
+    x = np.full(nt * nf * ns * 4, -128, dtype=np.int8)
+    for kernel in (1, 2):
+        raw, wide = run_i8(ns, nf, nt, x, X.TRIANGULAR_ORDER, kernel=kernel)
+        v = raw.reshape(-1, 2).astype(np.int64)
+        assert np.all(v[:, 0] == nt * 2 * 128 * 128) and np.all(v[:, 1] == 0), kernel
+        assert np.array_equal(wide.reshape(-1, 2), v)
+
+
+def test_i8_unaligned_input_falls_back_exact():
+    """A block that is not 16-byte aligned goes to the complex-plane kernel."""
+    ns, nf, nt = 64, 2, 200
+    x = oracle.fill_uniform(nt * nf * ns * 4 + 8, 8, -128, 127)
+    buf = X.DeviceBuffer.from_array(x)
+    flush("[i8] input at +8 bytes")
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
+    eng.correlate(buf.ptr + 8, nt)
+    ref = oracle.xengine_i64(x[8:], ns, nf, nt)
+    assert np.array_equal(eng.dump().astype(np.int64), ref)
+    eng.close()
+    buf.free()
 
 
 def test_i8_accumulates_across_calls_and_clear():
@@ -147,7 +180,8 @@ def test_i8_device_reorder_dump():
 
 def test_i8_full_size_cfg2_sampled():
     """NS=512 x 2 pol, NF=64, NT=8192: exact on sampled baselines of every
-    channel, plus checksums that cover every visibility."""
+    channel, every autocorrelation against the input power, and the two int8
+    kernels (complex-plane and real-Gram) bit-identical on every visibility."""
     ns, nf, nt = 512, 64, 8192
     flush("[i8] cfg2 full size")
     buf = X.DeviceBuffer(nt * nf * ns * 4)
@@ -173,6 +207,13 @@ def test_i8_full_size_cfg2_sampled():
     power = power.reshape(nf, ns, 2, 2).sum(axis=3)  # [f, s, p]
     autos = np.array([tri[:, i * (i + 1) // 2 + i, [0, 3], 0] for i in range(ns)])  # [s, f, p]
     assert np.array_equal(autos.transpose(1, 0, 2), power)
+    flush("[i8] cfg2 full size, kernel 1 vs kernel 2 (stage 64 and 128)")
+    first = eng.dump()
+    for kernel, stage in ((1, 0), (2, 128)):
+        eng.set_tuning(kernel=kernel, stage=stage)
+        eng.clear()
+        eng.correlate(buf, nt)
+        assert np.array_equal(eng.dump(), first), (kernel, stage)
     eng.close()
     buf.free()
 
@@ -304,16 +345,24 @@ def test_autotune_keeps_results_exact(tmp_path):
     assert ms > 0 and best["wg_per_cu"] in (1, 2) and best["prefetch"] in (1, 2)
     eng.correlate(buf, nt)
     assert np.array_equal(eng.dump().astype(np.int64), ref)
+    assert best["kernel"] in (1, 2) and best["stage"] in (64, 128)
     for wg in (1, 2):           # every candidate is exact
         for pf in (1, 2):
-            eng.set_tuning(wg, pf)
+            eng.set_tuning(wg, pf, kernel=1)
             eng.clear()
             eng.correlate(buf, nt)
             assert np.array_equal(eng.dump().astype(np.int64), ref), (wg, pf)
+    for stage in (64, 128):
+        eng.set_tuning(kernel=2, stage=stage)
+        eng.clear()
+        eng.correlate(buf, nt)
+        assert np.array_equal(eng.dump().astype(np.int64), ref), stage
     eng2 = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
     assert eng2.load_tuning(cache, nt)
     assert eng2.get_tuning()["wg_per_cu"] == best["wg_per_cu"]
     assert eng2.get_tuning()["prefetch"] == best["prefetch"]
+    assert eng2.get_tuning()["kernel"] == best["kernel"]
+    assert eng2.get_tuning()["stage"] == best["stage"]
     assert not eng2.load_tuning(cache, nt + 1)
     eng.close()
     eng2.close()

@@ -2,12 +2,14 @@
  * autotune.c -- kernel-parameter autotuner and tuning cache (no xGPU
  * counterpart).
  *
- * Candidates are the engine's kernel knobs (persistent workgroups per CU,
- * L2 prefetch of the MFMA kernels).  Each is timed with HIP events on the
+ * Candidates are the engine's kernel knobs: for int8 the kernel variant
+ * (complex-plane kernel x persistent workgroups per CU x L2 prefetch, and
+ * the real-Gram kernel x LDS stage depth where it applies), for the float
+ * paths workgroups per CU x prefetch.  Each is timed with HIP events on the
  * engine stream over `reps` single-kernel integrations of the caller's
  * device input; the fastest is kept.  The cache is a text file, one line per
  * result:
- *   <device name>|<nstation>|<nfrequency>|<ntime>|<compute>|<wg_per_cu>|<prefetch>|<ms>
+ *   <device>|<nstation>|<nfrequency>|<ntime>|<compute>|<wg_per_cu>|<prefetch>|<kernel>|<stage>|<ms>
  * Later lines win, so re-tuning simply appends.
  */
 #include <stdio.h>
@@ -39,10 +41,13 @@ int xgpuAmdLoadTuning(XGPUAmdEngine *e, const char *path, unsigned ntime) {
   memset(&t, 0, sizeof t);
   while (fgets(line, sizeof line, f)) {
     if (strncmp(line, key, klen) == 0 && line[klen] == '|') {
-      int wg = 0, pf = 0;
-      if (sscanf(line + klen + 1, "%d|%d|", &wg, &pf) == 2) {
+      int wg = 0, pf = 0, kern = 0, stg = 0;
+      float ms = 0.f;
+      if (sscanf(line + klen + 1, "%d|%d|%d|%d|%f", &wg, &pf, &kern, &stg, &ms) == 5) {
         t.wg_per_cu = wg;
         t.prefetch = pf;
+        t.kernel = kern;
+        t.stage = stg;
         found = 1;
       }
     }
@@ -64,26 +69,46 @@ int xgpuAmdAutotune(XGPUAmdEngine *e, const void *input_d, unsigned ntime, int w
   XGPUAmdTuning win = keep;
   float win_ms = 0.f;
   int have = 0;
+  XGPUAmdTuning cand[16];
+  int ncand = 0;
   const int npf = (e->family == XG_KERNEL_F32_PLAIN) ? 1 : 2;
   for (int wg = 1; wg <= 2; ++wg)
     for (int pf = 1; pf <= npf; ++pf) {
       XGPUAmdTuning t = keep;
       t.wg_per_cu = wg;
       t.prefetch = (e->family == XG_KERNEL_F32_PLAIN) ? 2 : pf;
-      int rc = xgpuAmdSetTuning(e, &t);
-      if (rc) return rc;
-      float ms = 0.f;
-      rc = xgpuAmdTimeCorrelate(e, input_d, ntime, warmup, reps, &ms);
-      if (rc) {
-        xgpuAmdSetTuning(e, &keep);
-        return rc;
-      }
-      if (!have || ms < win_ms) {
-        have = 1;
-        win_ms = ms;
-        win = t;
-      }
+      t.kernel = 1;
+      t.stage = keep.stage;
+      cand[ncand++] = t;
     }
+  if (e->family == XG_KERNEL_I8_MFMA) {
+    XGPUAmdTuning probe = keep;
+    probe.kernel = 2;
+    xgpuAmdSetTuning(e, &probe);
+    if (xg_launch_family(e, input_d) == XG_KERNEL_I8_RG)
+      for (int st = 64; st <= 128; st *= 2) {
+        XGPUAmdTuning t = keep;
+        t.kernel = 2;
+        t.stage = st;
+        cand[ncand++] = t;
+      }
+  }
+  for (int c = 0; c < ncand; ++c) {
+    const XGPUAmdTuning t = cand[c];
+    int rc = xgpuAmdSetTuning(e, &t);
+    if (rc) return rc;
+    float ms = 0.f;
+    rc = xgpuAmdTimeCorrelate(e, input_d, ntime, warmup, reps, &ms);
+    if (rc) {
+      xgpuAmdSetTuning(e, &keep);
+      return rc;
+    }
+    if (!have || ms < win_ms) {
+      have = 1;
+      win_ms = ms;
+      win = t;
+    }
+  }
   int rc = xgpuAmdSetTuning(e, &win);
   if (rc) return rc;
   xgpuAmdClear(e);
@@ -94,7 +119,8 @@ int xgpuAmdAutotune(XGPUAmdEngine *e, const void *input_d, unsigned ntime, int w
     if (!f) return xg_fail(XGPU_INVALID_CONFIG, "cannot write tuning cache %s", cache_path);
     char key[256];
     key_of(e, ntime, key, sizeof key);
-    fprintf(f, "%s|%d|%d|%.4f\n", key, win.wg_per_cu, win.prefetch, win_ms);
+    fprintf(f, "%s|%d|%d|%d|%d|%.4f\n", key, win.wg_per_cu, win.prefetch, win.kernel, win.stage,
+            win_ms);
     fclose(f);
   }
   return XGPU_OK;

@@ -161,6 +161,8 @@ int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
   e->cleared = 1;
   e->wg_per_cu = 2;
   e->prefetch = XG_DEFAULT_PREFETCH;
+  e->kernel_pref = 2;
+  e->stage = XG_DEFAULT_RG_STAGE;
   if ((rc = xgh_set_device(cfg->device)) != 0) goto hip_error;
   if ((rc = xgh_device_cu_count(cfg->device, &e->cus)) != 0) goto hip_error;
   if ((rc = xgh_malloc(&e->integ, e->integ_bytes)) != 0) goto hip_error;
@@ -223,9 +225,14 @@ int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
     return xg_fail(XGPU_INVALID_CONFIG, "wg_per_cu must be 0..2");
   if (t->prefetch < 0 || t->prefetch > 2) return xg_fail(XGPU_INVALID_CONFIG, "prefetch must be 0..2");
   if (t->graph < 0 || t->graph > 2) return xg_fail(XGPU_INVALID_CONFIG, "graph must be 0..2");
+  if (t->kernel < 0 || t->kernel > 2) return xg_fail(XGPU_INVALID_CONFIG, "kernel must be 0..2");
+  if (t->stage != 0 && t->stage != 64 && t->stage != 128)
+    return xg_fail(XGPU_INVALID_CONFIG, "stage must be 0, 64 or 128");
   e->use_graph = (t->graph == 1);
   e->wg_per_cu = t->wg_per_cu ? t->wg_per_cu : 2;
   e->prefetch = t->prefetch == 0 ? XG_DEFAULT_PREFETCH : (t->prefetch == 1);
+  e->kernel_pref = t->kernel ? t->kernel : 2;
+  e->stage = t->stage ? t->stage : XG_DEFAULT_RG_STAGE;
   return XGPU_OK;
 }
 
@@ -235,6 +242,8 @@ int xgpuAmdGetTuning(const XGPUAmdEngine *e, XGPUAmdTuning *t) {
   t->wg_per_cu = e->wg_per_cu;
   t->prefetch = e->prefetch ? 1 : 2;
   t->graph = e->use_graph ? 1 : 2;
+  t->kernel = e->kernel_pref;
+  t->stage = e->stage;
   return XGPU_OK;
 }
 
@@ -254,14 +263,31 @@ int xg_spill(XGPUAmdEngine *e) {
   return XGPU_OK;
 }
 
+/* The real-Gram int8 kernel stages 8 inputs (16 bytes) per load with 16-byte
+ * buffer loads: it needs nstation % 4 == 0 and a 16-byte aligned block, and
+ * keeps every in-stage byte offset (< 128 * tstride) below 2^31. */
+int xg_launch_family(const XGPUAmdEngine *e, const void *input_d) {
+  if (e->family != XG_KERNEL_I8_MFMA || e->kernel_pref == 1) return e->family;
+  if (e->cfg.nstation % 4 || ((uintptr_t)input_d & 15)) return XG_KERNEL_I8_MFMA;
+  if (128LL * e->tstride >= (1LL << 31) - 4096) return XG_KERNEL_I8_MFMA;
+  return XG_KERNEL_I8_RG;
+}
+
+/* one workgroup per CU: the real-Gram kernel holds 512 registers per lane */
+static int wg_per_cu_for(const XGPUAmdEngine *e, int family) {
+  return family == XG_KERNEL_I8_RG ? 1 : e->wg_per_cu;
+}
+
 int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime) {
   XgLaunch L;
   memset(&L, 0, sizeof L);
   L.njobs = 1;
+  const int family = xg_launch_family(e, input_d);
   job_fill(&L.jobs[0], e, input_d, ntime, !e->cleared, 0);
-  xg_plan_launch(&L, e->cus, e->wg_per_cu);
-  L.prefetch = (short)e->prefetch;
-  HIPCALL(xgh_launch_xengine(e->family, &L, e->stream), "xengine kernel");
+  xg_plan_launch(&L, e->cus, wg_per_cu_for(e, family));
+  L.prefetch = (char)e->prefetch;
+  L.stage = (char)(e->stage == 128);
+  HIPCALL(xgh_launch_xengine(family, &L, e->stream), "xengine kernel");
   e->cleared = 0;
   e->samples_in_integ += ntime;
   e->samples += ntime;
@@ -542,9 +568,14 @@ int xgpuAmdCorrelateBatch(const XGPUAmdBatchItem *items, int nitems, void *strea
     chan += (int)e->cfg.nfrequency;
   }
   L.njobs = nitems;
-  xg_plan_launch(&L, e0->cus, e0->wg_per_cu);
-  L.prefetch = (short)e0->prefetch;
-  HIPCALL(xgh_launch_xengine(e0->family, &L, s), "batched xengine kernel");
+  /* the real-Gram kernel only when it takes every item */
+  int family = xg_launch_family(e0, items[0].input_d);
+  for (int i = 1; i < nitems; ++i)
+    if (xg_launch_family(items[i].engine, items[i].input_d) != family) family = e0->family;
+  xg_plan_launch(&L, e0->cus, wg_per_cu_for(e0, family));
+  L.prefetch = (char)e0->prefetch;
+  L.stage = (char)(e0->stage == 128);
+  HIPCALL(xgh_launch_xengine(family, &L, s), "batched xengine kernel");
   HIPCALL(xgh_event_record(e0->ev_aux, s), "record");
   for (int i = 0; i < nitems; ++i) {
     XGPUAmdEngine *e = items[i].engine;

@@ -22,6 +22,8 @@ struct XGPUAmdEngineStruct {
   int cus;
   int wg_per_cu;
   int prefetch;      /* MFMA kernels: L2 prefetch 2 stages ahead */
+  int kernel_pref;   /* int8: 1 = complex-plane kernel, 2 = real-Gram when eligible */
+  int stage;         /* real-Gram kernel: samples per LDS stage (64 or 128) */
 
   void *integ;       /* int32 / f32 integration, 2*matLength scalars */
   void *wide;        /* int64 / f64 spill, allocated on first spill */
@@ -55,11 +57,15 @@ struct XGPUAmdEngineStruct {
 
 /* default of the L2 prefetch knob of the MFMA kernels */
 #define XG_DEFAULT_PREFETCH 0
+/* default samples per LDS stage of the real-Gram kernel */
+#define XG_DEFAULT_RG_STAGE 64
 
 int xg_fail(int code, const char *fmt, ...);
 void xg_plan_launch(XgLaunch *L, int cus, int wg_per_cu);
 int xg_spill(XGPUAmdEngine *e);
 int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime);
+/* kernel family that correlates this engine's input at input_d */
+int xg_launch_family(const XGPUAmdEngine *e, const void *input_d);
 /* queue a dump (flags XGPU_DUMP_*) on the engine stream without waiting */
 int xg_dump_async(XGPUAmdEngine *e, void *dst, int flags);
 size_t xg_dump_bytes(const XGPUAmdEngine *e, int flags);

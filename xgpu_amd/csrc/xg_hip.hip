@@ -129,7 +129,7 @@ int xgh_graph_end(void *stream, void **exec) {
   if (rc) return rc;
   hipGraphExec_t x = nullptr;
   rc = check(hipGraphInstantiate(&x, g, nullptr, nullptr, 0), "hipGraphInstantiate");
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   *exec = (void *)x;
   return rc;
 }

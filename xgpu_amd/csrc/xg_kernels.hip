@@ -416,6 +416,215 @@ void xg_xengine_mfma(XgLaunch L) {
 }
 
 // ---------------------------------------------------------------------------
+// Real-Gram int8 MFMA kernel (the default int8 path for nstation % 4 == 0).
+//
+// Each complex input is two real rows (re, im); one int8 MFMA stream
+// accumulates the real Gram matrix G[r][c] = sum_t y_r(t) y_c(t) of the
+// tile's 256 row-reals x 256 column-reals, and the epilogue combines
+//   Re V(m,n) = G[m_re][n_re] + G[m_im][n_im]
+//   Im V(m,n) = G[m_im][n_re] - G[m_re][n_im]
+// in int32: the real and imaginary planes are reordered into two
+// accumulators and subtracted, never negated, so -128 is exact.
+//
+// Data path per stage (STG samples): raw input bytes go global -> VGPR ->
+// LDS unchanged (buffer loads with scalar offsets; the hardware range check
+// returns zeros past the block), one 256-byte LDS row per sample with the
+// 16-byte chunks XOR-swizzled by (sample & 7).  MFMA operands come out of LDS
+// through the gfx950 transpose load ds_read_b64_tr_b8: a 16-lane group
+// supplies 16 eight-byte rows (sample k, half h) and lane j receives byte j of
+// the chunk for 8 consecutive samples -- one real row, time-contiguous.
+// 4 waves of 128x128 reals (4x4 blocks of v_mfma_i32_32x32x32_i8, 256
+// accumulator registers), one workgroup per CU.
+// ---------------------------------------------------------------------------
+typedef int v2i __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) v2i lds_v2i;
+
+#define RG_ROW 256  // LDS bytes per sample per tile side (128 inputs x 2 bytes)
+
+template <int STG>
+struct RgCfg {
+  static constexpr int kSide = STG * RG_ROW;             // bytes per side per stage
+  static constexpr int kBuf = 2 * kSide;                 // rows + columns
+  static constexpr int kLds = 2 * kBuf;                  // double buffered
+  static constexpr int kLoads = STG / 16;                // 16-byte loads per thread per side
+};
+
+// Per-lane LDS offset of a 32-real-row block operand (chunks cb, cb+1 of the
+// 256-byte sample rows) for K samples 0..31 of a kstep.  In a 16-lane group,
+// supplier r gives the 8-byte half (r & 1) of sample (r >> 1) + 8 k; since
+// every sample offset added later is a multiple of 8, the chunk swizzle
+// (sample & 7) is the lane constant r >> 1.
+__device__ __forceinline__ int rg_lane_off(int chunk_base, int lane) {
+  const int g = (lane >> 4) & 1, kg = lane >> 5, r = lane & 15;
+  const int chunk = chunk_base + g;
+  return (kg * 16 + (r >> 1)) * RG_ROW + ((chunk ^ (r >> 1)) << 4) + ((r & 1) << 3);
+}
+
+// Two transpose loads: samples kbase + 16 kg + {0..7} and {8..15}.
+__device__ __forceinline__ v4i rg_operand(const unsigned char *p) {
+  const v2i lo = __builtin_amdgcn_ds_read_tr8_b64_v2i32((lds_v2i *)p);
+  const v2i hi = __builtin_amdgcn_ds_read_tr8_b64_v2i32((lds_v2i *)(p + 8 * RG_ROW));
+  v4i out;
+  out[0] = lo.x;
+  out[1] = lo.y;
+  out[2] = hi.x;
+  out[3] = hi.y;
+  return out;
+}
+
+template <int STG>
+__device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsigned char *lds) {
+  typedef RgCfg<STG> C;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ninput = 2 * J.nstation;
+  const bool diag = (T.rb == T.cb);
+  const int nstage = (J.ntime + STG - 1) / STG;
+
+  // staging: thread -> (chunk, first sample row); load i covers sample
+  // srow + 16 i of the stage.  Each load has its own buffer resource, based at
+  // its first sample row and sized to end at the block's last valid sample, and
+  // a zero scalar offset: the range check (per-lane offset vs size) then zeroes
+  // every sample past the block and every invalid chunk, and all offsets stay
+  // below 16 * tstride.
+  const int chunk = tid & 15, srow = tid >> 4;
+  const unsigned char *in_f = (const unsigned char *)J.input + (long long)T.f * J.nstation * 4;
+  const unsigned char *row_base = in_f + T.rb * RG_ROW;
+  const unsigned char *col_base = in_f + T.cb * RG_ROW;
+  // bytes of a sample row that a valid chunk can touch (<= tstride)
+  const int ts = (int)J.tstride;  // < 2^24: the host keeps 128 * tstride < 2^31
+  const int row_bytes = ts < RG_ROW ? ts : RG_ROW;
+  // chunk covers inputs blk*128 + 8*chunk .. +7 (ninput % 8 == 0 here); an
+  // invalid chunk gets an offset past any resource's size and reads zeros
+  const int voff = (int)(srow * J.tstride + 16 * chunk);
+  const int voff0 = (T.rb * XG_TILE + 8 * chunk < ninput) ? voff : 0x7fffff00;
+  // diagonal tiles read their columns from the row side; the column loads then
+  // fall outside the buffer and return zeros (no branch in the stage loop)
+  const int voff1 = (!diag && T.cb * XG_TILE + 8 * chunk < ninput) ? voff : 0x7fffff00;
+  const int wpos = srow * RG_ROW + ((chunk ^ (srow & 7)) << 4);  // LDS write offset
+
+  v4i r0[C::kLoads], r1[C::kLoads];
+  auto gload = [&](int st) {
+#pragma unroll
+    for (int i = 0; i < C::kLoads; ++i) {
+      const int t0 = st * STG + 16 * i;             // first sample of this load
+      const int rows = min(J.ntime - t0, 16);       // valid samples of the 16
+      // (rows - 1) full sample rows + the last one's reachable bytes; <= 0: none
+      const int nrec = max(rows * ts - ts + row_bytes, 0);
+      const long long boff = (long long)t0 * J.tstride;
+      const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(row_base + boff), (short)0, nrec, 0x00020000);
+      const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(col_base + boff), (short)0, nrec, 0x00020000);
+      r0[i] = __builtin_amdgcn_raw_buffer_load_b128(rs0, voff0, 0, 0);
+      r1[i] = __builtin_amdgcn_raw_buffer_load_b128(rs1, voff1, 0, 0);
+    }
+  };
+  auto lwrite = [&](int buf) {
+    unsigned char *b = lds + buf * C::kBuf;
+#pragma unroll
+    for (int i = 0; i < C::kLoads; ++i) {
+      *(v4i *)(b + wpos + 16 * i * RG_ROW) = r0[i];
+      *(v4i *)(b + C::kSide + wpos + 16 * i * RG_ROW) = r1[i];
+    }
+  };
+
+  // On a diagonal tile the wave above the diagonal (wr < wc) duplicates its
+  // mirror; it runs the same MFMAs anyway (one wave per SIMD, so skipping them
+  // saves no time) and only skips the stores.
+  const int wr = wave >> 1, wc = wave & 1;
+  const bool active = !(diag && wr < wc);
+  int aoff[4], boff[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) aoff[a] = rg_lane_off(8 * wr + 2 * a, lane);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) boff[b] = rg_lane_off(8 * wc + 2 * b, lane) + (diag ? 0 : C::kSide);
+  v16i acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
+
+  gload(0);
+  lwrite(0);
+  __syncthreads();
+  for (int s = 0; s < nstage; ++s) {
+    const int buf = s & 1;
+    const bool more = (s + 1) < nstage;
+    if (more) gload(s + 1);
+    {
+      const unsigned char *base = lds + buf * C::kBuf;
+#pragma unroll
+      for (int ks = 0; ks < STG / 32; ++ks) {
+        v4i A[4], B[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) A[a] = rg_operand(base + aoff[a] + 32 * ks * RG_ROW);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) B[b] = rg_operand(base + boff[b] + 32 * ks * RG_ROW);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[a], B[b], acc[a][b], 0, 0, 0);
+      }
+    }
+    if (more) lwrite(buf ^ 1);
+    __syncthreads();
+  }
+  if (!active) return;
+
+  // Epilogue.  Element e of lane l in block (a,b) is G[R][Cc] with
+  // R = 128wr + 32a + 8(e/4) + 4(l/32) + e%4 and Cc = 128wc + 32b + l%32.
+  // Even lanes (column n_re) finish input row m = R/2 for e%4 in {0,1}, odd
+  // lanes (column n_im) the row m+1 (e%4 in {2,3}); each takes the two
+  // values it lacks from its partner lane.
+  const bool odd = lane & 1;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      // one block at a time: keeps the scheduler from reading every
+      // accumulator into VGPRs up front
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int x0 = acc[a][b][4 * q + 0], x1 = acc[a][b][4 * q + 1];
+        const int x2 = acc[a][b][4 * q + 2], x3 = acc[a][b][4 * q + 3];
+        // each lane sends what its partner lacks: odd lanes their e%4 = 0,1
+        // (column n_im of row m), even lanes their e%4 = 2,3 (column n_re of m+1)
+        const int y0 = __shfl_xor(odd ? x0 : x2, 1);
+        const int y1 = __shfl_xor(odd ? x1 : x3, 1);
+        int re, im;
+        if (!odd) {  // x0 = G[mr][nr], x1 = G[mi][nr], y0 = G[mr][ni], y1 = G[mi][ni]
+          re = (int)((unsigned)x0 + (unsigned)y1);
+          im = (int)((unsigned)x1 - (unsigned)y0);
+        } else {     // x2 = G[m'r][ni], x3 = G[m'i][ni], y0 = G[m'r][nr], y1 = G[m'i][nr]
+          re = (int)((unsigned)y0 + (unsigned)x3);
+          im = (int)((unsigned)y1 - (unsigned)x2);
+        }
+        const int R0 = 128 * wr + 32 * a + 8 * q + 4 * (lane >> 5);  // even real row
+        const int m = T.rb * XG_TILE + (R0 >> 1) + (odd ? 1 : 0);
+        const int n = T.cb * XG_TILE + ((128 * wc + 32 * b + (lane & 31)) >> 1);
+        store_int(J, T.f, m, n, re, im);
+      }
+    }
+}
+
+template <int STG>
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
+void xg_xengine_rg(XgLaunch L) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[RgCfg<STG>::kLds];
+  const int xcd = blockIdx.x % XG_NXCD;
+  const int wl = blockIdx.x / XG_NXCD;
+  TileRef T;
+  for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
+    rg_tile<STG>(L.jobs[T.job], T, lds);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FP32 plain kernel: exact int8 -> float, SIMT FMA.  Thread (ty,tx) of a
 // 16x16 layout owns rows ty+16a and columns tx+16b (a,b < 8) of the
 // 128x128-input tile.  Stage: 16 samples, float2 per input per sample.
@@ -671,6 +880,12 @@ extern "C" int xgh_launch_xengine(int family, const XgLaunch *L, void *stream) {
       break;
     case XG_KERNEL_F32_PLAIN:
       hipLaunchKernelGGL(xg_xengine_f32, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      break;
+    case XG_KERNEL_I8_RG:
+      if (L->stage)
+        hipLaunchKernelGGL(xg_xengine_rg<128>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      else
+        hipLaunchKernelGGL(xg_xengine_rg<64>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       break;
     default:
       return xgh_report(hipErrorInvalidValue, "xgh_launch_xengine(family)");

@@ -40,6 +40,7 @@
 #define XG_KERNEL_I8_MFMA   0
 #define XG_KERNEL_F16_MFMA  1
 #define XG_KERNEL_F32_PLAIN 2
+#define XG_KERNEL_I8_RG     3  /* int8 real-Gram MFMA (nstation % 4 == 0) */
 
 typedef struct XgJob {
   const void *input;   /* sample (t=0,f=0,s=0,p=0) of this block */
@@ -62,7 +63,9 @@ typedef struct XgLaunch {
   XgJob jobs[XG_MAX_JOBS];
   int njobs;
   short wg_per_xcd;                             /* grid = 8 * wg_per_xcd */
-  short prefetch;                               /* MFMA kernels: L2 prefetch 2 stages ahead */
+  char prefetch;                                /* MFMA kernels: L2 prefetch 2 stages ahead */
+  char stage;                                   /* real-Gram kernel: 0 = 64 samples per
+                                                   LDS stage, 1 = 128 */
   int xcd_start[XG_NXCD][XG_MAX_JOBS + 1];      /* per-XCD prefix of tile counts */
 } XgLaunch;
 
