@@ -140,7 +140,7 @@ typedef struct XGPUAmdTuningStruct {
                       a hipGraph once and replay it, 2 = issue them one by one */
   int kernel;      /* int8: 1 = complex-plane kernel (any nstation), 2 = real-Gram kernel
                       where it applies (nstation % 4 == 0, 16-byte aligned input; default) */
-  int stage;       /* real-Gram kernel: samples per LDS stage, 64 or 128 (0 = 64) */
+  int stage;       /* real-Gram kernel: samples per LDS stage, 64 or 128 (0 = 128) */
   int reserved[3];
 } XGPUAmdTuning;
 int xgpuAmdSetTuning(XGPUAmdEngine *engine, const XGPUAmdTuning *tuning);

@@ -58,7 +58,7 @@ struct XGPUAmdEngineStruct {
 /* default of the L2 prefetch knob of the MFMA kernels */
 #define XG_DEFAULT_PREFETCH 0
 /* default samples per LDS stage of the real-Gram kernel */
-#define XG_DEFAULT_RG_STAGE 64
+#define XG_DEFAULT_RG_STAGE 128
 
 int xg_fail(int code, const char *fmt, ...);
 void xg_plan_launch(XgLaunch *L, int cus, int wg_per_cu);

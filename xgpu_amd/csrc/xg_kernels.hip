@@ -426,15 +426,22 @@ void xg_xengine_mfma(XgLaunch L) {
 // in int32: the real and imaginary planes are reordered into two
 // accumulators and subtracted, never negated, so -128 is exact.
 //
-// Data path per stage (STG samples): raw input bytes go global -> VGPR ->
-// LDS unchanged (buffer loads with scalar offsets; the hardware range check
-// returns zeros past the block), one 256-byte LDS row per sample with the
-// 16-byte chunks XOR-swizzled by (sample & 7).  MFMA operands come out of LDS
-// through the gfx950 transpose load ds_read_b64_tr_b8: a 16-lane group
-// supplies 16 eight-byte rows (sample k, half h) and lane j receives byte j of
-// the chunk for 8 consecutive samples -- one real row, time-contiguous.
-// 4 waves of 128x128 reals (4x4 blocks of v_mfma_i32_32x32x32_i8, 256
-// accumulator registers), one workgroup per CU.
+// Data path: raw input bytes go global -> VGPR -> LDS unchanged (16-byte
+// buffer loads; the hardware range check returns zeros past the block and for
+// inputs past the station count), one 256-byte LDS row per sample and tile
+// side.  MFMA operands come out of LDS through the gfx950 transpose load
+// ds_read_b64_tr_b8: a 16-lane group supplies 16 eight-byte rows (sample k,
+// half h) and lane j receives byte j of the chunk for 8 consecutive samples --
+// one real row, time-contiguous.  4 waves of 128x128 reals (4x4 blocks of
+// v_mfma_i32_32x32x32_i8, 256 accumulator registers), one workgroup per CU.
+//
+// Software pipeline (one wave per SIMD, so nothing else hides latency):
+//  * two operand register sets: the 16 transpose reads of kstep k+1 are
+//    interleaved with the 16 MFMAs of kstep k;
+//  * one barrier per stage, placed before the last kstep, whose MFMAs then
+//    cover the reads of the next stage's first kstep;
+//  * the global loads of stage s+2 are issued right after that barrier and
+//    written to LDS one stage later, interleaved with MFMAs.
 // ---------------------------------------------------------------------------
 typedef int v2i __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) v2i lds_v2i;
@@ -447,17 +454,26 @@ struct RgCfg {
   static constexpr int kBuf = 2 * kSide;                 // rows + columns
   static constexpr int kLds = 2 * kBuf;                  // double buffered
   static constexpr int kLoads = STG / 16;                // 16-byte loads per thread per side
+  static constexpr int kSteps = STG / 32;                // MFMA ksteps per stage
 };
+
+// 16-byte slot of chunk c in a sample row with (sample & 7) == s7.  The 16
+// chunks of a row fill the 16 slots (staging writes are conflict free), and a
+// transpose read of chunks (c, c+1), c even, over 8 consecutive samples lands
+// on 8 slots of each half of the row: all 64 banks.
+__device__ __forceinline__ int rg_slot(int chunk, int s7) {
+  return chunk ^ s7 ^ ((chunk & 1) << 3);
+}
 
 // Per-lane LDS offset of a 32-real-row block operand (chunks cb, cb+1 of the
 // 256-byte sample rows) for K samples 0..31 of a kstep.  In a 16-lane group,
 // supplier r gives the 8-byte half (r & 1) of sample (r >> 1) + 8 k; since
-// every sample offset added later is a multiple of 8, the chunk swizzle
-// (sample & 7) is the lane constant r >> 1.
+// every sample offset added later is a multiple of 8, (sample & 7) is the
+// lane constant r >> 1.
 __device__ __forceinline__ int rg_lane_off(int chunk_base, int lane) {
   const int g = (lane >> 4) & 1, kg = lane >> 5, r = lane & 15;
   const int chunk = chunk_base + g;
-  return (kg * 16 + (r >> 1)) * RG_ROW + ((chunk ^ (r >> 1)) << 4) + ((r & 1) << 3);
+  return (kg * 16 + (r >> 1)) * RG_ROW + (rg_slot(chunk, r >> 1) << 4) + ((r & 1) << 3);
 }
 
 // Two transpose loads: samples kbase + 16 kg + {0..7} and {8..15}.
@@ -472,6 +488,24 @@ __device__ __forceinline__ v4i rg_operand(const unsigned char *p) {
   return out;
 }
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations
+// (not its global loads, which stay in flight across it).  The memory clobber
+// keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void rg_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <typename T, T V>
+struct integral_c {
+  static constexpr T value = V;
+};
+
+// scheduling-group masks (llvm.amdgcn.sched.group.barrier)
+#define SG_MFMA 0x008
+#define SG_VMEM_READ 0x020
+#define SG_DS_READ 0x100
+#define SG_DS_WRITE 0x200
+
 template <int STG>
 __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsigned char *lds) {
   typedef RgCfg<STG> C;
@@ -480,37 +514,44 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
   const bool diag = (T.rb == T.cb);
   const int nstage = (J.ntime + STG - 1) / STG;
 
-  // staging: thread -> (chunk, first sample row); load i covers sample
+  // Staging: thread -> (chunk, first sample row); load i covers sample
   // srow + 16 i of the stage.  Each load has its own buffer resource, based at
   // its first sample row and sized to end at the block's last valid sample, and
   // a zero scalar offset: the range check (per-lane offset vs size) then zeroes
   // every sample past the block and every invalid chunk, and all offsets stay
   // below 16 * tstride.
+  //
+  // Job fields live in registers: the barriers' memory clobber would otherwise
+  // make the compiler reload them from the kernel arguments every stage.
+  const int ntime = J.ntime;
+  const int ts = (int)J.tstride;  // < 2^24: the host keeps 128 * tstride < 2^31
   const int chunk = tid & 15, srow = tid >> 4;
   const unsigned char *in_f = (const unsigned char *)J.input + (long long)T.f * J.nstation * 4;
   const unsigned char *row_base = in_f + T.rb * RG_ROW;
   const unsigned char *col_base = in_f + T.cb * RG_ROW;
+  const long long load_step = 16LL * ts;           // bytes between loads i and i+1
+  const long long stage_step = (long long)STG * ts;  // bytes between stages
   // bytes of a sample row that a valid chunk can touch (<= tstride)
-  const int ts = (int)J.tstride;  // < 2^24: the host keeps 128 * tstride < 2^31
   const int row_bytes = ts < RG_ROW ? ts : RG_ROW;
   // chunk covers inputs blk*128 + 8*chunk .. +7 (ninput % 8 == 0 here); an
   // invalid chunk gets an offset past any resource's size and reads zeros
-  const int voff = (int)(srow * J.tstride + 16 * chunk);
+  const int voff = srow * ts + 16 * chunk;
   const int voff0 = (T.rb * XG_TILE + 8 * chunk < ninput) ? voff : 0x7fffff00;
   // diagonal tiles read their columns from the row side; the column loads then
   // fall outside the buffer and return zeros (no branch in the stage loop)
   const int voff1 = (!diag && T.cb * XG_TILE + 8 * chunk < ninput) ? voff : 0x7fffff00;
-  const int wpos = srow * RG_ROW + ((chunk ^ (srow & 7)) << 4);  // LDS write offset
+  const int wpos = srow * RG_ROW + (rg_slot(chunk, srow & 7) << 4);  // LDS write offset
 
   v4i r0[C::kLoads], r1[C::kLoads];
   auto gload = [&](int st) {
+    const long long sbase = st * stage_step;
 #pragma unroll
     for (int i = 0; i < C::kLoads; ++i) {
       const int t0 = st * STG + 16 * i;             // first sample of this load
-      const int rows = min(J.ntime - t0, 16);       // valid samples of the 16
+      const int rows = min(ntime - t0, 16);         // valid samples of the 16
       // (rows - 1) full sample rows + the last one's reachable bytes; <= 0: none
       const int nrec = max(rows * ts - ts + row_bytes, 0);
-      const long long boff = (long long)t0 * J.tstride;
+      const long long boff = sbase + i * load_step;
       const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
           (void *)(row_base + boff), (short)0, nrec, 0x00020000);
       const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(
@@ -546,31 +587,82 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
 
+  v4i A0[4], B0[4], A1[4], B1[4];
+  auto lread = [&](const unsigned char *base, v4i *A, v4i *B) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) A[a] = rg_operand(base + aoff[a]);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) B[b] = rg_operand(base + boff[b]);
+  };
+  // One kstep: the 16 MFMAs on (A, B), with the 16 transpose reads of the next
+  // kstep into (NA, NB) over the first 8 MFMAs (when READ) and `fill` (global
+  // loads or LDS staging writes, FILL's scheduling class, 2 per MFMA) over the
+  // last 8.
+  auto kstep = [&](auto read_c, auto fill_c, const v4i *A, const v4i *B, v4i *NA, v4i *NB,
+                   const unsigned char *nbase, auto fill) {
+    constexpr bool READ = decltype(read_c)::value;
+    constexpr int FILL = decltype(fill_c)::value;
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      acc[i >> 2][i & 3] =
+          __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i >> 2], B[i & 3], acc[i >> 2][i & 3], 0, 0, 0);
+    if constexpr (READ) lread(nbase, NA, NB);
+    fill();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      __builtin_amdgcn_sched_group_barrier(SG_MFMA, 1, 0);
+      if constexpr (READ) __builtin_amdgcn_sched_group_barrier(SG_DS_READ, 2, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      __builtin_amdgcn_sched_group_barrier(SG_MFMA, 1, 0);
+      if constexpr (FILL != 0) __builtin_amdgcn_sched_group_barrier(FILL, 2, 0);
+    }
+  };
+  typedef integral_c<bool, true> yes;
+  typedef integral_c<int, 0> nofill;
+  auto none = [] {};
+
+  // Prologue: stage 0 in LDS, stage 1 in flight, kstep 0 operands in set 0.
+  // The stage body below has no branches (a branch around the MFMAs makes the
+  // compiler copy the accumulators): past the last stage, the global loads
+  // have an empty range and read zeros without touching memory, and the
+  // staging write and operand reads go to the idle LDS buffer.
   gload(0);
   lwrite(0);
-  __syncthreads();
+  gload(1);
+  rg_barrier();
+  lread(lds, A0, B0);
   for (int s = 0; s < nstage; ++s) {
     const int buf = s & 1;
-    const bool more = (s + 1) < nstage;
-    if (more) gload(s + 1);
-    {
-      const unsigned char *base = lds + buf * C::kBuf;
+    const unsigned char *cur = lds + buf * C::kBuf;
+    const unsigned char *nxt = lds + (buf ^ 1) * C::kBuf;
+    // ksteps 0 .. kSteps-3 of this stage
 #pragma unroll
-      for (int ks = 0; ks < STG / 32; ++ks) {
-        v4i A[4], B[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) A[a] = rg_operand(base + aoff[a] + 32 * ks * RG_ROW);
-#pragma unroll
-        for (int b = 0; b < 4; ++b) B[b] = rg_operand(base + boff[b] + 32 * ks * RG_ROW);
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b)
-            acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[a], B[b], acc[a][b], 0, 0, 0);
-      }
+    for (int ks = 0; ks < C::kSteps - 2; ++ks) {
+      if (ks & 1)
+        kstep(yes(), nofill(), A1, B1, A0, B0, cur + 32 * (ks + 1) * RG_ROW, none);
+      else
+        kstep(yes(), nofill(), A0, B0, A1, B1, cur + 32 * (ks + 1) * RG_ROW, none);
     }
-    if (more) lwrite(buf ^ 1);
-    __syncthreads();
+    // kstep kSteps-2 also writes the next stage's staging registers into the
+    // other LDS buffer
+    constexpr int kw = C::kSteps - 2;
+    auto wr_next = [&] { lwrite(buf ^ 1); };
+    if constexpr (kw & 1)
+      kstep(yes(), integral_c<int, SG_DS_WRITE>(), A1, B1, A0, B0, cur + 32 * (kw + 1) * RG_ROW,
+            wr_next);
+    else
+      kstep(yes(), integral_c<int, SG_DS_WRITE>(), A0, B0, A1, B1, cur + 32 * (kw + 1) * RG_ROW,
+            wr_next);
+    rg_barrier();  // next stage visible in LDS; this stage's buffer is free
+    // last kstep of this stage: reads kstep 0 of the next stage into set 0
+    // and issues the global loads of stage s+2
+    auto ld_next = [&] { gload(s + 2); };
+    if constexpr ((C::kSteps - 1) & 1)
+      kstep(yes(), integral_c<int, SG_VMEM_READ>(), A1, B1, A0, B0, nxt, ld_next);
+    else
+      kstep(yes(), integral_c<int, SG_VMEM_READ>(), A0, B0, A1, B1, nxt, ld_next);
   }
   if (!active) return;
 
@@ -620,7 +712,7 @@ void xg_xengine_rg(XgLaunch L) {
   TileRef T;
   for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
     rg_tile<STG>(L.jobs[T.job], T, lds);
-    __syncthreads();
+    rg_barrier();  // every wave is done with this tile's LDS buffers
   }
 }
 
