@@ -157,7 +157,16 @@ def serving_bench(X, seed=5, nstreams=8, dumps_per_stream=2, load=0.7):
     }
 
 
-def profile_only(X, which, steps, warmup, kernel=0, stage=0):
+def kernel_name(tun):
+    """Kernel symbol of an int8 tuning (as rocprofv3 lists it)."""
+    if tun["kernel"] != 2:
+        return "xg_xengine_mfma<int8>"
+    if tun["staging"] == 2:  # 128 KiB of LDS stage buffers
+        return f"xg_xengine_rgd<{tun['stage']}, {65536 // (tun['stage'] * 256)}>"
+    return f"xg_xengine_rg<{tun['stage']}>"
+
+
+def profile_only(X, which, steps, warmup, kernel=0, stage=0, staging=0):
     """Run one path's workload warmup+steps times (one kernel per step).
     "copy": a plain 1 GiB device-to-device copy per step (counter calibration)."""
     if which == "copy":
@@ -179,7 +188,8 @@ def profile_only(X, which, steps, warmup, kernel=0, stage=0):
     else:
         buf.fill_uniform_int8(2, -128, 127)
     eng = X.Engine(ns, nf, compute=comp, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
-    eng.set_tuning(kernel=kernel, stage=stage)
+    if comp == X.XGPU_INT8:
+        eng.set_tuning(kernel=kernel, stage=stage, staging=staging)
     for _ in range(warmup + steps):
         eng.clear()
         eng.correlate(buf, nt)
@@ -200,7 +210,9 @@ def main():
                     help="run just this path's workload (for profiling); prints no JSON line")
     ap.add_argument("--kernel", type=int, default=0, choices=[0, 1, 2],
                     help="int8 kernel: 1 complex-plane, 2 real-Gram (0 = default)")
-    ap.add_argument("--stage", type=int, default=0, choices=[0, 64, 128],
+    ap.add_argument("--staging", type=int, default=0, choices=[0, 1, 2],
+                    help="real-Gram kernel staging: 1 registers, 2 global->LDS DMA (0 = default)")
+    ap.add_argument("--stage", type=int, default=0, choices=[0, 32, 64, 128],
                     help="real-Gram kernel samples per LDS stage (0 = default)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -212,17 +224,16 @@ def main():
     import xgpu_amd as X
 
     if args.only:
-        profile_only(X, args.only, args.steps, args.warmup, args.kernel, args.stage)
+        profile_only(X, args.only, args.steps, args.warmup, args.kernel, args.stage, args.staging)
         return
     cmac = X.cmac_per_integration(NS, NF, NT)
     log(f"[bench] int8 MFMA NS={NS} NF={NF} NT={NT}: {cmac:.4e} CMAC per step")
     buf = X.DeviceBuffer(NT * NF * NS * 4)
     buf.fill_uniform_int8(2, -128, 127)
     eng = X.Engine(NS, NF, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
-    eng.set_tuning(kernel=args.kernel, stage=args.stage)
+    eng.set_tuning(kernel=args.kernel, stage=args.stage, staging=args.staging)
     tun = eng.get_tuning()
-    kname = (f"xg_xengine_rg<{tun['stage']}>" if tun["kernel"] == 2
-             else "xg_xengine_mfma<int8>")
+    kname = kernel_name(tun)
 
     for _ in range(args.warmup):
         eng.clear()
@@ -243,12 +254,12 @@ def main():
         f"{value:.1f} TCMAC/s")
     # the other int8 kernels on the same input, kernel time only
     variants = {}
-    for k, st in ((1, 0), (2, 64), (2, 128)):
-        if (k, st) == (tun["kernel"], tun["stage"]) or (k == 1 and tun["kernel"] == 1):
+    for k, stg, st in ((1, 0, 0), (2, 1, 64), (2, 1, 128), (2, 2, 32), (2, 2, 64)):
+        name = kernel_name({"kernel": k, "staging": stg, "stage": st})
+        if name == kname:
             continue
-        eng.set_tuning(kernel=k, stage=st)
+        eng.set_tuning(kernel=k, staging=stg, stage=st)
         ms_v = eng.time_correlate(buf, NT, warmup=1, reps=5)
-        name = f"xg_xengine_rg<{st}>" if k == 2 else "xg_xengine_mfma<int8>"
         variants[name] = {"ms": round(ms_v, 4), "tcmac_s": round(cmac / (ms_v * 1e-3) / 1e12, 1)}
         log(f"[bench] int8 {name}: {ms_v:.3f} ms kernel")
     eng.close()

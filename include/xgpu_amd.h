@@ -140,8 +140,11 @@ typedef struct XGPUAmdTuningStruct {
                       a hipGraph once and replay it, 2 = issue them one by one */
   int kernel;      /* int8: 1 = complex-plane kernel (any nstation), 2 = real-Gram kernel
                       where it applies (nstation % 4 == 0, 16-byte aligned input; default) */
-  int stage;       /* real-Gram kernel: samples per LDS stage, 64 or 128 (0 = 128) */
-  int reserved[3];
+  int stage;       /* real-Gram kernel: samples per LDS stage: 64 or 128 with register
+                      staging, 32 or 64 with DMA staging (0 = default) */
+  int staging;     /* real-Gram kernel: 1 = global -> VGPR -> LDS, 2 = global -> LDS DMA
+                      (0 = default) */
+  int reserved[2];
 } XGPUAmdTuning;
 int xgpuAmdSetTuning(XGPUAmdEngine *engine, const XGPUAmdTuning *tuning);
 int xgpuAmdGetTuning(const XGPUAmdEngine *engine, XGPUAmdTuning *tuning);

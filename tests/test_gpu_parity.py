@@ -52,6 +52,11 @@ def run_i8(ns, nf, nt, x, order=X.REGISTER_TILE_TRIANGULAR_ORDER, calls=1, **tun
     return raw, wide
 
 
+# (kernel, staging, stage): complex-plane kernel; real-Gram kernel with register
+# staging (64 / 128 samples per stage) and with global->LDS DMA (32 / 64)
+I8_VARIANTS = [(1, 0, 0), (2, 1, 64), (2, 1, 128), (2, 2, 32), (2, 2, 64)]
+
+
 @pytest.mark.parametrize("ns,nf,nt,order", [
     (16, 2, 64, "regtile"),
     (64, 3, 128, "regtile"),
@@ -62,22 +67,23 @@ def run_i8(ns, nf, nt, x, order=X.REGISTER_TILE_TRIANGULAR_ORDER, calls=1, **tun
     (6, 2, 65, "tri"),            # nstation % 4 == 2
     (256, 4, 1000, "regtile"),
 ])
-@pytest.mark.parametrize("kernel,stage", [(1, 0), (2, 64), (2, 128)])
-def test_i8_matches_exact_oracle(ns, nf, nt, order, kernel, stage):
+@pytest.mark.parametrize("kernel,staging,stage", I8_VARIANTS)
+def test_i8_matches_exact_oracle(ns, nf, nt, order, kernel, staging, stage):
     """kernel 1: complex-plane int8 MFMA kernel; kernel 2: real-Gram kernel
-    (used where nstation % 4 == 0, else the engine takes kernel 1)."""
+    (used where nstation % 4 == 0, else the engine takes kernel 1) with
+    register (1) or global->LDS DMA (2) staging."""
     x = oracle.fill_uniform(nt * nf * ns * 4, ns * 1000 + nt, -128, 127)
-    raw, wide = run_i8(ns, nf, nt, x, ORDERS[order], kernel=kernel, stage=stage)
+    raw, wide = run_i8(ns, nf, nt, x, ORDERS[order], kernel=kernel, staging=staging, stage=stage)
     ref = oracle.xengine_i64(x, ns, nf, nt)
     assert np.array_equal(to_tri(None, raw, ns, nf, ORDERS[order]).astype(np.int64), ref)
     assert np.array_equal(to_tri(None, wide, ns, nf, ORDERS[order]), ref)
 
 
-@pytest.mark.parametrize("kernel", [1, 2])
-def test_i8_minus_128_everywhere(kernel):
+@pytest.mark.parametrize("kernel,staging", [(1, 0), (2, 1), (2, 2)])
+def test_i8_minus_128_everywhere(kernel, staging):
     ns, nf, nt = 64, 2, 4096
     x = np.full(nt * nf * ns * 4, -128, dtype=np.int8)
-    raw, wide = run_i8(ns, nf, nt, x, X.TRIANGULAR_ORDER, kernel=kernel)
+    raw, wide = run_i8(ns, nf, nt, x, X.TRIANGULAR_ORDER, kernel=kernel, staging=staging)
     v = raw.reshape(-1, 2).astype(np.int64)
     assert np.all(v[:, 0] == nt * 2 * 128 * 128)
     assert np.all(v[:, 1] == 0)
@@ -85,7 +91,7 @@ def test_i8_minus_128_everywhere(kernel):
     x = np.zeros((nt, nf, ns, 2, 2), dtype=np.int8)
     x[:, :, 0::2] = (-128, 127)
     x[:, :, 1::2] = (127, -128)
-    raw, _ = run_i8(ns, nf, nt, x.reshape(-1), X.TRIANGULAR_ORDER, kernel=kernel)
+    raw, _ = run_i8(ns, nf, nt, x.reshape(-1), X.TRIANGULAR_ORDER, kernel=kernel, staging=staging)
     assert np.array_equal(raw.astype(np.int64), oracle.xengine_i64(x, ns, nf, nt))
 
 
@@ -95,10 +101,10 @@ def test_i8_int32_bound_block_exact():
     autocorrelation) without a spill, in both int8 kernels."""
     ns, nf, nt = 4, 1, 65535
     x = np.full(nt * nf * ns * 4, -128, dtype=np.int8)
-    for kernel in (1, 2):
-        raw, wide = run_i8(ns, nf, nt, x, X.TRIANGULAR_ORDER, kernel=kernel)
+    for kernel, staging in ((1, 0), (2, 1), (2, 2)):
+        raw, wide = run_i8(ns, nf, nt, x, X.TRIANGULAR_ORDER, kernel=kernel, staging=staging)
         v = raw.reshape(-1, 2).astype(np.int64)
-        assert np.all(v[:, 0] == nt * 2 * 128 * 128) and np.all(v[:, 1] == 0), kernel
+        assert np.all(v[:, 0] == nt * 2 * 128 * 128) and np.all(v[:, 1] == 0), (kernel, staging)
         assert np.array_equal(wide.reshape(-1, 2), v)
 
 
@@ -207,13 +213,13 @@ def test_i8_full_size_cfg2_sampled():
     power = power.reshape(nf, ns, 2, 2).sum(axis=3)  # [f, s, p]
     autos = np.array([tri[:, i * (i + 1) // 2 + i, [0, 3], 0] for i in range(ns)])  # [s, f, p]
     assert np.array_equal(autos.transpose(1, 0, 2), power)
-    flush("[i8] cfg2 full size, kernel 1 vs kernel 2 (stage 64 and 128)")
+    flush("[i8] cfg2 full size, every int8 kernel variant against the default")
     first = eng.dump()
-    for kernel, stage in ((1, 0), (2, 128)):
-        eng.set_tuning(kernel=kernel, stage=stage)
+    for kernel, staging, stage in I8_VARIANTS:
+        eng.set_tuning(kernel=kernel, staging=staging, stage=stage)
         eng.clear()
         eng.correlate(buf, nt)
-        assert np.array_equal(eng.dump(), first), (kernel, stage)
+        assert np.array_equal(eng.dump(), first), (kernel, staging, stage)
     eng.close()
     buf.free()
 
@@ -345,24 +351,25 @@ def test_autotune_keeps_results_exact(tmp_path):
     assert ms > 0 and best["wg_per_cu"] in (1, 2) and best["prefetch"] in (1, 2)
     eng.correlate(buf, nt)
     assert np.array_equal(eng.dump().astype(np.int64), ref)
-    assert best["kernel"] in (1, 2) and best["stage"] in (64, 128)
+    assert best["kernel"] in (1, 2) and best["stage"] in (0, 32, 64, 128)
     for wg in (1, 2):           # every candidate is exact
         for pf in (1, 2):
             eng.set_tuning(wg, pf, kernel=1)
             eng.clear()
             eng.correlate(buf, nt)
             assert np.array_equal(eng.dump().astype(np.int64), ref), (wg, pf)
-    for stage in (64, 128):
-        eng.set_tuning(kernel=2, stage=stage)
+    for _, staging, stage in I8_VARIANTS[1:]:
+        eng.set_tuning(kernel=2, staging=staging, stage=stage)
         eng.clear()
         eng.correlate(buf, nt)
-        assert np.array_equal(eng.dump().astype(np.int64), ref), stage
+        assert np.array_equal(eng.dump().astype(np.int64), ref), (staging, stage)
     eng2 = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
     assert eng2.load_tuning(cache, nt)
     assert eng2.get_tuning()["wg_per_cu"] == best["wg_per_cu"]
     assert eng2.get_tuning()["prefetch"] == best["prefetch"]
     assert eng2.get_tuning()["kernel"] == best["kernel"]
-    assert eng2.get_tuning()["stage"] == best["stage"]
+    assert eng2.get_tuning()["stage"] == best["stage"] or best["kernel"] == 1
+    assert eng2.get_tuning()["staging"] == best["staging"] or best["kernel"] == 1
     assert not eng2.load_tuning(cache, nt + 1)
     eng.close()
     eng2.close()

@@ -157,20 +157,21 @@ class Engine:
         return float(ms.value)
 
     def set_tuning(self, wg_per_cu: int = 0, prefetch: int = 0, graph: int = 0,
-                   kernel: int = 0, stage: int = 0) -> None:
+                   kernel: int = 0, stage: int = 0, staging: int = 0) -> None:
         """wg_per_cu: 1 or 2; prefetch / graph: 1 on, 2 off; kernel (int8): 1
-        complex-plane, 2 real-Gram where it applies; stage: 64 or 128 samples
-        per LDS stage of the real-Gram kernel (0 = default everywhere)."""
+        complex-plane, 2 real-Gram where it applies; staging (real-Gram): 1
+        registers, 2 global->LDS DMA; stage: samples per LDS stage (64 / 128
+        with register staging, 32 / 64 with DMA) (0 = default everywhere)."""
         t = _lib.XGPUAmdTuning()
         t.wg_per_cu, t.prefetch, t.graph = wg_per_cu, prefetch, graph
-        t.kernel, t.stage = kernel, stage
+        t.kernel, t.stage, t.staging = kernel, stage, staging
         check(core().xgpuAmdSetTuning(self.handle, C.byref(t)))
 
     def get_tuning(self) -> dict:
         t = _lib.XGPUAmdTuning()
         check(core().xgpuAmdGetTuning(self.handle, C.byref(t)))
         return {"wg_per_cu": t.wg_per_cu, "prefetch": t.prefetch, "graph": t.graph,
-                "kernel": t.kernel, "stage": t.stage}
+                "kernel": t.kernel, "stage": t.stage, "staging": t.staging}
 
     def autotune(self, input_d, ntime: int, warmup: int = 1, reps: int = 3,
                  cache_path: str | None = None):
@@ -182,7 +183,7 @@ class Engine:
                                      cache_path.encode() if cache_path else None,
                                      C.byref(t), C.byref(ms)))
         return ({"wg_per_cu": t.wg_per_cu, "prefetch": t.prefetch, "kernel": t.kernel,
-                 "stage": t.stage}, float(ms.value))
+                 "stage": t.stage, "staging": t.staging}, float(ms.value))
 
     def load_tuning(self, cache_path: str, ntime: int) -> bool:
         return core().xgpuAmdLoadTuning(self.handle, cache_path.encode(), ntime) == 0

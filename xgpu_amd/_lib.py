@@ -76,7 +76,8 @@ class XGPUAmdBatchItem(C.Structure):
 
 class XGPUAmdTuning(C.Structure):
     _fields_ = [("wg_per_cu", C.c_int), ("prefetch", C.c_int), ("graph", C.c_int),
-                ("kernel", C.c_int), ("stage", C.c_int), ("reserved", C.c_int * 3)]
+                ("kernel", C.c_int), ("stage", C.c_int), ("staging", C.c_int),
+                ("reserved", C.c_int * 2)]
 
 
 class XGPUServerConfig(C.Structure):

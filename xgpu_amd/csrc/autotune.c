@@ -4,12 +4,13 @@
  *
  * Candidates are the engine's kernel knobs: for int8 the kernel variant
  * (complex-plane kernel x persistent workgroups per CU x L2 prefetch, and
- * the real-Gram kernel x LDS stage depth where it applies), for the float
+ * the real-Gram kernel x staging path (registers or global->LDS DMA) x LDS
+ * stage depth where it applies), for the float
  * paths workgroups per CU x prefetch.  Each is timed with HIP events on the
  * engine stream over `reps` single-kernel integrations of the caller's
  * device input; the fastest is kept.  The cache is a text file, one line per
  * result:
- *   <device>|<nstation>|<nfrequency>|<ntime>|<compute>|<wg_per_cu>|<prefetch>|<kernel>|<stage>|<ms>
+ *   <device>|<ns>|<nf>|<ntime>|<compute>|<wg_per_cu>|<prefetch>|<kernel>|<stage>|<staging>|<ms>
  * Later lines win, so re-tuning simply appends.
  */
 #include <stdio.h>
@@ -41,13 +42,14 @@ int xgpuAmdLoadTuning(XGPUAmdEngine *e, const char *path, unsigned ntime) {
   memset(&t, 0, sizeof t);
   while (fgets(line, sizeof line, f)) {
     if (strncmp(line, key, klen) == 0 && line[klen] == '|') {
-      int wg = 0, pf = 0, kern = 0, stg = 0;
+      int wg = 0, pf = 0, kern = 0, stg = 0, stgg = 0;
       float ms = 0.f;
-      if (sscanf(line + klen + 1, "%d|%d|%d|%d|%f", &wg, &pf, &kern, &stg, &ms) == 5) {
+      if (sscanf(line + klen + 1, "%d|%d|%d|%d|%d|%f", &wg, &pf, &kern, &stg, &stgg, &ms) == 6) {
         t.wg_per_cu = wg;
         t.prefetch = pf;
         t.kernel = kern;
         t.stage = stg;
+        t.staging = stgg;
         found = 1;
       }
     }
@@ -78,20 +80,24 @@ int xgpuAmdAutotune(XGPUAmdEngine *e, const void *input_d, unsigned ntime, int w
       t.wg_per_cu = wg;
       t.prefetch = (e->family == XG_KERNEL_F32_PLAIN) ? 2 : pf;
       t.kernel = 1;
-      t.stage = keep.stage;
+      t.stage = 0;
+      t.staging = 0;
       cand[ncand++] = t;
     }
   if (e->family == XG_KERNEL_I8_MFMA) {
     XGPUAmdTuning probe = keep;
     probe.kernel = 2;
     xgpuAmdSetTuning(e, &probe);
-    if (xg_launch_family(e, input_d) == XG_KERNEL_I8_RG)
-      for (int st = 64; st <= 128; st *= 2) {
+    if (xg_launch_family(e, input_d) == XG_KERNEL_I8_RG) {
+      static const int variants[4][2] = {{1, 64}, {1, 128}, {2, 32}, {2, 64}};  /* staging, stage */
+      for (int v = 0; v < 4; ++v) {
         XGPUAmdTuning t = keep;
         t.kernel = 2;
-        t.stage = st;
+        t.staging = variants[v][0];
+        t.stage = variants[v][1];
         cand[ncand++] = t;
       }
+    }
   }
   for (int c = 0; c < ncand; ++c) {
     const XGPUAmdTuning t = cand[c];
@@ -119,8 +125,8 @@ int xgpuAmdAutotune(XGPUAmdEngine *e, const void *input_d, unsigned ntime, int w
     if (!f) return xg_fail(XGPU_INVALID_CONFIG, "cannot write tuning cache %s", cache_path);
     char key[256];
     key_of(e, ntime, key, sizeof key);
-    fprintf(f, "%s|%d|%d|%d|%d|%.4f\n", key, win.wg_per_cu, win.prefetch, win.kernel, win.stage,
-            win_ms);
+    fprintf(f, "%s|%d|%d|%d|%d|%d|%.4f\n", key, win.wg_per_cu, win.prefetch, win.kernel,
+            win.stage, win.staging, win_ms);
     fclose(f);
   }
   return XGPU_OK;

@@ -163,6 +163,7 @@ int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
   e->prefetch = XG_DEFAULT_PREFETCH;
   e->kernel_pref = 2;
   e->stage = XG_DEFAULT_RG_STAGE;
+  e->staging = XG_DEFAULT_RG_STAGING;
   if ((rc = xgh_set_device(cfg->device)) != 0) goto hip_error;
   if ((rc = xgh_device_cu_count(cfg->device, &e->cus)) != 0) goto hip_error;
   if ((rc = xgh_malloc(&e->integ, e->integ_bytes)) != 0) goto hip_error;
@@ -226,13 +227,18 @@ int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
   if (t->prefetch < 0 || t->prefetch > 2) return xg_fail(XGPU_INVALID_CONFIG, "prefetch must be 0..2");
   if (t->graph < 0 || t->graph > 2) return xg_fail(XGPU_INVALID_CONFIG, "graph must be 0..2");
   if (t->kernel < 0 || t->kernel > 2) return xg_fail(XGPU_INVALID_CONFIG, "kernel must be 0..2");
-  if (t->stage != 0 && t->stage != 64 && t->stage != 128)
-    return xg_fail(XGPU_INVALID_CONFIG, "stage must be 0, 64 or 128");
+  if (t->staging < 0 || t->staging > 2) return xg_fail(XGPU_INVALID_CONFIG, "staging must be 0..2");
+  const int staging = t->staging ? t->staging : XG_DEFAULT_RG_STAGING;
+  if (t->stage != 0 && !(staging == 1 && (t->stage == 64 || t->stage == 128)) &&
+      !(staging == 2 && (t->stage == 32 || t->stage == 64)))
+    return xg_fail(XGPU_INVALID_CONFIG,
+                   "stage must be 0, or 64 / 128 with register staging, 32 / 64 with DMA staging");
   e->use_graph = (t->graph == 1);
   e->wg_per_cu = t->wg_per_cu ? t->wg_per_cu : 2;
   e->prefetch = t->prefetch == 0 ? XG_DEFAULT_PREFETCH : (t->prefetch == 1);
   e->kernel_pref = t->kernel ? t->kernel : 2;
-  e->stage = t->stage ? t->stage : XG_DEFAULT_RG_STAGE;
+  e->staging = staging;
+  e->stage = t->stage ? t->stage : (staging == XG_DEFAULT_RG_STAGING ? XG_DEFAULT_RG_STAGE : 64);
   return XGPU_OK;
 }
 
@@ -244,6 +250,7 @@ int xgpuAmdGetTuning(const XGPUAmdEngine *e, XGPUAmdTuning *t) {
   t->graph = e->use_graph ? 1 : 2;
   t->kernel = e->kernel_pref;
   t->stage = e->stage;
+  t->staging = e->staging;
   return XGPU_OK;
 }
 
@@ -273,6 +280,12 @@ int xg_launch_family(const XGPUAmdEngine *e, const void *input_d) {
   return XG_KERNEL_I8_RG;
 }
 
+/* XgLaunch.stage code of the real-Gram kernel variant */
+static char rg_stage_code(const XGPUAmdEngine *e) {
+  if (e->staging == 2) return (char)(e->stage == 32 ? 4 : 3);
+  return (char)(e->stage == 128 ? 1 : 0);
+}
+
 /* one workgroup per CU: the real-Gram kernel holds 512 registers per lane */
 static int wg_per_cu_for(const XGPUAmdEngine *e, int family) {
   return family == XG_KERNEL_I8_RG ? 1 : e->wg_per_cu;
@@ -286,7 +299,7 @@ int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime) {
   job_fill(&L.jobs[0], e, input_d, ntime, !e->cleared, 0);
   xg_plan_launch(&L, e->cus, wg_per_cu_for(e, family));
   L.prefetch = (char)e->prefetch;
-  L.stage = (char)(e->stage == 128);
+  L.stage = rg_stage_code(e);
   HIPCALL(xgh_launch_xengine(family, &L, e->stream), "xengine kernel");
   e->cleared = 0;
   e->samples_in_integ += ntime;
@@ -574,7 +587,7 @@ int xgpuAmdCorrelateBatch(const XGPUAmdBatchItem *items, int nitems, void *strea
     if (xg_launch_family(items[i].engine, items[i].input_d) != family) family = e0->family;
   xg_plan_launch(&L, e0->cus, wg_per_cu_for(e0, family));
   L.prefetch = (char)e0->prefetch;
-  L.stage = (char)(e0->stage == 128);
+  L.stage = rg_stage_code(e0);
   HIPCALL(xgh_launch_xengine(family, &L, s), "batched xengine kernel");
   HIPCALL(xgh_event_record(e0->ev_aux, s), "record");
   for (int i = 0; i < nitems; ++i) {

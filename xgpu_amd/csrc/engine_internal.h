@@ -23,7 +23,8 @@ struct XGPUAmdEngineStruct {
   int wg_per_cu;
   int prefetch;      /* MFMA kernels: L2 prefetch 2 stages ahead */
   int kernel_pref;   /* int8: 1 = complex-plane kernel, 2 = real-Gram when eligible */
-  int stage;         /* real-Gram kernel: samples per LDS stage (64 or 128) */
+  int stage;         /* real-Gram kernel: samples per LDS stage */
+  int staging;       /* real-Gram kernel: 1 registers, 2 DMA */
 
   void *integ;       /* int32 / f32 integration, 2*matLength scalars */
   void *wide;        /* int64 / f64 spill, allocated on first spill */
@@ -57,8 +58,9 @@ struct XGPUAmdEngineStruct {
 
 /* default of the L2 prefetch knob of the MFMA kernels */
 #define XG_DEFAULT_PREFETCH 0
-/* default samples per LDS stage of the real-Gram kernel */
-#define XG_DEFAULT_RG_STAGE 128
+/* default staging and samples per LDS stage of the real-Gram kernel */
+#define XG_DEFAULT_RG_STAGING 2
+#define XG_DEFAULT_RG_STAGE 32
 
 int xg_fail(int code, const char *fmt, ...);
 void xg_plan_launch(XgLaunch *L, int cus, int wg_per_cu);

@@ -59,6 +59,10 @@ int xgh_graph_end(void *stream, void **exec);
 int xgh_graph_launch(void *exec, void *stream);
 int xgh_graph_destroy(void *exec);
 
+/* the tile the kernels take as the k-th of XCD xcd (CPU only, for tests);
+ * 0 past the end of that XCD's share */
+int xgh_schedule(const XgLaunch *launch, int xcd, int k, int *job, int *f, int *rb, int *cb);
+
 /* kernels (xg_kernels.hip) */
 int xgh_launch_xengine(int family, const XgLaunch *launch, void *stream);
 int xgh_launch_fill_zero(void *ptr, size_t bytes, void *stream);

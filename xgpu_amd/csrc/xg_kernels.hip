@@ -61,25 +61,84 @@ struct TileRef {
   int job, f, rb, cb;
 };
 
+// Order of the tiles of one channel: the lower triangle of nblk x nblk blocks
+// is cut into super-blocks of XG_SB x XG_SB blocks, taken column by column
+// (lower triangle only), and the blocks inside a super-block column by column.
+// Tiles that run at the same time share input chunks (a tile (rb, cb) reads
+// chunks rb and cb), and the last tiles of the order are packed into few
+// chunks: at nblk = 8 the last 4 tiles touch 3 chunks and the last 8 touch 4.
+#define XG_SB 8
+__host__ __device__ inline void xg_tile_order(int nblk, int u, int *rb, int *cb) {
+  const int nsb = (nblk + XG_SB - 1) / XG_SB;
+  for (int C = 0; C < nsb; ++C) {
+    const int c0 = C * XG_SB, c1 = min(c0 + XG_SB, nblk);
+    for (int R = C; R < nsb; ++R) {
+      const int r0 = R * XG_SB, r1 = min(r0 + XG_SB, nblk);
+      const int m = r1 - r0;
+      const int count = (R == C) ? m * (m + 1) / 2 : m * (c1 - c0);
+      if (u >= count) {
+        u -= count;
+        continue;
+      }
+      for (int c = c0; c < c1; ++c) {
+        const int top = (R == C) ? c : r0;  // first row of column c in this super-block
+        const int n = r1 - top;
+        if (u < n) {
+          *rb = top + u;
+          *cb = c;
+          return;
+        }
+        u -= n;
+      }
+    }
+  }
+  *rb = *cb = -1;  // u past the triangle (not reached for u < nblk (nblk+1) / 2)
+}
+
 // k-th tile of XCD `xcd`'s share (see xg_launch.h).  Returns false past the end.
-__device__ __forceinline__ bool xg_tile(const XgLaunch &L, int xcd, int k, TileRef &t) {
+// Within a job the XCD's channels are visited in two passes: first, for each
+// channel in turn, the largest multiple W of its tiles that fills whole rounds
+// of the XCD's W persistent workgroups (so a round works on one channel and
+// its tiles share their input in L2), then the remaining tiles of all the
+// channels together.
+__host__ __device__ inline bool xg_tile(const XgLaunch &L, int xcd, int k, TileRef &t) {
   const int *start = L.xcd_start[xcd];
   if (k >= start[L.njobs]) return false;
   int j = 0;
   while (k >= start[j + 1]) ++j;
   const XgJob &J = L.jobs[j];
   const int local = k - start[j];
-  const int ci = local / J.tiles_per_chan;
-  const int ti = local - ci * J.tiles_per_chan;
+  const int T = J.tiles_per_chan;
+  const int nch = (start[j + 1] - start[j]) / T;
+  const int W = L.wg_per_xcd > 0 ? L.wg_per_xcd : 1;
+  const int full = (T / W) * W;  // tiles per channel in the first pass
+  int ci, u;
+  if (local < nch * full) {
+    ci = local / full;
+    u = local - ci * full;
+  } else {
+    const int rest = T - full, l2 = local - nch * full;
+    ci = l2 / rest;
+    u = full + (l2 - ci * rest);
+  }
   const int f0 = (((xcd - J.chan_base) % XG_NXCD) + XG_NXCD) % XG_NXCD;
-  int rb = (int)((sqrtf(8.0f * (float)ti + 1.0f) - 1.0f) * 0.5f);
-  while (rb * (rb + 1) / 2 > ti) --rb;
-  while ((rb + 1) * (rb + 2) / 2 <= ti) ++rb;
   t.job = j;
   t.f = f0 + XG_NXCD * ci;
-  t.rb = rb;
-  t.cb = ti - rb * (rb + 1) / 2;
+  xg_tile_order(J.nblk, u, &t.rb, &t.cb);
   return true;
+}
+
+// Host view of the schedule, for tests: the tile the kernels take as the k-th
+// of XCD `xcd`.  Returns 0 past the end of the XCD's share.
+extern "C" int xgh_schedule(const XgLaunch *L, int xcd, int k, int *job, int *f, int *rb,
+                            int *cb) {
+  TileRef t;
+  if (!xg_tile(*L, xcd, k, t)) return 0;
+  *job = t.job;
+  *f = t.f;
+  *rb = t.rb;
+  *cb = t.cb;
+  return 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -439,9 +498,9 @@ void xg_xengine_mfma(XgLaunch L) {
 //  * two operand register sets: the 16 transpose reads of kstep k+1 are
 //    interleaved with the 16 MFMAs of kstep k;
 //  * one barrier per stage, placed before the last kstep, whose MFMAs then
-//    cover the reads of the next stage's first kstep;
-//  * the global loads of stage s+2 are issued right after that barrier and
-//    written to LDS one stage later, interleaved with MFMAs.
+//    cover the reads of the next stage's first kstep, the LDS writes of stage
+//    s+2 and the global loads of stage s+4 (two whole stages before their data
+//    is written to LDS; two staging register sets).
 // ---------------------------------------------------------------------------
 typedef int v2i __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) v2i lds_v2i;
@@ -500,6 +559,148 @@ struct integral_c {
   static constexpr T value = V;
 };
 
+// Epilogue of the real-Gram kernels.  Element e of lane l in block (a,b) is
+// G[R][Cc] with R = 128wr + 32a + 8(e/4) + 4(l/32) + e%4 and
+// Cc = 128wc + 32b + l%32.  Even lanes (column n_re) finish input row m = R/2
+// for e%4 in {0,1}, odd lanes (column n_im) the row m+1 (e%4 in {2,3}); each
+// takes the two values it lacks from its partner lane.
+__device__ __forceinline__ void rg_combine(const v16i &blk, int q, bool odd, int &re, int &im) {
+  const int x0 = blk[4 * q + 0], x1 = blk[4 * q + 1];
+  const int x2 = blk[4 * q + 2], x3 = blk[4 * q + 3];
+  // each lane sends what its partner lacks: odd lanes their e%4 = 0,1
+  // (column n_im of row m), even lanes their e%4 = 2,3 (column n_re of m+1)
+  const int y0 = __shfl_xor(odd ? x0 : x2, 1);
+  const int y1 = __shfl_xor(odd ? x1 : x3, 1);
+  if (!odd) {  // x0 = G[mr][nr], x1 = G[mi][nr], y0 = G[mr][ni], y1 = G[mi][ni]
+    re = (int)((unsigned)x0 + (unsigned)y1);
+    im = (int)((unsigned)x1 - (unsigned)y0);
+  } else {     // x2 = G[m'r][ni], x3 = G[m'i][ni], y0 = G[m'r][nr], y1 = G[m'i][nr]
+    re = (int)((unsigned)y0 + (unsigned)x3);
+    im = (int)((unsigned)y1 - (unsigned)x2);
+  }
+}
+
+// int32 stores of the three output orders with the index arithmetic split
+// into a per-tile 64-bit base and 32-bit lane, row and column terms.  With
+// m = 2i + p1 (row input) and n = 2j + p2 (column input), this lane's element
+// (a, b, q) has p1 = l & 1, p2 = (l >> 1) & 1 and stations
+//   i = rb*64 + 32wr + (l >> 5) + 8a + 2q,   j = cb*64 + 32wc + (l%32 >> 2) + 8b.
+// Index formulas as in vis_index (reference src/kernel.cuh:49-117).
+template <int ORDER>
+__device__ __forceinline__ void rg_store_i32(const XgJob &J, const TileRef &T, v16i (&acc)[4][4],
+                                             int wr, int wc, int lane) {
+  const bool odd = lane & 1;
+  const int hi = lane >> 5, l31 = lane & 31;
+  const int p1 = odd, p2 = (l31 >> 1) & 1;
+  const int ns = J.nstation;
+  const int i0 = T.rb * 64 + 32 * wr + hi;
+  const int j0 = T.cb * 64 + 32 * wc + (l31 >> 2);
+  const bool diag = T.rb == T.cb;
+  const bool full = !diag && (T.rb * 64 + 64 <= ns);  // rb > cb: every (i, j) valid, i > j
+  const int accum = J.accumulate;
+  const long long plane = J.plane;
+  unsigned *base;
+  unsigned kl;  // lane term
+  if (ORDER == XG_ORDER_REGTILE) {
+    // l = (f*4 + 2ry + rx) RT + R(R+1)/2 + C, element 4l + 2p1 + p2, imag at +plane;
+    // R = i >> 1 = rb*32 + 16wr + 4a + q, rx = l >> 5, C = j >> 1, ry = j & 1
+    const long long RT = (long long)(ns >> 1) * ((ns >> 1) + 1) / 2;
+    base = (unsigned *)J.out + 4 * ((long long)T.f * 4 * RT + T.cb * 32 + 16 * wc);
+    const int ry = (l31 >> 2) & 1;
+    kl = (unsigned)(4 * ((2 * ry + hi) * RT) + 2 * p1 + p2 + 4 * (l31 >> 3));
+  } else {
+    const long long NB = (long long)ns * (ns + 1) / 2;
+    if (ORDER == XG_ORDER_TRI) {  // element 8k + 4p1 + 2p2, imag at +1
+      base = (unsigned *)J.out + 8 * ((long long)T.f * NB + T.cb * 64 + 32 * wc);
+      kl = (unsigned)(4 * p1 + 2 * p2 + 8 * (l31 >> 2));
+    } else {                      // REAL_IMAG: element 4k + 2p1 + p2, imag at +plane
+      base = (unsigned *)J.out + 4 * ((long long)T.f * NB + T.cb * 64 + 32 * wc);
+      kl = (unsigned)(2 * p1 + p2 + 4 * (l31 >> 2));
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      // one block at a time: keeps the scheduler from reading every
+      // accumulator into VGPRs up front
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int re, im;
+        rg_combine(acc[a][b], q, odd, re, im);
+        const int i = i0 + 8 * a + 2 * q, j = j0 + 8 * b;
+        unsigned off;
+        bool zero = false;
+        if (ORDER == XG_ORDER_REGTILE) {
+          const unsigned R = (unsigned)(T.rb * 32 + 16 * wr + 4 * a + q);
+          off = kl + 16 * b + 2 * R * (R + 1);
+        } else if (ORDER == XG_ORDER_TRI) {
+          off = kl + 64 * b + 4 * (unsigned)i * (unsigned)(i + 1);
+        } else {
+          off = kl + 32 * b + 2 * (unsigned)i * (unsigned)(i + 1);
+        }
+        if (!full) {
+          if (i >= ns || j >= ns) continue;
+          if (i < j) {
+            // register-tile quadrant "12" of a diagonal cell is never written by
+            // the reference (kernel.cuh:65-71): zero after a clear
+            if (ORDER != XG_ORDER_REGTILE || (i >> 1) != (j >> 1) || accum) continue;
+            zero = true;
+          }
+        }
+        if (zero) re = im = 0;
+        unsigned *p = base + off;
+        if (ORDER == XG_ORDER_TRI) {
+          uint2 v = make_uint2((unsigned)re, (unsigned)im);
+          if (accum) {
+            const uint2 o = *(const uint2 *)p;
+            v.x += o.x;
+            v.y += o.y;
+          }
+          *(uint2 *)p = v;
+        } else {
+          if (accum) {
+            p[0] += (unsigned)re;
+            p[plane] += (unsigned)im;
+          } else {
+            p[0] = (unsigned)re;
+            p[plane] = (unsigned)im;
+          }
+        }
+      }
+    }
+}
+
+__device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, v16i (&acc)[4][4],
+                                            int wr, int wc, int lane) {
+  if (J.out_kind == XG_OUT_I32) {
+    if (J.order == XG_ORDER_REGTILE)
+      rg_store_i32<XG_ORDER_REGTILE>(J, T, acc, wr, wc, lane);
+    else if (J.order == XG_ORDER_TRI)
+      rg_store_i32<XG_ORDER_TRI>(J, T, acc, wr, wc, lane);
+    else
+      rg_store_i32<XG_ORDER_REALIMAG>(J, T, acc, wr, wc, lane);
+    return;
+  }
+  const bool odd = lane & 1;  // int64 output: generic path
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int re, im;
+        rg_combine(acc[a][b], q, odd, re, im);
+        const int R0 = 128 * wr + 32 * a + 8 * q + 4 * (lane >> 5);  // even real row
+        const int m = T.rb * XG_TILE + (R0 >> 1) + (odd ? 1 : 0);
+        const int n = T.cb * XG_TILE + ((128 * wc + 32 * b + (lane & 31)) >> 1);
+        store_int(J, T.f, m, n, re, im);
+      }
+    }
+}
+
 // scheduling-group masks (llvm.amdgcn.sched.group.barrier)
 #define SG_MFMA 0x008
 #define SG_VMEM_READ 0x020
@@ -542,8 +743,9 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
   const int voff1 = (!diag && T.cb * XG_TILE + 8 * chunk < ninput) ? voff : 0x7fffff00;
   const int wpos = srow * RG_ROW + (rg_slot(chunk, srow & 7) << 4);  // LDS write offset
 
-  v4i r0[C::kLoads], r1[C::kLoads];
-  auto gload = [&](int st) {
+  // two staging register sets (a: even stages, b: odd stages)
+  v4i ra0[C::kLoads], ra1[C::kLoads], rb0[C::kLoads], rb1[C::kLoads];
+  auto gload = [&](v4i *r0, v4i *r1, int st) {
     const long long sbase = st * stage_step;
 #pragma unroll
     for (int i = 0; i < C::kLoads; ++i) {
@@ -560,7 +762,7 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
       r1[i] = __builtin_amdgcn_raw_buffer_load_b128(rs1, voff1, 0, 0);
     }
   };
-  auto lwrite = [&](int buf) {
+  auto lwrite = [&](const v4i *r0, const v4i *r1, int buf) {
     unsigned char *b = lds + buf * C::kBuf;
 #pragma unroll
     for (int i = 0; i < C::kLoads; ++i) {
@@ -595,112 +797,86 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
     for (int b = 0; b < 4; ++b) B[b] = rg_operand(base + boff[b]);
   };
   // One kstep: the 16 MFMAs on (A, B), with the 16 transpose reads of the next
-  // kstep into (NA, NB) over the first 8 MFMAs (when READ) and `fill` (global
-  // loads or LDS staging writes, FILL's scheduling class, 2 per MFMA) over the
+  // kstep into (NA, NB) over the first 8 MFMAs and, when FILL, the staging
+  // writes of one stage into LDS and the global loads of a later one over the
   // last 8.
-  auto kstep = [&](auto read_c, auto fill_c, const v4i *A, const v4i *B, v4i *NA, v4i *NB,
+  auto kstep = [&](auto fill_c, const v4i *A, const v4i *B, v4i *NA, v4i *NB,
                    const unsigned char *nbase, auto fill) {
-    constexpr bool READ = decltype(read_c)::value;
-    constexpr int FILL = decltype(fill_c)::value;
+    constexpr bool FILL = decltype(fill_c)::value;
+    constexpr int kFill = 2 * C::kLoads / 8;  // staging writes (and loads) per MFMA
 #pragma unroll
     for (int i = 0; i < 16; ++i)
       acc[i >> 2][i & 3] =
           __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i >> 2], B[i & 3], acc[i >> 2][i & 3], 0, 0, 0);
-    if constexpr (READ) lread(nbase, NA, NB);
+    lread(nbase, NA, NB);
     fill();
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       __builtin_amdgcn_sched_group_barrier(SG_MFMA, 1, 0);
-      if constexpr (READ) __builtin_amdgcn_sched_group_barrier(SG_DS_READ, 2, 0);
+      __builtin_amdgcn_sched_group_barrier(SG_DS_READ, 2, 0);
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       __builtin_amdgcn_sched_group_barrier(SG_MFMA, 1, 0);
-      if constexpr (FILL != 0) __builtin_amdgcn_sched_group_barrier(FILL, 2, 0);
+      if constexpr (FILL) {
+        __builtin_amdgcn_sched_group_barrier(SG_DS_WRITE, kFill, 0);
+        __builtin_amdgcn_sched_group_barrier(SG_VMEM_READ, kFill, 0);
+      }
     }
   };
   typedef integral_c<bool, true> yes;
-  typedef integral_c<int, 0> nofill;
+  typedef integral_c<bool, false> no;
   auto none = [] {};
 
-  // Prologue: stage 0 in LDS, stage 1 in flight, kstep 0 operands in set 0.
-  // The stage body below has no branches (a branch around the MFMAs makes the
-  // compiler copy the accumulators): past the last stage, the global loads
-  // have an empty range and read zeros without touching memory, and the
-  // staging write and operand reads go to the idle LDS buffer.
-  gload(0);
-  lwrite(0);
-  gload(1);
+  // Pipeline.  Stage s+1 is in LDS buffer (s+1)&1 while stage s is read from
+  // buffer s&1; after the barrier that ends the reads of buffer s&1, the last
+  // kstep of stage s writes stage s+2 into it from the staging registers and
+  // issues the loads of stage s+4 into the same registers, so a global load
+  // has two whole stages to land.  Even stages use register set a, odd ones
+  // set b (the stage loop is unrolled by two; a block with an odd number of
+  // stages runs one more stage of zeros).  Prologue: stages 0 and 1 in LDS,
+  // stages 2 and 3 in flight, kstep 0 operands in set 0.
+  //
+  // The stage body has no branches (a branch around the MFMAs makes the
+  // compiler copy the accumulators): past the last stage the global loads have
+  // an empty range and read zeros without touching memory, and the staging
+  // writes and operand reads go to the idle LDS buffer.
+  gload(ra0, ra1, 0);
+  lwrite(ra0, ra1, 0);
+  gload(ra0, ra1, 1);
+  lwrite(ra0, ra1, 1);
+  gload(ra0, ra1, 2);
+  __builtin_amdgcn_sched_barrier(0);  // set a's loads first: they are waited for first
+  gload(rb0, rb1, 3);
   rg_barrier();
   lread(lds, A0, B0);
-  for (int s = 0; s < nstage; ++s) {
-    const int buf = s & 1;
-    const unsigned char *cur = lds + buf * C::kBuf;
-    const unsigned char *nxt = lds + (buf ^ 1) * C::kBuf;
-    // ksteps 0 .. kSteps-3 of this stage
+  auto stage = [&](int s, const unsigned char *cur, const unsigned char *nxt, v4i *r0, v4i *r1,
+                   int buf) {
+    // ksteps 0 .. kSteps-2 of this stage
 #pragma unroll
-    for (int ks = 0; ks < C::kSteps - 2; ++ks) {
+    for (int ks = 0; ks < C::kSteps - 1; ++ks) {
       if (ks & 1)
-        kstep(yes(), nofill(), A1, B1, A0, B0, cur + 32 * (ks + 1) * RG_ROW, none);
+        kstep(no(), A1, B1, A0, B0, cur + 32 * (ks + 1) * RG_ROW, none);
       else
-        kstep(yes(), nofill(), A0, B0, A1, B1, cur + 32 * (ks + 1) * RG_ROW, none);
+        kstep(no(), A0, B0, A1, B1, cur + 32 * (ks + 1) * RG_ROW, none);
     }
-    // kstep kSteps-2 also writes the next stage's staging registers into the
-    // other LDS buffer
-    constexpr int kw = C::kSteps - 2;
-    auto wr_next = [&] { lwrite(buf ^ 1); };
-    if constexpr (kw & 1)
-      kstep(yes(), integral_c<int, SG_DS_WRITE>(), A1, B1, A0, B0, cur + 32 * (kw + 1) * RG_ROW,
-            wr_next);
-    else
-      kstep(yes(), integral_c<int, SG_DS_WRITE>(), A0, B0, A1, B1, cur + 32 * (kw + 1) * RG_ROW,
-            wr_next);
-    rg_barrier();  // next stage visible in LDS; this stage's buffer is free
-    // last kstep of this stage: reads kstep 0 of the next stage into set 0
-    // and issues the global loads of stage s+2
-    auto ld_next = [&] { gload(s + 2); };
+    rg_barrier();  // every wave is done reading this stage's buffer
+    // last kstep: reads kstep 0 of stage s+1 into set 0, writes stage s+2 into
+    // this stage's buffer and issues the global loads of stage s+4
+    auto refill = [&] {
+      lwrite(r0, r1, buf);
+      gload(r0, r1, s + 4);
+    };
     if constexpr ((C::kSteps - 1) & 1)
-      kstep(yes(), integral_c<int, SG_VMEM_READ>(), A1, B1, A0, B0, nxt, ld_next);
+      kstep(yes(), A1, B1, A0, B0, nxt, refill);
     else
-      kstep(yes(), integral_c<int, SG_VMEM_READ>(), A0, B0, A1, B1, nxt, ld_next);
+      kstep(yes(), A0, B0, A1, B1, nxt, refill);
+  };
+  for (int s = 0; s < nstage; s += 2) {
+    stage(s, lds, lds + C::kBuf, ra0, ra1, 0);
+    stage(s + 1, lds + C::kBuf, lds, rb0, rb1, 1);
   }
-  if (!active) return;
-
-  // Epilogue.  Element e of lane l in block (a,b) is G[R][Cc] with
-  // R = 128wr + 32a + 8(e/4) + 4(l/32) + e%4 and Cc = 128wc + 32b + l%32.
-  // Even lanes (column n_re) finish input row m = R/2 for e%4 in {0,1}, odd
-  // lanes (column n_im) the row m+1 (e%4 in {2,3}); each takes the two
-  // values it lacks from its partner lane.
-  const bool odd = lane & 1;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      // one block at a time: keeps the scheduler from reading every
-      // accumulator into VGPRs up front
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int x0 = acc[a][b][4 * q + 0], x1 = acc[a][b][4 * q + 1];
-        const int x2 = acc[a][b][4 * q + 2], x3 = acc[a][b][4 * q + 3];
-        // each lane sends what its partner lacks: odd lanes their e%4 = 0,1
-        // (column n_im of row m), even lanes their e%4 = 2,3 (column n_re of m+1)
-        const int y0 = __shfl_xor(odd ? x0 : x2, 1);
-        const int y1 = __shfl_xor(odd ? x1 : x3, 1);
-        int re, im;
-        if (!odd) {  // x0 = G[mr][nr], x1 = G[mi][nr], y0 = G[mr][ni], y1 = G[mi][ni]
-          re = (int)((unsigned)x0 + (unsigned)y1);
-          im = (int)((unsigned)x1 - (unsigned)y0);
-        } else {     // x2 = G[m'r][ni], x3 = G[m'i][ni], y0 = G[m'r][nr], y1 = G[m'i][nr]
-          re = (int)((unsigned)y0 + (unsigned)x3);
-          im = (int)((unsigned)y1 - (unsigned)x2);
-        }
-        const int R0 = 128 * wr + 32 * a + 8 * q + 4 * (lane >> 5);  // even real row
-        const int m = T.rb * XG_TILE + (R0 >> 1) + (odd ? 1 : 0);
-        const int n = T.cb * XG_TILE + ((128 * wc + 32 * b + (lane & 31)) >> 1);
-        store_int(J, T.f, m, n, re, im);
-      }
-    }
+  if (active) rg_epilogue(J, T, acc, wr, wc, lane);
 }
 
 template <int STG>
@@ -712,6 +888,199 @@ void xg_xengine_rg(XgLaunch L) {
   TileRef T;
   for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
     rg_tile<STG>(L.jobs[T.job], T, lds);
+    rg_barrier();  // every wave is done with this tile's LDS buffers
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// Real-Gram kernel with global -> LDS DMA staging.  Same arithmetic, LDS
+// layout and MFMA pipeline as rg_tile, but the input goes straight from memory
+// into LDS (buffer_load_dwordx4 ... lds: lane L writes LDS bytes
+// [M0 + 16 L, +16), an out-of-range lane writes zeros), so there are no
+// staging registers and no LDS write instructions, and NBUF stage buffers
+// (128 KiB) let each stage's loads be issued NBUF-1 stages before its data is
+// read.  Each lane loads the chunk that belongs in its LDS slot: slot
+// = rg_slot(chunk, sample & 7), i.e. chunk = g(slot ^ (sample & 7)) with
+// g(x) = x ^ ((x & 1) << 3).
+//
+// The DMAs are issued from inline assembly: the compiler would otherwise
+// wait for every outstanding LDS DMA before each LDS read.  Each wave waits
+// for its own DMAs of the next stage (vmcnt) in the stage barrier; nothing
+// else in the stage loop uses the vector-memory counter.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void rg_dma16(v4i rsrc, int voff, unsigned m0) {
+  asm volatile("buffer_load_dwordx4 %0, %1, 0 offen lds" ::"v"(voff), "s"(rsrc), "{m0}"(m0));
+}
+
+// Stage barrier of the DMA kernel: this wave's DMAs except the newest N have
+// landed in LDS, its LDS reads are done, and every wave got here.
+template <int N>
+__device__ __forceinline__ void rgd_barrier() {
+  asm volatile("s_waitcnt vmcnt(%0)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
+}
+
+template <int STG, int NBUF>
+struct RgdCfg {
+  static constexpr int kSide = STG * RG_ROW;
+  static constexpr int kBuf = 2 * kSide;
+  static constexpr int kLds = NBUF * kBuf;
+  static constexpr int kDma = STG / 16;      // 1 KiB DMAs per wave per side per stage
+  static constexpr int kSteps = STG / 32;
+};
+
+template <int STG, int NBUF>
+__device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsigned char *lds) {
+  typedef RgdCfg<STG, NBUF> C;
+  static_assert((NBUF & (NBUF - 1)) == 0 && NBUF >= 2, "NBUF: power of two");
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ninput = 2 * J.nstation;
+  const bool diag = (T.rb == T.cb);
+  const int nstage = (J.ntime + STG - 1) / STG;
+  const int ntime = J.ntime;
+  const int ts = (int)J.tstride;  // < 2^24: the host keeps 128 * tstride < 2^31
+  const unsigned char *in_f = (const unsigned char *)J.input + (long long)T.f * J.nstation * 4;
+  const unsigned long long row_base = (unsigned long long)(in_f + T.rb * RG_ROW);
+  const unsigned long long col_base = (unsigned long long)(in_f + T.cb * RG_ROW);
+  const int row_bytes = ts < RG_ROW ? ts : RG_ROW;
+  const unsigned lds_base = (unsigned)(unsigned long long)lds;
+
+  // DMA lane mapping: a wave DMA covers 4 consecutive samples (16 lanes each);
+  // lane L fills slot L & 15 of sample row L >> 4.  (sample & 7) is the same for
+  // every DMA of this wave: DMA j of a stage starts at sample 16 j + 4 wave.
+  const int s7 = (4 * wave + (lane >> 4)) & 7;
+  const int slot = lane & 15;
+  const int chunk = (slot ^ s7) ^ (((slot ^ s7) & 1) << 3);
+  const int dvoff = (lane >> 4) * ts + 16 * chunk;
+  const int dvoff0 = (T.rb * XG_TILE + 8 * chunk < ninput) ? dvoff : 0x7fffff00;
+  const int dvoff1 = (!diag && T.cb * XG_TILE + 8 * chunk < ninput) ? dvoff : 0x7fffff00;
+  // the DMAs of stage st, in order: side 0 then side 1, j = 0 .. kDma-1
+  auto dma = [&](int st, int side, int j) {
+    const int s0 = st * STG + 16 * j + 4 * wave;  // first sample of this DMA
+    const int rows = min(ntime - s0, 4);
+    const int nrec = max(rows * ts - ts + row_bytes, 0);
+    const unsigned long long a = (side ? col_base : row_base) + (unsigned long long)(long long)s0 * ts;
+    v4i rs;
+    rs.x = (int)(unsigned)a;
+    rs.y = (int)((a >> 32) & 0xffff);
+    rs.z = nrec;
+    rs.w = 0x00020000;
+    const unsigned m0 = lds_base + (st & (NBUF - 1)) * C::kBuf + side * C::kSide +
+                        (16 * j + 4 * wave) * RG_ROW;
+    rg_dma16(rs, side ? dvoff1 : dvoff0, m0);
+  };
+  auto dma_stage = [&](int st) {
+#pragma unroll
+    for (int side = 0; side < 2; ++side)
+#pragma unroll
+      for (int j = 0; j < C::kDma; ++j) dma(st, side, j);
+  };
+
+  const int wr = wave >> 1, wc = wave & 1;
+  const bool active = !(diag && wr < wc);
+  int aoff[4], boff[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) aoff[a] = rg_lane_off(8 * wr + 2 * a, lane);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) boff[b] = rg_lane_off(8 * wc + 2 * b, lane) + (diag ? 0 : C::kSide);
+  v16i acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
+
+  v4i A0[4], B0[4], A1[4], B1[4];
+  auto lread = [&](const unsigned char *base, v4i *A, v4i *B) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) A[a] = rg_operand(base + aoff[a]);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) B[b] = rg_operand(base + boff[b]);
+  };
+  // One kstep: 16 MFMAs on (A, B), the 16 transpose reads of the next kstep
+  // over the first 8, and (when FILL) the DMAs of stage `st` over the last 8.
+  auto kstep = [&](auto fill_c, const v4i *A, const v4i *B, v4i *NA, v4i *NB,
+                   const unsigned char *nbase, int st) {
+    constexpr bool FILL = decltype(fill_c)::value;
+    constexpr int kTot = 2 * C::kDma;  // DMAs of one stage, spread over MFMAs 8..15
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      acc[i >> 2][i & 3] =
+          __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i >> 2], B[i & 3], acc[i >> 2][i & 3], 0, 0, 0);
+    lread(nbase, NA, NB);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      __builtin_amdgcn_sched_group_barrier(SG_MFMA, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(SG_DS_READ, 2, 0);
+    }
+#pragma unroll
+    for (int i = 8; i < 16; ++i) {
+      acc[i >> 2][i & 3] =
+          __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i >> 2], B[i & 3], acc[i >> 2][i & 3], 0, 0, 0);
+      if constexpr (FILL) {
+        // DMAs [(i-8) kTot / 8, (i-7) kTot / 8): every DMA exactly once
+#pragma unroll
+        for (int d = (i - 8) * kTot / 8; d < (i - 7) * kTot / 8; ++d) dma(st, d / C::kDma, d % C::kDma);
+      }
+    }
+  };
+  typedef integral_c<bool, true> yes;
+  typedef integral_c<bool, false> no;
+
+  // prologue: DMAs of stages 0 .. NBUF-1, wait for stage 0
+#pragma unroll
+  for (int st = 0; st < NBUF; ++st) dma_stage(st);
+  rgd_barrier<(NBUF - 1) * 2 * C::kDma>();
+  lread(lds, A0, B0);
+  // One stage whose kstep-0 operands are in operand set P.  The last kstep
+  // reads the next stage's kstep 0 into set (P + kSteps) & 1.
+  auto run_stage = [&](int s, auto p_c) {
+    constexpr int P = decltype(p_c)::value;
+    const unsigned char *cur = lds + (s & (NBUF - 1)) * C::kBuf;
+    const unsigned char *nxt = lds + ((s + 1) & (NBUF - 1)) * C::kBuf;
+#pragma unroll
+    for (int ks = 0; ks < C::kSteps - 1; ++ks) {
+      if ((P + ks) & 1)
+        kstep(no(), A1, B1, A0, B0, cur + 32 * (ks + 1) * RG_ROW, 0);
+      else
+        kstep(no(), A0, B0, A1, B1, cur + 32 * (ks + 1) * RG_ROW, 0);
+    }
+    // stage s+1 has landed (this wave's DMAs; the barrier covers the others'),
+    // and every wave is done reading stage s's buffer, which is refilled with
+    // stage s+NBUF during the last kstep
+    rgd_barrier<(NBUF - 2) * 2 * C::kDma>();
+    if constexpr ((P + C::kSteps - 1) & 1)
+      kstep(yes(), A1, B1, A0, B0, nxt, s + NBUF);
+    else
+      kstep(yes(), A0, B0, A1, B1, nxt, s + NBUF);
+  };
+  if constexpr ((C::kSteps & 1) == 0) {
+    for (int s = 0; s < nstage; ++s) run_stage(s, integral_c<int, 0>());
+  } else {
+    // the operand sets alternate between stages: two stages per iteration (a
+    // block with an odd number of stages runs one more, zero-filled stage)
+    for (int s = 0; s < nstage; s += 2) {
+      run_stage(s, integral_c<int, 0>());
+      run_stage(s + 1, integral_c<int, 1>());
+    }
+  }
+  // DMAs of stages past the block (zero-filled) may still be in flight: drain
+  // them before the next tile reuses the buffers
+  rgd_barrier<0>();
+  if (active) rg_epilogue(J, T, acc, wr, wc, lane);
+}
+
+template <int STG, int NBUF>
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
+void xg_xengine_rgd(XgLaunch L) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[RgdCfg<STG, NBUF>::kLds];
+  const int xcd = blockIdx.x % XG_NXCD;
+  const int wl = blockIdx.x / XG_NXCD;
+  TileRef T;
+  for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
+    rgd_tile<STG, NBUF>(L.jobs[T.job], T, lds);
     rg_barrier();  // every wave is done with this tile's LDS buffers
   }
 }
@@ -974,7 +1343,11 @@ extern "C" int xgh_launch_xengine(int family, const XgLaunch *L, void *stream) {
       hipLaunchKernelGGL(xg_xengine_f32, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       break;
     case XG_KERNEL_I8_RG:
-      if (L->stage)
+      if (L->stage == 3)
+        hipLaunchKernelGGL((xg_xengine_rgd<64, 4>), dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      else if (L->stage == 4)
+        hipLaunchKernelGGL((xg_xengine_rgd<32, 8>), dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      else if (L->stage == 1)
         hipLaunchKernelGGL(xg_xengine_rg<128>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       else
         hipLaunchKernelGGL(xg_xengine_rg<64>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);

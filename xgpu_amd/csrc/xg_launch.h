@@ -64,8 +64,10 @@ typedef struct XgLaunch {
   int njobs;
   short wg_per_xcd;                             /* grid = 8 * wg_per_xcd */
   char prefetch;                                /* MFMA kernels: L2 prefetch 2 stages ahead */
-  char stage;                                   /* real-Gram kernel: 0 = 64 samples per
-                                                   LDS stage, 1 = 128 */
+  char stage;                                   /* real-Gram kernel: 0 / 1 = register
+                                                   staging, 64 / 128 samples per stage;
+                                                   3 / 4 = global->LDS DMA staging, 64 /
+                                                   32 samples per stage */
   int xcd_start[XG_NXCD][XG_MAX_JOBS + 1];      /* per-XCD prefix of tile counts */
 } XgLaunch;
 
