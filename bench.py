@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import os
 import subprocess
 import sys
@@ -155,6 +156,22 @@ def serving_bench(X, seed=5, nstreams=8, dumps_per_stream=2, load=0.7):
                     "mean_batch": round(lat["mean_batch"], 2)},
         "dumps": "exact int64, triangular order, copied to host",
     }
+
+
+def profiled_traffic(kname):
+    """HBM bytes per launch of the headline kernel, from the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE scaled by a plain copy of known
+    size); None when that summary is for a different kernel."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "xg_xengine_rgd_int8_ns512_nf64_nt8192.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch: (\d+)$", text, re.M)
+    if not m or f"`{kname}`" not in text.splitlines()[0]:
+        return None
+    return int(m.group(1))
 
 
 def kernel_name(tun):
@@ -328,7 +345,7 @@ def main():
             "peak": PEAK_INT8_TOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tops / PEAK_INT8_TOPS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(kname),
             "kernel": kname,
             "ops_per_cmac": 8,
             "kernel_ms": round(ms_kernel, 4),
