@@ -269,6 +269,10 @@ def main():
     achieved_tops = cmac * 8 / (ms_kernel * 1e-3) / 1e12
     log(f"[bench] int8 {kname}: {ms_step:.3f} ms/step wall, {ms_kernel:.3f} ms kernel, "
         f"{value:.1f} TCMAC/s")
+    # per-launch kernel times (HIP events around each launch) for median / p99
+    per = sorted(eng.time_correlate(buf, NT, warmup=0, reps=1) for _ in range(args.steps))
+    ms_med = per[len(per) // 2]
+    ms_p99 = per[min(len(per) - 1, int(round(0.99 * (len(per) - 1))))]
     # the other int8 kernels on the same input, kernel time only
     variants = {}
     for k, stg, st in ((1, 0, 0), (2, 1, 64), (2, 1, 128), (2, 2, 32), (2, 2, 64)):
@@ -287,7 +291,51 @@ def main():
     eng32.close()
     t32 = cmac / (ms32 * 1e-3) / 1e12
     log(f"[bench] fp32 plain: {ms32:.3f} ms, {t32:.2f} TCMAC/s")
+
+    # host-buffer path (xgpuCudaXengine semantics): the block starts in pinned
+    # host memory and crosses PCIe in NTIME_PIPE chunks overlapped with the
+    # kernels; wall time per call including the transfers
+    host = None
+    try:
+        hb = X.HostBuffer(NT * NF * NS * 4)
+        X.check(X.core().xgpuAmdMemcpy(hb.ptr, buf.ptr, hb.nbytes, 2))
+        eh = X.Engine(NS, NF, NT, 256, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+        lib = X.core()
+        for _ in range(2):
+            X.check(lib.xgpuAmdCorrelateHost(eh.handle, hb.ptr, X.SYNCOP_SYNC_COMPUTE, None))
+        nh = 5
+        t0 = time.perf_counter()
+        for _ in range(nh):
+            X.check(lib.xgpuAmdCorrelateHost(eh.handle, hb.ptr, X.SYNCOP_SYNC_COMPUTE, None))
+        sec = (time.perf_counter() - t0) / nh
+        eh.close()
+        hb.free()
+        host = {"ms_per_block": round(sec * 1e3, 3), "tcmac_s": round(cmac / sec / 1e12, 2),
+                "input_gb_s": round(NT * NF * NS * 4 / sec / 1e9, 1),
+                "ntimepipe": 256, "note": "pinned host input, H2D chunks overlapped with kernels, "
+                "SYNCOP_SYNC_COMPUTE per call; bounded by the host-to-device link"}
+        log(f"[bench] host path: {sec * 1e3:.2f} ms per block, {host['input_gb_s']} GB/s input")
+    except Exception as exc:  # reported, not fatal
+        host = {"error": str(exc)}
     buf.free()
+
+    # cfg3 shape: NSTATION=2048 x 2 pol, NFREQUENCY=16, NTIME=4096 (int8, same kernel)
+    ns3, nf3, nt3 = 2048, 16, 4096
+    cfg3 = None
+    try:
+        b3 = X.DeviceBuffer(nt3 * nf3 * ns3 * 4)
+        b3.fill_uniform_int8(3, -128, 127)
+        e3 = X.Engine(ns3, nf3, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+        ms3 = e3.time_correlate(b3, nt3, warmup=1, reps=5)
+        e3.close()
+        b3.free()
+        c3 = X.cmac_per_integration(ns3, nf3, nt3)
+        cfg3 = {"workload": "NSTATION=2048 x 2 pol, NFREQUENCY=16, NTIME=4096, int8",
+                "kernel_ms": round(ms3, 4), "tcmac_s": round(c3 / (ms3 * 1e-3) / 1e12, 1),
+                "pct_int8_mfma_peak": round(100 * c3 * 8 / (ms3 * 1e-3) / 1e12 / PEAK_INT8_TOPS, 1)}
+        log(f"[bench] cfg3: {ms3:.3f} ms, {cfg3['tcmac_s']} TCMAC/s")
+    except Exception as exc:
+        cfg3 = {"error": str(exc)}
 
     # FP16 MFMA path on its own workload (NS=256, NF=128, NT=2048, fp16 input)
     ns16, nf16, nt16 = 256, 128, 2048
@@ -300,6 +348,16 @@ def main():
     cmac16 = X.cmac_per_integration(ns16, nf16, nt16)
     t16 = cmac16 / (ms16 * 1e-3) / 1e12
     log(f"[bench] fp16 MFMA: {ms16:.3f} ms, {t16:.1f} TCMAC/s")
+
+    # on-board MFMA-only ceiling and the clocks the run saw
+    peak_meas = X.mfma_peak() / 1e12
+    log(f"[bench] MFMA-only int8 probe: {peak_meas:.1f} TCMAC/s")
+    clocks = None
+    try:
+        r = subprocess.run(["rocm-smi", "--showclocks"], capture_output=True, text=True, timeout=30)
+        clocks = [l.strip() for l in r.stdout.splitlines() if "sclk" in l or "mclk" in l][:4]
+    except Exception:
+        clocks = None
 
     serving = None
     if not args.no_serve:
@@ -333,6 +391,11 @@ def main():
             "cmac_per_step": cmac, "input": "uniform int8 [-128,127], seed 2",
         },
         "pct_int8_mfma_peak": round(100.0 * cmac * 8 / (ms_step * 1e-3) / 1e12 / PEAK_INT8_TOPS, 2),
+        "kernel_ms_median": round(ms_med, 4),
+        "kernel_ms_p99": round(ms_p99, 4),
+        "mfma_only_probe_tcmac_s": round(peak_meas, 1),
+        "pct_of_mfma_only_probe": round(100.0 * cmac / (ms_med * 1e-3) / 1e12 / peak_meas, 1),
+        "clocks": clocks,
         "fp32_plain": {"tcmac_s": round(t32, 3), "ms": round(ms32, 3),
                        "pct_fp32_peak": round(100 * t32 * 8 / PEAK_FP32_TFLOPS, 1)},
         "int8_over_fp32_plain": round(value / t32, 2),
@@ -351,6 +414,8 @@ def main():
             "kernel_ms": round(ms_kernel, 4),
         },
         "int8_kernel_variants": variants,
+        "cfg3": cfg3,
+        "host_path": host,
         "cpu_baseline": cpu,
         "serving": serving,
     }

@@ -124,6 +124,10 @@ int xgpuAmdDeviceSynchronize(void);
  * counter-based generator: byte i = lo + ((hi32(mix64(mix64(seed) + i)) *
  * (hi-lo+1)) >> 32), mix64 = SplitMix64 finaliser. */
 int xgpuAmdFillUniformInt8(void *ptr_d, size_t nbytes, unsigned long long seed, int lo, int hi);
+/* Measured MFMA-only int8 throughput of `device` in complex MACs per second
+ * (4 int8 MACs each): the on-board ceiling the X-engine's rate is compared
+ * with.  Best of `reps` launches after a warm-up. */
+int xgpuAmdMfmaPeak(int device, int reps, double *cmac_per_s);
 
 /* ---- measurement -------------------------------------------------------- */
 /* Run xgpuAmdCorrelate(engine, input_d, ntime) warmup + reps times on the

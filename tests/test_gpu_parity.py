@@ -374,3 +374,98 @@ def test_autotune_keeps_results_exact(tmp_path):
     eng.close()
     eng2.close()
     buf.free()
+
+
+# ------------------------------------------------------------------ full-size shapes
+def test_i8_cfg3_shape_exact_by_properties():
+    """NSTATION=2048 x 2 pol, NFREQUENCY=16, NTIME=4096 (1 GiB of int32 output),
+    input round(N(0, 20^2)) plus a unit tone per station (the cfg3 generator),
+    clipped to [-127, 127].  Exact on sampled baselines of every channel
+    (int64 oracle) and on every autocorrelation (input power), via the device
+    reorder to triangular order."""
+    ns, nf, nt = 2048, 16, 4096
+    flush("[i8] cfg3 shape")
+    rng = np.random.default_rng(3)
+    omega, phi = 0.0123, rng.uniform(0, 2 * np.pi, ns)
+    x = np.empty((nt, nf, ns, 2, 2), dtype=np.int8)
+    t = np.arange(nt, dtype=np.float32)[:, None, None]
+    for f in range(nf):  # channel by channel to bound host memory
+        tone = 8 * np.exp(1j * (omega * t + phi[None, None, :]))
+        g = rng.normal(0, 20, (nt, ns, 2, 2)).astype(np.float32)
+        g[..., 0] += tone.real[:, 0, :, None].astype(np.float32)
+        g[..., 1] += tone.imag[:, 0, :, None].astype(np.float32)
+        x[:, f] = np.clip(np.rint(g), -127, 127).astype(np.int8)
+    x = x.reshape(-1)
+    buf = X.DeviceBuffer.from_array(x)
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    eng.correlate(buf, nt)
+    tri = eng.dump(triangular=True).reshape(nf, -1, 4, 2).astype(np.int64)
+    eng.close()
+    buf.free()
+    fij = [(f, i, j) for f in range(nf)
+           for i, j in [(int(a), int(rng.integers(a + 1))) for a in rng.integers(0, ns, 8)]]
+    fij += [(f, ns - 1, 0) for f in range(nf)] + [(f, ns - 1, ns - 2) for f in range(nf)]
+    ref = oracle.xengine_i64_pairs(x, ns, nf, nt, fij)
+    for n, (f, i, j) in enumerate(fij):
+        assert np.array_equal(tri[f, i * (i + 1) // 2 + j], ref[n]), (f, i, j)
+    xt = x.reshape(nt, nf * ns * 4)
+    power = np.zeros(nf * ns * 4, dtype=np.int64)
+    for t0 in range(0, nt, 256):
+        c = xt[t0:t0 + 256].astype(np.int32)
+        power += (c * c).sum(axis=0, dtype=np.int64)
+    power = power.reshape(nf, ns, 2, 2).sum(axis=3)  # [f, s, p]
+    d = np.arange(ns) * (np.arange(ns) + 1) // 2 + np.arange(ns)
+    autos = tri[:, d][:, :, [0, 3], 0]  # [f, s, (xx, yy)] real parts
+    assert np.array_equal(autos, power)
+    assert not tri[:, d][:, :, [0, 3], 1].any()  # autocorrelations are real
+
+
+def test_f16_cfg4_shape_errors_vs_f64():
+    """NSTATION=256 x 2 pol, NFREQUENCY=128, NTIME=2048, fp16 complex
+    N(0, 16^2) plus a delayed tone 4 e^{i(2 pi f tau_s / NF + w t)} (the cfg4
+    generator).  Against the FP64 oracle of the same fp16 inputs on 4
+    channels: max |dV| <= 1e-5 max|V|, relative RMS <= 1e-6, and normalized
+    error |dV_ij| / sqrt(V_ii V_jj) <= (NT/16) 2^-24.  The fp16 products are
+    exact in fp32 and each MFMA adds 16 of them to an fp32 accumulator, so the
+    accumulation error is at most (NT/16) u sum_t |x_i x_j| with u = 2^-24, and
+    sum_t |x_i x_j| <= sqrt(V_ii V_jj) (Cauchy-Schwarz)."""
+    ns, nf, nt = 256, 128, 2048
+    flush("[f16] cfg4 shape")
+    rng = np.random.default_rng(4)
+    tau, omega = rng.uniform(0, 8, ns), 0.0371
+    t = np.arange(nt)[:, None, None, None]
+    f = np.arange(nf)[None, :, None, None]
+    ph = 2 * np.pi * f * tau[None, None, :, None] / nf + omega * t
+    x = np.empty((nt, nf, ns, 2, 2), dtype=np.float16)
+    for t0 in range(0, nt, 256):  # in slabs to bound host memory
+        sl = slice(t0, t0 + 256)
+        g = rng.normal(0, 16, (256, nf, ns, 2, 2))
+        g[..., 0] += 4 * np.cos(ph[sl])
+        g[..., 1] += 4 * np.sin(ph[sl])
+        x[sl] = g.astype(np.float16)
+    buf = X.DeviceBuffer.from_array(x)
+    eng = X.Engine(ns, nf, compute=X.XGPU_FLOAT16, order=X.TRIANGULAR_ORDER)
+    eng.correlate(buf, nt)
+    got = eng.dump().reshape(nf, -1, 4, 2).astype(np.float64)
+    eng.close()
+    buf.free()
+    chans = [0, 37, 90, 127]
+    ref = oracle.xengine_f64_f16(np.ascontiguousarray(x[:, chans]), ns, len(chans), nt)
+    ref = ref.reshape(len(chans), -1, 4, 2)
+    g = got[chans]
+    dv = np.abs((g[..., 0] - ref[..., 0]) + 1j * (g[..., 1] - ref[..., 1]))
+    mag = np.abs(ref[..., 0] + 1j * ref[..., 1])
+    assert dv.max() <= 1e-5 * mag.max(), (dv.max(), mag.max())
+    rel_rms = np.sqrt(np.mean(dv ** 2)) / np.sqrt(np.mean(mag ** 2))
+    assert rel_rms <= 1e-6, rel_rms
+    i = np.repeat(np.arange(ns), np.arange(1, ns + 1))
+    j = np.concatenate([np.arange(k + 1) for k in range(ns)])
+    di = i * (i + 1) // 2 + i
+    dj = j * (j + 1) // 2 + j
+    # V_ii, V_jj per polarization pair: xx autos of (i, p1) and (j, p2)
+    auto = ref[:, :, [0, 3], 0]  # [c, baseline, (p=0, p=1)]
+    for p1 in (0, 1):
+        for p2 in (0, 1):
+            norm = np.sqrt(auto[:, di, p1] * auto[:, dj, p2])
+            worst = (dv[:, :, 2 * p1 + p2] / norm).max()
+            assert worst <= (nt / 16) * 2.0 ** -24, (p1, p2, worst)

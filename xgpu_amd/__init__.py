@@ -200,6 +200,13 @@ class Engine:
             pass
 
 
+def mfma_peak(device: int = 0, reps: int = 3) -> float:
+    """Measured MFMA-only int8 throughput of the device, complex MACs per second."""
+    v = C.c_double()
+    check(core().xgpuAmdMfmaPeak(device, reps, C.byref(v)))
+    return float(v.value)
+
+
 def correlate_batch(items, stream=None) -> None:
     """Ragged batch in one persistent launch: items = [(engine, input_d, ntime)]."""
     arr = (_lib.XGPUAmdBatchItem * len(items))()

@@ -81,6 +81,9 @@ int xgh_launch_fill_uniform_i8(void *ptr, size_t n, uint64_t seed, int lo, int h
  * interleaved TRIANGULAR order (elem: 4 = int32/f32, 8 = int64/f64) */
 int xgh_launch_reorder(int src_order, int elem_bytes, void *dst, const void *src,
                        int nstation, int nfreq, long long plane, void *stream);
+/* MFMA-only probe: grid workgroups of 256 threads, each wave `iters` x 8
+ * independent int8 32x32x32 MFMAs; out holds grid*256 ints */
+int xgh_launch_mfma_peak(int grid, int iters, int *out, void *stream);
 /* device corner-turn to the depth-4 swizzled layout of the dp4a input
  * format: [t/4][f][s][p][c][t%4] */
 int xgh_launch_swizzle(void *dst, const void *src, int nstation, int nfreq,

@@ -1464,3 +1464,33 @@ extern "C" int xgh_launch_swizzle(void *dst, const void *src, int nstation, int 
                      nfreq, ntime);
   return xgh_report(hipGetLastError(), "xg_swizzle launch");
 }
+
+// ---------------------------------------------------------------------------
+// MFMA-only throughput probe (measurement): 8 independent int8 32x32x32
+// accumulator chains per wave, no memory traffic in the loop.  It gives the
+// on-board ceiling against which the X-engine's MFMA rate is compared.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void xg_mfma_peak_k(int iters, int *out) {
+  const v4i a = {(int)threadIdx.x, 1, 2, 3}, b = {3, (int)blockIdx.x, 1, 0};
+  v16i c[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) c[j][e] = 0;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) c[j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c[j], 0, 0, 0);
+  }
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) s += c[j][e];
+  out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+extern "C" int xgh_launch_mfma_peak(int grid, int iters, int *out, void *stream) {
+  if (grid <= 0 || iters <= 0) return 0;
+  hipLaunchKernelGGL(xg_mfma_peak_k, dim3(grid), dim3(256), 0, (hipStream_t)stream, iters, out);
+  return xgh_report(hipGetLastError(), "xg_mfma_peak_k launch");
+}
