@@ -658,14 +658,20 @@ __device__ __forceinline__ void rg_store_i32(const XgJob &J, const TileRef &T, v
             v.x += o.x;
             v.y += o.y;
           }
-          *(uint2 *)p = v;
+          typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+          u32x2 w;
+          w.x = v.x;
+          w.y = v.y;
+          __builtin_nontemporal_store(w, (u32x2 *)p);
         } else {
           if (accum) {
             p[0] += (unsigned)re;
             p[plane] += (unsigned)im;
           } else {
-            p[0] = (unsigned)re;
-            p[plane] = (unsigned)im;
+            // streaming stores: the tile's output is not read again by this
+            // launch and should not evict the input the next round reuses
+            __builtin_nontemporal_store((unsigned)re, p);
+            __builtin_nontemporal_store((unsigned)im, p + plane);
           }
         }
       }
