@@ -142,12 +142,14 @@ typedef struct XGPUAmdTuningStruct {
   int prefetch;    /* MFMA kernels: 1 = touch the input 2 stages ahead into L2, 2 = off */
   int graph;       /* host-buffer path: 1 = capture each call's transfers and kernels in
                       a hipGraph once and replay it, 2 = issue them one by one */
-  int kernel;      /* int8: 1 = complex-plane kernel (any nstation), 2 = real-Gram kernel
-                      where it applies (nstation % 4 == 0, 16-byte aligned input; default) */
-  int stage;       /* real-Gram kernel: samples per LDS stage: 64 or 128 with register
-                      staging, 32 or 64 with DMA staging (0 = default) */
-  int staging;     /* real-Gram kernel: 1 = global -> VGPR -> LDS, 2 = global -> LDS DMA
+  int kernel;      /* int8 and fp16: 1 = complex-plane kernel (any input), 2 = real-Gram
+                      kernel where it applies (16-byte aligned input and, for int8,
+                      nstation % 4 == 0; default) */
+  int stage;       /* real-Gram kernel: samples per LDS stage: int8 64 or 128 with
+                      register staging, 32 or 64 with DMA staging; fp16 32 or 64
                       (0 = default) */
+  int staging;     /* int8 real-Gram kernel: 1 = global -> VGPR -> LDS, 2 = global -> LDS
+                      DMA (0 = default); fp16: 0 or 1 (register staging only) */
   int reserved[2];
 } XGPUAmdTuning;
 int xgpuAmdSetTuning(XGPUAmdEngine *engine, const XGPUAmdTuning *tuning);

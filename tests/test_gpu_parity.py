@@ -267,21 +267,115 @@ def test_f32_cfg1_through_xgpu_h():
 
 
 # ------------------------------------------------------------------ FP16 MFMA
-def test_f16_against_f64_oracle():
-    ns, nf, nt = 64, 2, 512
-    rng = np.random.default_rng(4)
-    xc = rng.normal(0, 16, (nt, nf, ns, 2, 2))
-    x16 = xc.astype(np.float16)
-    flush("[f16] small")
+# (kernel, stage): complex-plane kernel; real-Gram kernel with 32 / 64 samples
+# per LDS stage
+F16_VARIANTS = [(1, 0), (2, 32), (2, 64)]
+
+
+def f16_input(nt, nf, ns, seed, sigma=16.0):
+    rng = np.random.default_rng(seed)
+    return rng.normal(0, sigma, (nt, nf, ns, 2, 2)).astype(np.float16)
+
+
+def assert_f16_close(got, ref, ns, nf, nt, rms=True):
+    """fp32-accumulated visibilities (interleaved triangular) against the
+    FP64 oracle of the same fp16 inputs.  Per baseline and polarization pair,
+    |dV_ij| <= (nt/8 + 1) 2^-24 sqrt(V_ii V_jj): the fp16 products are exact
+    in fp32, every MFMA adds at least 8 samples' products to an fp32
+    accumulator (one rounding of at most 2^-24 times the running sum of
+    |products| per MFMA, then one more for the real-Gram combine), and
+    sum_t |x_i x_j| <= sqrt(V_ii V_jj) (Cauchy-Schwarz).  Also max |dV| <=
+    1e-5 max |V| and, for short blocks, relative RMS <= 1e-6."""
+    g = got.reshape(nf, -1, 4, 2).astype(np.float64)
+    r = ref.reshape(nf, -1, 4, 2)
+    dv = np.abs((g[..., 0] - r[..., 0]) + 1j * (g[..., 1] - r[..., 1]))
+    mag = np.abs(r[..., 0] + 1j * r[..., 1])
+    assert dv.max() <= 1e-5 * mag.max(), (dv.max(), mag.max())
+    if rms:
+        rel_rms = np.sqrt(np.mean(dv ** 2)) / np.sqrt(np.mean(mag ** 2))
+        assert rel_rms <= 1e-6, rel_rms
+    i = np.repeat(np.arange(ns), np.arange(1, ns + 1))
+    j = np.concatenate([np.arange(k + 1) for k in range(ns)])
+    di = i * (i + 1) // 2 + i
+    dj = j * (j + 1) // 2 + j
+    auto = r[:, :, [0, 3], 0]  # [f, baseline, (p=0, p=1)] real autocorrelations
+    for p1 in (0, 1):
+        for p2 in (0, 1):
+            norm = np.sqrt(auto[:, di, p1] * auto[:, dj, p2])
+            worst = (dv[:, :, 2 * p1 + p2] / norm).max()
+            assert worst <= (nt / 8 + 1) * 2.0 ** -24, (p1, p2, worst)
+
+
+def run_f16(ns, nf, nt, x16, order, calls=1, **tuning):
+    flush(f"[f16] ns={ns} nf={nf} nt={nt} order={order} calls={calls} {tuning}")
+    eng = X.Engine(ns, nf, compute=X.XGPU_FLOAT16, order=order)
+    if tuning:
+        eng.set_tuning(**tuning)
+    buf = X.DeviceBuffer.from_array(x16)
+    per = nt // calls
+    step = per * nf * ns * 8
+    for c in range(calls):
+        n = per if c < calls - 1 else nt - per * (calls - 1)
+        eng.correlate(buf.ptr + c * step, n)
+    raw = eng.dump()
+    wide = eng.dump(wide=True)
+    eng.close()
+    buf.free()
+    return to_tri(None, raw, ns, nf, order), to_tri(None, wide, ns, nf, order)
+
+
+@pytest.mark.parametrize("ns,nf,nt,order,calls", [
+    (64, 2, 512, "tri", 1),
+    (100, 3, 300, "regtile", 1),    # ragged tile edge, partial last stage
+    (256, 1, 97, "realimag", 2),    # two tiles per side, accumulate
+    (2, 1, 33, "tri", 1),           # one valid chunk
+    (132, 2, 200, "regtile", 2),    # 3 tiles per side, accumulate, ragged edge
+    (130, 1, 64, "realimag", 1),    # nstation % 4 == 2
+])
+@pytest.mark.parametrize("kernel,stage", F16_VARIANTS)
+def test_f16_against_f64_oracle(ns, nf, nt, order, calls, kernel, stage):
+    x16 = f16_input(nt, nf, ns, 4 + ns + nt)
+    got, wide = run_f16(ns, nf, nt, x16, ORDERS[order], calls, kernel=kernel, stage=stage)
+    ref = oracle.xengine_f64_f16(x16, ns, nf, nt)
+    assert_f16_close(got, ref, ns, nf, nt)
+    assert np.array_equal(wide, got.astype(np.float64))
+
+
+def test_f16_variants_agree_cfg2_shape():
+    """NSTATION=512 x 2 pol, NFREQUENCY=64 over 1024 samples: the real-Gram
+    kernels (both stage depths) agree with each other bit for bit (same
+    MFMA sequence per accumulator) and with the complex-plane kernel within
+    the fp32 accumulation bound."""
+    ns, nf, nt = 512, 64, 1024
+    x16 = f16_input(nt, nf, ns, 11)
+    outs = {}
+    for kernel, stage in F16_VARIANTS:
+        outs[(kernel, stage)], _ = run_f16(ns, nf, nt, x16, X.TRIANGULAR_ORDER, 1,
+                                           kernel=kernel, stage=stage)
+    assert np.array_equal(outs[(2, 32)], outs[(2, 64)])
+    chans = [0, 63]
+    ref = oracle.xengine_f64_f16(np.ascontiguousarray(x16[:, chans]), ns, len(chans), nt)
+    for v, got in outs.items():
+        g = got.reshape(nf, -1)[chans].reshape(-1)
+        assert_f16_close(g, ref, ns, len(chans), nt)
+
+
+def test_f16_spill_to_f64():
+    """70000 samples in one call: the fp32 integration spills into the FP64
+    one past 65536 samples; the wide dump matches the FP64 oracle within the
+    accumulation bound."""
+    ns, nf, nt = 8, 1, 70000
+    x16 = f16_input(nt, nf, ns, 21)
+    flush("[f16] spill")
     eng = X.Engine(ns, nf, compute=X.XGPU_FLOAT16, order=X.TRIANGULAR_ORDER)
     buf = X.DeviceBuffer.from_array(x16)
     eng.correlate(buf, nt)
-    got = eng.dump().astype(np.float64)
-    ref = oracle.xengine_f64_f16(x16, ns, nf, nt)
-    scale = np.abs(ref).max()
-    assert np.abs(got - ref).max() <= 1e-5 * scale
-    assert np.sqrt(np.mean((got - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)) <= 1e-6
+    wide = eng.dump(wide=True)
+    assert eng.samples == nt
     eng.close()
+    buf.free()
+    ref = oracle.xengine_f64_f16(x16, ns, nf, nt)
+    assert_f16_close(wide, ref, ns, nf, nt, rms=False)
 
 
 # ------------------------------------------------------------------ batching

@@ -5,8 +5,8 @@
  * Candidates are the engine's kernel knobs: for int8 the kernel variant
  * (complex-plane kernel x persistent workgroups per CU x L2 prefetch, and
  * the real-Gram kernel x staging path (registers or global->LDS DMA) x LDS
- * stage depth where it applies), for the float
- * paths workgroups per CU x prefetch.  Each is timed with HIP events on the
+ * stage depth where it applies), for fp16 the same with the fp16 real-Gram
+ * kernel's stage depths, for the FP32 plain path workgroups per CU.  Each is timed with HIP events on the
  * engine stream over `reps` single-kernel integrations of the caller's
  * device input; the fastest is kept.  The cache is a text file, one line per
  * result:
@@ -84,19 +84,23 @@ int xgpuAmdAutotune(XGPUAmdEngine *e, const void *input_d, unsigned ntime, int w
       t.staging = 0;
       cand[ncand++] = t;
     }
-  if (e->family == XG_KERNEL_I8_MFMA) {
+  if (e->family != XG_KERNEL_F32_PLAIN) {
     XGPUAmdTuning probe = keep;
     probe.kernel = 2;
+    probe.staging = 0;
+    probe.stage = 0;
     xgpuAmdSetTuning(e, &probe);
-    if (xg_launch_family(e, input_d) == XG_KERNEL_I8_RG) {
-      static const int variants[4][2] = {{1, 64}, {1, 128}, {2, 32}, {2, 64}};  /* staging, stage */
-      for (int v = 0; v < 4; ++v) {
-        XGPUAmdTuning t = keep;
-        t.kernel = 2;
-        t.staging = variants[v][0];
-        t.stage = variants[v][1];
-        cand[ncand++] = t;
-      }
+    /* staging, stage */
+    static const int v_i8[4][2] = {{1, 64}, {1, 128}, {2, 32}, {2, 64}};
+    static const int v_f16[2][2] = {{1, 32}, {1, 64}};
+    const int fam = xg_launch_family(e, input_d);
+    const int nv = fam == XG_KERNEL_I8_RG ? 4 : fam == XG_KERNEL_F16_RG ? 2 : 0;
+    for (int v = 0; v < nv; ++v) {
+      XGPUAmdTuning t = keep;
+      t.kernel = 2;
+      t.staging = fam == XG_KERNEL_I8_RG ? v_i8[v][0] : v_f16[v][0];
+      t.stage = fam == XG_KERNEL_I8_RG ? v_i8[v][1] : v_f16[v][1];
+      cand[ncand++] = t;
     }
   }
   for (int c = 0; c < ncand; ++c) {

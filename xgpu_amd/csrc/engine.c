@@ -162,8 +162,13 @@ int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
   e->wg_per_cu = 2;
   e->prefetch = XG_DEFAULT_PREFETCH;
   e->kernel_pref = 2;
-  e->stage = XG_DEFAULT_RG_STAGE;
-  e->staging = XG_DEFAULT_RG_STAGING;
+  if (e->family == XG_KERNEL_F16_MFMA) {
+    e->stage = XG_DEFAULT_RGH_STAGE;
+    e->staging = 1;
+  } else {
+    e->stage = XG_DEFAULT_RG_STAGE;
+    e->staging = XG_DEFAULT_RG_STAGING;
+  }
   if ((rc = xgh_set_device(cfg->device)) != 0) goto hip_error;
   if ((rc = xgh_device_cu_count(cfg->device, &e->cus)) != 0) goto hip_error;
   if ((rc = xgh_malloc(&e->integ, e->integ_bytes)) != 0) goto hip_error;
@@ -228,17 +233,28 @@ int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
   if (t->graph < 0 || t->graph > 2) return xg_fail(XGPU_INVALID_CONFIG, "graph must be 0..2");
   if (t->kernel < 0 || t->kernel > 2) return xg_fail(XGPU_INVALID_CONFIG, "kernel must be 0..2");
   if (t->staging < 0 || t->staging > 2) return xg_fail(XGPU_INVALID_CONFIG, "staging must be 0..2");
-  const int staging = t->staging ? t->staging : XG_DEFAULT_RG_STAGING;
-  if (t->stage != 0 && !(staging == 1 && (t->stage == 64 || t->stage == 128)) &&
-      !(staging == 2 && (t->stage == 32 || t->stage == 64)))
-    return xg_fail(XGPU_INVALID_CONFIG,
-                   "stage must be 0, or 64 / 128 with register staging, 32 / 64 with DMA staging");
+  int staging, stage;
+  if (e->family == XG_KERNEL_F16_MFMA) {
+    /* the fp16 real-Gram kernel stages through registers only */
+    if (t->staging == 2) return xg_fail(XGPU_INVALID_CONFIG, "fp16 engines have register staging only");
+    if (t->stage != 0 && t->stage != 32 && t->stage != 64)
+      return xg_fail(XGPU_INVALID_CONFIG, "fp16 stage must be 0, 32 or 64");
+    staging = 1;
+    stage = t->stage ? t->stage : XG_DEFAULT_RGH_STAGE;
+  } else {
+    staging = t->staging ? t->staging : XG_DEFAULT_RG_STAGING;
+    if (t->stage != 0 && !(staging == 1 && (t->stage == 64 || t->stage == 128)) &&
+        !(staging == 2 && (t->stage == 32 || t->stage == 64)))
+      return xg_fail(XGPU_INVALID_CONFIG,
+                     "stage must be 0, or 64 / 128 with register staging, 32 / 64 with DMA staging");
+    stage = t->stage ? t->stage : (staging == XG_DEFAULT_RG_STAGING ? XG_DEFAULT_RG_STAGE : 64);
+  }
   e->use_graph = (t->graph == 1);
   e->wg_per_cu = t->wg_per_cu ? t->wg_per_cu : 2;
   e->prefetch = t->prefetch == 0 ? XG_DEFAULT_PREFETCH : (t->prefetch == 1);
   e->kernel_pref = t->kernel ? t->kernel : 2;
   e->staging = staging;
-  e->stage = t->stage ? t->stage : (staging == XG_DEFAULT_RG_STAGING ? XG_DEFAULT_RG_STAGE : 64);
+  e->stage = stage;
   return XGPU_OK;
 }
 
@@ -270,25 +286,29 @@ int xg_spill(XGPUAmdEngine *e) {
   return XGPU_OK;
 }
 
-/* The real-Gram int8 kernel stages 8 inputs (16 bytes) per load with 16-byte
- * buffer loads: it needs nstation % 4 == 0 and a 16-byte aligned block, and
- * keeps every in-stage byte offset (< 128 * tstride) below 2^31. */
+/* The real-Gram kernels stage 16-byte chunks (8 int8 or 4 fp16 inputs) with
+ * 16-byte buffer loads: they need a 16-byte aligned block and, for int8,
+ * nstation % 4 == 0 (fp16 rows are 16-byte multiples for every even
+ * nstation), and keep every in-stage byte offset (< 128 * tstride) below
+ * 2^31. */
 int xg_launch_family(const XGPUAmdEngine *e, const void *input_d) {
-  if (e->family != XG_KERNEL_I8_MFMA || e->kernel_pref == 1) return e->family;
-  if (e->cfg.nstation % 4 || ((uintptr_t)input_d & 15)) return XG_KERNEL_I8_MFMA;
-  if (128LL * e->tstride >= (1LL << 31) - 4096) return XG_KERNEL_I8_MFMA;
-  return XG_KERNEL_I8_RG;
+  if (e->family == XG_KERNEL_F32_PLAIN || e->kernel_pref == 1) return e->family;
+  if ((uintptr_t)input_d & 15) return e->family;
+  if (e->family == XG_KERNEL_I8_MFMA && e->cfg.nstation % 4) return e->family;
+  if (128LL * e->tstride >= (1LL << 31) - 4096) return e->family;
+  return e->family == XG_KERNEL_I8_MFMA ? XG_KERNEL_I8_RG : XG_KERNEL_F16_RG;
 }
 
 /* XgLaunch.stage code of the real-Gram kernel variant */
 static char rg_stage_code(const XGPUAmdEngine *e) {
+  if (e->family == XG_KERNEL_F16_MFMA) return (char)(e->stage == 64 ? 1 : 0);
   if (e->staging == 2) return (char)(e->stage == 32 ? 4 : 3);
   return (char)(e->stage == 128 ? 1 : 0);
 }
 
-/* one workgroup per CU: the real-Gram kernel holds 512 registers per lane */
+/* one workgroup per CU: the real-Gram kernels hold 512 registers per lane */
 static int wg_per_cu_for(const XGPUAmdEngine *e, int family) {
-  return family == XG_KERNEL_I8_RG ? 1 : e->wg_per_cu;
+  return (family == XG_KERNEL_I8_RG || family == XG_KERNEL_F16_RG) ? 1 : e->wg_per_cu;
 }
 
 int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime) {

@@ -58,9 +58,11 @@ struct XGPUAmdEngineStruct {
 
 /* default of the L2 prefetch knob of the MFMA kernels */
 #define XG_DEFAULT_PREFETCH 0
-/* default staging and samples per LDS stage of the real-Gram kernel */
+/* default staging and samples per LDS stage of the int8 real-Gram kernel */
 #define XG_DEFAULT_RG_STAGING 2
 #define XG_DEFAULT_RG_STAGE 32
+/* samples per LDS stage of the fp16 real-Gram kernel (register staging) */
+#define XG_DEFAULT_RGH_STAGE 32
 
 int xg_fail(int code, const char *fmt, ...);
 void xg_plan_launch(XgLaunch *L, int cus, int wg_per_cu);

@@ -507,13 +507,15 @@ typedef __attribute__((address_space(3))) v2i lds_v2i;
 
 #define RG_ROW 256  // LDS bytes per sample per tile side (128 inputs x 2 bytes)
 
-template <int STG>
+// E: element format (RgInt8 or RgHalf, below); STG samples per stage.
+template <class E, int STG>
 struct RgCfg {
-  static constexpr int kSide = STG * RG_ROW;             // bytes per side per stage
+  static constexpr int kSide = STG * E::kRow;            // bytes per side per stage
   static constexpr int kBuf = 2 * kSide;                 // rows + columns
   static constexpr int kLds = 2 * kBuf;                  // double buffered
-  static constexpr int kLoads = STG / 16;                // 16-byte loads per thread per side
-  static constexpr int kSteps = STG / 32;                // MFMA ksteps per stage
+  static constexpr int kPass = XG_WG_THREADS / E::kChunks;  // sample rows per staging pass
+  static constexpr int kLoads = STG / kPass;             // 16-byte loads per thread per side
+  static constexpr int kSteps = STG / E::kK;             // MFMA ksteps per stage
 };
 
 // 16-byte slot of chunk c in a sample row with (sample & 7) == s7.  The 16
@@ -547,6 +549,78 @@ __device__ __forceinline__ v4i rg_operand(const unsigned char *p) {
   return out;
 }
 
+// FP16 format: a complex sample is 4 bytes (re, im halves), so a sample row
+// of one tile side is 512 bytes and real row R of the tile is half R of the
+// row, as in the int8 format.  32 chunks of 16 bytes (4 inputs) per row.
+#define RGH_ROW 512
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) v4s lds_v4s;
+
+// 16-byte slot of chunk c in a sample row with (sample & 3) == s3.  A staging
+// pass writes 16 consecutive chunks of a row to 16 distinct slots; an operand
+// read touches chunks 4m .. 4m+3 of 4 consecutive samples, which land on the 16
+// slots of distinct bank positions (slot & 15): all 64 banks.
+__device__ __forceinline__ int rgh_slot(int chunk, int s3) { return chunk ^ (s3 << 2); }
+
+// Per-lane LDS offset of the operand of the 32-real-row block `blk` (chunks
+// 4 blk .. 4 blk + 3) for samples 0..15 of a kstep.  ds_read_b64_tr_b16 in a
+// 16-lane group: output lane j receives, as its half k, half (j & 3) of the
+// 8 bytes that lane 4k + (j >> 2) supplies.  Supplier r = 4k + q (k = sample,
+// q = 4-row group) gives real rows 16g + 4q .. +3, so output lane j holds
+// real row 16g + j for 4 consecutive samples; lanes 32..63 take samples 8..15.
+__device__ __forceinline__ int rgh_lane_off(int blk, int lane) {
+  const int g = (lane >> 4) & 1, kg = lane >> 5, k = (lane >> 2) & 3, q = lane & 3;
+  const int chunk = 4 * blk + 2 * g + (q >> 1);
+  return (8 * kg + k) * RGH_ROW + (rgh_slot(chunk, k) << 4) + ((q & 1) << 3);
+}
+
+// Two transpose loads: samples kbase + 8 kg + {0..3} and {4..7}, i.e. the
+// 8 halves of one real row that v_mfma_f32_32x32x16_f16 takes from a lane.
+__device__ __forceinline__ v4i rgh_operand(const unsigned char *p) {
+  const v4s lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)p);
+  const v4s hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)(p + 4 * RGH_ROW));
+  const v2i l2 = __builtin_bit_cast(v2i, lo), h2 = __builtin_bit_cast(v2i, hi);
+  v4i out;
+  out[0] = l2.x;
+  out[1] = l2.y;
+  out[2] = h2.x;
+  out[3] = h2.y;
+  return out;
+}
+
+// Element formats of the real-Gram kernels: LDS row of one sample on one tile
+// side, its chunk swizzle, the operand of a 32-real-row block and the MFMA.
+// Both keep 256 real rows per tile and a 16-byte operand per lane.
+struct RgInt8 {
+  typedef v16i Acc;
+  static constexpr int kRow = RG_ROW;
+  static constexpr int kChunks = 16;   // 16-byte chunks per row
+  static constexpr int kInputs = 8;    // inputs per chunk
+  static constexpr int kK = 32;        // samples per MFMA
+  __device__ static int slot(int chunk, int srow) { return rg_slot(chunk, srow & 7); }
+  __device__ static int lane_off(int blk, int lane) { return rg_lane_off(2 * blk, lane); }
+  __device__ static v4i operand(const unsigned char *p) { return rg_operand(p); }
+  __device__ static v16i mfma(v4i a, v4i b, v16i c) {
+    return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+  }
+};
+
+struct RgHalf {
+  typedef v16f Acc;
+  static constexpr int kRow = RGH_ROW;
+  static constexpr int kChunks = 32;
+  static constexpr int kInputs = 4;
+  static constexpr int kK = 16;
+  __device__ static int slot(int chunk, int srow) { return rgh_slot(chunk, srow & 3); }
+  __device__ static int lane_off(int blk, int lane) { return rgh_lane_off(blk, lane); }
+  __device__ static v4i operand(const unsigned char *p) { return rgh_operand(p); }
+  __device__ static v16f mfma(v4i a, v4i b, v16f c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(v8h, a),
+                                                   __builtin_bit_cast(v8h, b), c, 0, 0, 0);
+  }
+};
+
 // Workgroup barrier for LDS traffic only: waits for this wave's LDS operations
 // (not its global loads, which stay in flight across it).  The memory clobber
 // keeps the compiler from moving LDS accesses across it.
@@ -563,32 +637,64 @@ struct integral_c {
 // G[R][Cc] with R = 128wr + 32a + 8(e/4) + 4(l/32) + e%4 and
 // Cc = 128wc + 32b + l%32.  Even lanes (column n_re) finish input row m = R/2
 // for e%4 in {0,1}, odd lanes (column n_im) the row m+1 (e%4 in {2,3}); each
-// takes the two values it lacks from its partner lane.
-__device__ __forceinline__ void rg_combine(const v16i &blk, int q, bool odd, int &re, int &im) {
-  const int x0 = blk[4 * q + 0], x1 = blk[4 * q + 1];
-  const int x2 = blk[4 * q + 2], x3 = blk[4 * q + 3];
+// takes the two values it lacks from its partner lane.  Integer sums wrap
+// modulo 2^32 (the host bounds the block so that they do not overflow).
+template <class Acc>
+struct RgElem;
+template <>
+struct RgElem<v16i> {
+  typedef int T;       // accumulator element
+  typedef unsigned S;  // 32-bit output element
+  static constexpr int kOut32 = XG_OUT_I32;
+};
+template <>
+struct RgElem<v16f> {
+  typedef float T;
+  typedef float S;
+  static constexpr int kOut32 = XG_OUT_F32;
+};
+
+__device__ __forceinline__ int rg_add(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
+__device__ __forceinline__ int rg_sub(int a, int b) { return (int)((unsigned)a - (unsigned)b); }
+__device__ __forceinline__ float rg_add(float a, float b) { return a + b; }
+__device__ __forceinline__ float rg_sub(float a, float b) { return a - b; }
+
+template <class Acc, class T = typename RgElem<Acc>::T>
+__device__ __forceinline__ void rg_combine(const Acc &blk, int q, bool odd, T &re, T &im) {
+  const T x0 = blk[4 * q + 0], x1 = blk[4 * q + 1];
+  const T x2 = blk[4 * q + 2], x3 = blk[4 * q + 3];
   // each lane sends what its partner lacks: odd lanes their e%4 = 0,1
   // (column n_im of row m), even lanes their e%4 = 2,3 (column n_re of m+1)
-  const int y0 = __shfl_xor(odd ? x0 : x2, 1);
-  const int y1 = __shfl_xor(odd ? x1 : x3, 1);
+  const T y0 = __shfl_xor(odd ? x0 : x2, 1);
+  const T y1 = __shfl_xor(odd ? x1 : x3, 1);
   if (!odd) {  // x0 = G[mr][nr], x1 = G[mi][nr], y0 = G[mr][ni], y1 = G[mi][ni]
-    re = (int)((unsigned)x0 + (unsigned)y1);
-    im = (int)((unsigned)x1 - (unsigned)y0);
+    re = rg_add(x0, y1);
+    im = rg_sub(x1, y0);
   } else {     // x2 = G[m'r][ni], x3 = G[m'i][ni], y0 = G[m'r][nr], y1 = G[m'i][nr]
-    re = (int)((unsigned)y0 + (unsigned)x3);
-    im = (int)((unsigned)y1 - (unsigned)x2);
+    re = rg_add(y0, x3);
+    im = rg_sub(y1, x2);
   }
 }
 
-// int32 stores of the three output orders with the index arithmetic split
-// into a per-tile 64-bit base and 32-bit lane, row and column terms.  With
-// m = 2i + p1 (row input) and n = 2j + p2 (column input), this lane's element
-// (a, b, q) has p1 = l & 1, p2 = (l >> 1) & 1 and stations
+__device__ __forceinline__ void store_vis(const XgJob &J, int f, int m, int n, int re, int im) {
+  store_int(J, f, m, n, re, im);
+}
+__device__ __forceinline__ void store_vis(const XgJob &J, int f, int m, int n, float re, float im) {
+  store_flt(J, f, m, n, re, im);
+}
+
+// 32-bit (int32 or float) stores of the three output orders with the index
+// arithmetic split into a per-tile 64-bit base and 32-bit lane, row and
+// column terms.  With m = 2i + p1 (row input) and n = 2j + p2 (column input),
+// this lane's element (a, b, q) has p1 = l & 1, p2 = (l >> 1) & 1 and stations
 //   i = rb*64 + 32wr + (l >> 5) + 8a + 2q,   j = cb*64 + 32wc + (l%32 >> 2) + 8b.
 // Index formulas as in vis_index (reference src/kernel.cuh:49-117).
-template <int ORDER>
-__device__ __forceinline__ void rg_store_i32(const XgJob &J, const TileRef &T, v16i (&acc)[4][4],
-                                             int wr, int wc, int lane) {
+template <int ORDER, class Acc>
+__device__ __forceinline__ void rg_store32(const XgJob &J, const TileRef &T, Acc (&acc)[4][4],
+                                           int wr, int wc, int lane) {
+  typedef typename RgElem<Acc>::T E;
+  typedef typename RgElem<Acc>::S S;
+  typedef S S2 __attribute__((ext_vector_type(2)));
   const bool odd = lane & 1;
   const int hi = lane >> 5, l31 = lane & 31;
   const int p1 = odd, p2 = (l31 >> 1) & 1;
@@ -599,22 +705,22 @@ __device__ __forceinline__ void rg_store_i32(const XgJob &J, const TileRef &T, v
   const bool full = !diag && (T.rb * 64 + 64 <= ns);  // rb > cb: every (i, j) valid, i > j
   const int accum = J.accumulate;
   const long long plane = J.plane;
-  unsigned *base;
+  S *base;
   unsigned kl;  // lane term
   if (ORDER == XG_ORDER_REGTILE) {
     // l = (f*4 + 2ry + rx) RT + R(R+1)/2 + C, element 4l + 2p1 + p2, imag at +plane;
     // R = i >> 1 = rb*32 + 16wr + 4a + q, rx = l >> 5, C = j >> 1, ry = j & 1
     const long long RT = (long long)(ns >> 1) * ((ns >> 1) + 1) / 2;
-    base = (unsigned *)J.out + 4 * ((long long)T.f * 4 * RT + T.cb * 32 + 16 * wc);
+    base = (S *)J.out + 4 * ((long long)T.f * 4 * RT + T.cb * 32 + 16 * wc);
     const int ry = (l31 >> 2) & 1;
     kl = (unsigned)(4 * ((2 * ry + hi) * RT) + 2 * p1 + p2 + 4 * (l31 >> 3));
   } else {
     const long long NB = (long long)ns * (ns + 1) / 2;
     if (ORDER == XG_ORDER_TRI) {  // element 8k + 4p1 + 2p2, imag at +1
-      base = (unsigned *)J.out + 8 * ((long long)T.f * NB + T.cb * 64 + 32 * wc);
+      base = (S *)J.out + 8 * ((long long)T.f * NB + T.cb * 64 + 32 * wc);
       kl = (unsigned)(4 * p1 + 2 * p2 + 8 * (l31 >> 2));
     } else {                      // REAL_IMAG: element 4k + 2p1 + p2, imag at +plane
-      base = (unsigned *)J.out + 4 * ((long long)T.f * NB + T.cb * 64 + 32 * wc);
+      base = (S *)J.out + 4 * ((long long)T.f * NB + T.cb * 64 + 32 * wc);
       kl = (unsigned)(2 * p1 + p2 + 4 * (l31 >> 2));
     }
   }
@@ -627,7 +733,7 @@ __device__ __forceinline__ void rg_store_i32(const XgJob &J, const TileRef &T, v
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        int re, im;
+        E re, im;
         rg_combine(acc[a][b], q, odd, re, im);
         const int i = i0 + 8 * a + 2 * q, j = j0 + 8 * b;
         unsigned off;
@@ -650,46 +756,45 @@ __device__ __forceinline__ void rg_store_i32(const XgJob &J, const TileRef &T, v
           }
         }
         if (zero) re = im = 0;
-        unsigned *p = base + off;
+        S *p = base + off;
         if (ORDER == XG_ORDER_TRI) {
-          uint2 v = make_uint2((unsigned)re, (unsigned)im);
+          S2 w;
+          w.x = (S)re;
+          w.y = (S)im;
           if (accum) {
-            const uint2 o = *(const uint2 *)p;
-            v.x += o.x;
-            v.y += o.y;
+            const S2 o = *(const S2 *)p;
+            w.x += o.x;
+            w.y += o.y;
           }
-          typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-          u32x2 w;
-          w.x = v.x;
-          w.y = v.y;
-          __builtin_nontemporal_store(w, (u32x2 *)p);
+          __builtin_nontemporal_store(w, (S2 *)p);
         } else {
           if (accum) {
-            p[0] += (unsigned)re;
-            p[plane] += (unsigned)im;
+            p[0] += (S)re;
+            p[plane] += (S)im;
           } else {
             // streaming stores: the tile's output is not read again by this
             // launch and should not evict the input the next round reuses
-            __builtin_nontemporal_store((unsigned)re, p);
-            __builtin_nontemporal_store((unsigned)im, p + plane);
+            __builtin_nontemporal_store((S)re, p);
+            __builtin_nontemporal_store((S)im, p + plane);
           }
         }
       }
     }
 }
 
-__device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, v16i (&acc)[4][4],
+template <class Acc>
+__device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, Acc (&acc)[4][4],
                                             int wr, int wc, int lane) {
-  if (J.out_kind == XG_OUT_I32) {
+  if (J.out_kind == RgElem<Acc>::kOut32) {
     if (J.order == XG_ORDER_REGTILE)
-      rg_store_i32<XG_ORDER_REGTILE>(J, T, acc, wr, wc, lane);
+      rg_store32<XG_ORDER_REGTILE>(J, T, acc, wr, wc, lane);
     else if (J.order == XG_ORDER_TRI)
-      rg_store_i32<XG_ORDER_TRI>(J, T, acc, wr, wc, lane);
+      rg_store32<XG_ORDER_TRI>(J, T, acc, wr, wc, lane);
     else
-      rg_store_i32<XG_ORDER_REALIMAG>(J, T, acc, wr, wc, lane);
+      rg_store32<XG_ORDER_REALIMAG>(J, T, acc, wr, wc, lane);
     return;
   }
-  const bool odd = lane & 1;  // int64 output: generic path
+  const bool odd = lane & 1;  // int64 / double output: generic path
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -697,12 +802,12 @@ __device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, v1
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        int re, im;
+        typename RgElem<Acc>::T re, im;
         rg_combine(acc[a][b], q, odd, re, im);
         const int R0 = 128 * wr + 32 * a + 8 * q + 4 * (lane >> 5);  // even real row
         const int m = T.rb * XG_TILE + (R0 >> 1) + (odd ? 1 : 0);
         const int n = T.cb * XG_TILE + ((128 * wc + 32 * b + (lane & 31)) >> 1);
-        store_int(J, T.f, m, n, re, im);
+        store_vis(J, T.f, m, n, re, im);
       }
     }
 }
@@ -713,16 +818,16 @@ __device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, v1
 #define SG_DS_READ 0x100
 #define SG_DS_WRITE 0x200
 
-template <int STG>
+template <class E, int STG>
 __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsigned char *lds) {
-  typedef RgCfg<STG> C;
+  typedef RgCfg<E, STG> C;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ninput = 2 * J.nstation;
   const bool diag = (T.rb == T.cb);
   const int nstage = (J.ntime + STG - 1) / STG;
 
   // Staging: thread -> (chunk, first sample row); load i covers sample
-  // srow + 16 i of the stage.  Each load has its own buffer resource, based at
+  // srow + kPass i of the stage.  Each load has its own buffer resource, based at
   // its first sample row and sized to end at the block's last valid sample, and
   // a zero scalar offset: the range check (per-lane offset vs size) then zeroes
   // every sample past the block and every invalid chunk, and all offsets stay
@@ -732,22 +837,25 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
   // make the compiler reload them from the kernel arguments every stage.
   const int ntime = J.ntime;
   const int ts = (int)J.tstride;  // < 2^24: the host keeps 128 * tstride < 2^31
-  const int chunk = tid & 15, srow = tid >> 4;
-  const unsigned char *in_f = (const unsigned char *)J.input + (long long)T.f * J.nstation * 4;
-  const unsigned char *row_base = in_f + T.rb * RG_ROW;
-  const unsigned char *col_base = in_f + T.cb * RG_ROW;
-  const long long load_step = 16LL * ts;           // bytes between loads i and i+1
+  const int chunk = tid % E::kChunks, srow = tid / E::kChunks;
+  // a sample row of one tile side holds 128 inputs: E::kRow / 128 bytes per input
+  const unsigned char *in_f =
+      (const unsigned char *)J.input + (long long)T.f * ninput * (E::kRow / XG_TILE);
+  const unsigned char *row_base = in_f + T.rb * E::kRow;
+  const unsigned char *col_base = in_f + T.cb * E::kRow;
+  const long long load_step = (long long)C::kPass * ts;  // bytes between loads i and i+1
   const long long stage_step = (long long)STG * ts;  // bytes between stages
   // bytes of a sample row that a valid chunk can touch (<= tstride)
-  const int row_bytes = ts < RG_ROW ? ts : RG_ROW;
-  // chunk covers inputs blk*128 + 8*chunk .. +7 (ninput % 8 == 0 here); an
-  // invalid chunk gets an offset past any resource's size and reads zeros
+  const int row_bytes = ts < E::kRow ? ts : E::kRow;
+  // chunk covers inputs blk*128 + kInputs*chunk and the next kInputs - 1
+  // (ninput is a multiple of kInputs here); an invalid chunk gets an offset
+  // past any resource's size and reads zeros
   const int voff = srow * ts + 16 * chunk;
-  const int voff0 = (T.rb * XG_TILE + 8 * chunk < ninput) ? voff : 0x7fffff00;
+  const int voff0 = (T.rb * XG_TILE + E::kInputs * chunk < ninput) ? voff : 0x7fffff00;
   // diagonal tiles read their columns from the row side; the column loads then
   // fall outside the buffer and return zeros (no branch in the stage loop)
-  const int voff1 = (!diag && T.cb * XG_TILE + 8 * chunk < ninput) ? voff : 0x7fffff00;
-  const int wpos = srow * RG_ROW + (rg_slot(chunk, srow & 7) << 4);  // LDS write offset
+  const int voff1 = (!diag && T.cb * XG_TILE + E::kInputs * chunk < ninput) ? voff : 0x7fffff00;
+  const int wpos = srow * E::kRow + (E::slot(chunk, srow) << 4);  // LDS write offset
 
   // two staging register sets (a: even stages, b: odd stages)
   v4i ra0[C::kLoads], ra1[C::kLoads], rb0[C::kLoads], rb1[C::kLoads];
@@ -755,8 +863,8 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
     const long long sbase = st * stage_step;
 #pragma unroll
     for (int i = 0; i < C::kLoads; ++i) {
-      const int t0 = st * STG + 16 * i;             // first sample of this load
-      const int rows = min(ntime - t0, 16);         // valid samples of the 16
+      const int t0 = st * STG + C::kPass * i;       // first sample of this load
+      const int rows = min(ntime - t0, C::kPass);   // valid samples of the kPass
       // (rows - 1) full sample rows + the last one's reachable bytes; <= 0: none
       const int nrec = max(rows * ts - ts + row_bytes, 0);
       const long long boff = sbase + i * load_step;
@@ -772,8 +880,8 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
     unsigned char *b = lds + buf * C::kBuf;
 #pragma unroll
     for (int i = 0; i < C::kLoads; ++i) {
-      *(v4i *)(b + wpos + 16 * i * RG_ROW) = r0[i];
-      *(v4i *)(b + C::kSide + wpos + 16 * i * RG_ROW) = r1[i];
+      *(v4i *)(b + wpos + C::kPass * i * E::kRow) = r0[i];
+      *(v4i *)(b + C::kSide + wpos + C::kPass * i * E::kRow) = r1[i];
     }
   };
 
@@ -784,10 +892,10 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
   const bool active = !(diag && wr < wc);
   int aoff[4], boff[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) aoff[a] = rg_lane_off(8 * wr + 2 * a, lane);
+  for (int a = 0; a < 4; ++a) aoff[a] = E::lane_off(4 * wr + a, lane);
 #pragma unroll
-  for (int b = 0; b < 4; ++b) boff[b] = rg_lane_off(8 * wc + 2 * b, lane) + (diag ? 0 : C::kSide);
-  v16i acc[4][4];
+  for (int b = 0; b < 4; ++b) boff[b] = E::lane_off(4 * wc + b, lane) + (diag ? 0 : C::kSide);
+  typename E::Acc acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -798,9 +906,9 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
   v4i A0[4], B0[4], A1[4], B1[4];
   auto lread = [&](const unsigned char *base, v4i *A, v4i *B) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) A[a] = rg_operand(base + aoff[a]);
+    for (int a = 0; a < 4; ++a) A[a] = E::operand(base + aoff[a]);
 #pragma unroll
-    for (int b = 0; b < 4; ++b) B[b] = rg_operand(base + boff[b]);
+    for (int b = 0; b < 4; ++b) B[b] = E::operand(base + boff[b]);
   };
   // One kstep: the 16 MFMAs on (A, B), with the 16 transpose reads of the next
   // kstep into (NA, NB) over the first 8 MFMAs and, when FILL, the staging
@@ -812,8 +920,7 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
     constexpr int kFill = 2 * C::kLoads / 8;  // staging writes (and loads) per MFMA
 #pragma unroll
     for (int i = 0; i < 16; ++i)
-      acc[i >> 2][i & 3] =
-          __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i >> 2], B[i & 3], acc[i >> 2][i & 3], 0, 0, 0);
+      acc[i >> 2][i & 3] = E::mfma(A[i >> 2], B[i & 3], acc[i >> 2][i & 3]);
     lread(nbase, NA, NB);
     fill();
 #pragma unroll
@@ -862,9 +969,9 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
 #pragma unroll
     for (int ks = 0; ks < C::kSteps - 1; ++ks) {
       if (ks & 1)
-        kstep(no(), A1, B1, A0, B0, cur + 32 * (ks + 1) * RG_ROW, none);
+        kstep(no(), A1, B1, A0, B0, cur + E::kK * (ks + 1) * E::kRow, none);
       else
-        kstep(no(), A0, B0, A1, B1, cur + 32 * (ks + 1) * RG_ROW, none);
+        kstep(no(), A0, B0, A1, B1, cur + E::kK * (ks + 1) * E::kRow, none);
     }
     rg_barrier();  // every wave is done reading this stage's buffer
     // last kstep: reads kstep 0 of stage s+1 into set 0, writes stage s+2 into
@@ -885,17 +992,31 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
   if (active) rg_epilogue(J, T, acc, wr, wc, lane);
 }
 
-template <int STG>
-__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
-void xg_xengine_rg(XgLaunch L) {
-  __shared__ __attribute__((aligned(16))) unsigned char lds[RgCfg<STG>::kLds];
+template <class E, int STG>
+__device__ __forceinline__ void rg_run(const XgLaunch &L, unsigned char *lds) {
   const int xcd = blockIdx.x % XG_NXCD;
   const int wl = blockIdx.x / XG_NXCD;
   TileRef T;
   for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
-    rg_tile<STG>(L.jobs[T.job], T, lds);
+    rg_tile<E, STG>(L.jobs[T.job], T, lds);
     rg_barrier();  // every wave is done with this tile's LDS buffers
   }
+}
+
+// int8 input (nstation % 4 == 0)
+template <int STG>
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
+void xg_xengine_rg(XgLaunch L) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[RgCfg<RgInt8, STG>::kLds];
+  rg_run<RgInt8, STG>(L, lds);
+}
+
+// fp16 input: v_mfma_f32_32x32x16_f16, float accumulation
+template <int STG>
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
+void xg_xengine_rgh(XgLaunch L) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[RgCfg<RgHalf, STG>::kLds];
+  rg_run<RgHalf, STG>(L, lds);
 }
 
 
@@ -1357,6 +1478,12 @@ extern "C" int xgh_launch_xengine(int family, const XgLaunch *L, void *stream) {
         hipLaunchKernelGGL(xg_xengine_rg<128>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       else
         hipLaunchKernelGGL(xg_xengine_rg<64>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      break;
+    case XG_KERNEL_F16_RG:
+      if (L->stage == 1)
+        hipLaunchKernelGGL(xg_xengine_rgh<64>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      else
+        hipLaunchKernelGGL(xg_xengine_rgh<32>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       break;
     default:
       return xgh_report(hipErrorInvalidValue, "xgh_launch_xengine(family)");

@@ -41,6 +41,7 @@
 #define XG_KERNEL_F16_MFMA  1
 #define XG_KERNEL_F32_PLAIN 2
 #define XG_KERNEL_I8_RG     3  /* int8 real-Gram MFMA (nstation % 4 == 0) */
+#define XG_KERNEL_F16_RG    4  /* fp16 real-Gram MFMA (16-byte aligned input) */
 
 typedef struct XgJob {
   const void *input;   /* sample (t=0,f=0,s=0,p=0) of this block */
@@ -67,7 +68,8 @@ typedef struct XgLaunch {
   char stage;                                   /* real-Gram kernel: 0 / 1 = register
                                                    staging, 64 / 128 samples per stage;
                                                    3 / 4 = global->LDS DMA staging, 64 /
-                                                   32 samples per stage */
+                                                   32 samples per stage; fp16: 0 / 1 =
+                                                   32 / 64 samples per stage */
   int xcd_start[XG_NXCD][XG_MAX_JOBS + 1];      /* per-XCD prefix of tile counts */
 } XgLaunch;
 
