@@ -659,14 +659,27 @@ __device__ __forceinline__ int rg_sub(int a, int b) { return (int)((unsigned)a -
 __device__ __forceinline__ float rg_add(float a, float b) { return a + b; }
 __device__ __forceinline__ float rg_sub(float a, float b) { return a - b; }
 
-template <class Acc, class T = typename RgElem<Acc>::T>
-__device__ __forceinline__ void rg_combine(const Acc &blk, int q, bool odd, T &re, T &im) {
+// Exchange of a value between lanes l and l ^ 1 through ds_bpermute, with an
+// address the caller computed from its own lane index.
+struct RgBperm {
+  int addr;  // 4 * (lane ^ 1)
+  __device__ __forceinline__ int operator()(int x) const {
+    return __builtin_amdgcn_ds_bpermute(addr, x);
+  }
+  __device__ __forceinline__ float operator()(float x) const {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
+  }
+};
+
+template <class Acc, class SW, class T = typename RgElem<Acc>::T>
+__device__ __forceinline__ void rg_combine(const Acc &blk, int q, bool odd, const SW &swap, T &re,
+                                           T &im) {
   const T x0 = blk[4 * q + 0], x1 = blk[4 * q + 1];
   const T x2 = blk[4 * q + 2], x3 = blk[4 * q + 3];
   // each lane sends what its partner lacks: odd lanes their e%4 = 0,1
   // (column n_im of row m), even lanes their e%4 = 2,3 (column n_re of m+1)
-  const T y0 = __shfl_xor(odd ? x0 : x2, 1);
-  const T y1 = __shfl_xor(odd ? x1 : x3, 1);
+  const T y0 = swap(odd ? x0 : x2);
+  const T y1 = swap(odd ? x1 : x3);
   if (!odd) {  // x0 = G[mr][nr], x1 = G[mi][nr], y0 = G[mr][ni], y1 = G[mi][ni]
     re = rg_add(x0, y1);
     im = rg_sub(x1, y0);
@@ -689,9 +702,9 @@ __device__ __forceinline__ void store_vis(const XgJob &J, int f, int m, int n, f
 // this lane's element (a, b, q) has p1 = l & 1, p2 = (l >> 1) & 1 and stations
 //   i = rb*64 + 32wr + (l >> 5) + 8a + 2q,   j = cb*64 + 32wc + (l%32 >> 2) + 8b.
 // Index formulas as in vis_index (reference src/kernel.cuh:49-117).
-template <int ORDER, class Acc>
+template <int ORDER, class Acc, class SW>
 __device__ __forceinline__ void rg_store32(const XgJob &J, const TileRef &T, Acc (&acc)[4][4],
-                                           int wr, int wc, int lane) {
+                                           int wr, int wc, int lane, const SW &swap) {
   typedef typename RgElem<Acc>::T E;
   typedef typename RgElem<Acc>::S S;
   typedef S S2 __attribute__((ext_vector_type(2)));
@@ -734,7 +747,7 @@ __device__ __forceinline__ void rg_store32(const XgJob &J, const TileRef &T, Acc
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         E re, im;
-        rg_combine(acc[a][b], q, odd, re, im);
+        rg_combine(acc[a][b], q, odd, swap, re, im);
         const int i = i0 + 8 * a + 2 * q, j = j0 + 8 * b;
         unsigned off;
         bool zero = false;
@@ -782,16 +795,16 @@ __device__ __forceinline__ void rg_store32(const XgJob &J, const TileRef &T, Acc
     }
 }
 
-template <class Acc>
+template <class Acc, class SW>
 __device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, Acc (&acc)[4][4],
-                                            int wr, int wc, int lane) {
+                                            int wr, int wc, int lane, const SW &swap) {
   if (J.out_kind == RgElem<Acc>::kOut32) {
     if (J.order == XG_ORDER_REGTILE)
-      rg_store32<XG_ORDER_REGTILE>(J, T, acc, wr, wc, lane);
+      rg_store32<XG_ORDER_REGTILE>(J, T, acc, wr, wc, lane, swap);
     else if (J.order == XG_ORDER_TRI)
-      rg_store32<XG_ORDER_TRI>(J, T, acc, wr, wc, lane);
+      rg_store32<XG_ORDER_TRI>(J, T, acc, wr, wc, lane, swap);
     else
-      rg_store32<XG_ORDER_REALIMAG>(J, T, acc, wr, wc, lane);
+      rg_store32<XG_ORDER_REALIMAG>(J, T, acc, wr, wc, lane, swap);
     return;
   }
   const bool odd = lane & 1;  // int64 / double output: generic path
@@ -803,7 +816,7 @@ __device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, Ac
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         typename RgElem<Acc>::T re, im;
-        rg_combine(acc[a][b], q, odd, re, im);
+        rg_combine(acc[a][b], q, odd, swap, re, im);
         const int R0 = 128 * wr + 32 * a + 8 * q + 4 * (lane >> 5);  // even real row
         const int m = T.rb * XG_TILE + (R0 >> 1) + (odd ? 1 : 0);
         const int n = T.cb * XG_TILE + ((128 * wc + 32 * b + (lane & 31)) >> 1);
@@ -812,54 +825,72 @@ __device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, Ac
     }
 }
 
+// Output stage of a tile.  The lane index reaches the epilogue through an
+// opaque copy, so the values derived from it are computed here instead of
+// being hoisted out of the tile loop and spilled across it: a spill reload
+// waits for every store and load issued before it, which would serialize the
+// epilogue's stores.  On a diagonal tile the wave above the diagonal
+// (wr < wc) duplicates its mirror and stores nothing.
+template <class Acc>
+__device__ __forceinline__ void rg_finish(const XgJob &J, const TileRef &T, Acc (&acc)[4][4]) {
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  if (!(T.rb == T.cb && wr < wc)) rg_epilogue(J, T, acc, wr, wc, lane, RgBperm{4 * (lane ^ 1)});
+}
+
 // scheduling-group masks (llvm.amdgcn.sched.group.barrier)
 #define SG_MFMA 0x008
 #define SG_VMEM_READ 0x020
 #define SG_DS_READ 0x100
 #define SG_DS_WRITE 0x200
 
+// Global-load state of one tile for the register-staged real-Gram kernel.
+// Staging: thread -> (chunk, first sample row); load i covers sample
+// srow + kPass i of a stage.  Each load has its own buffer resource, based at
+// its first sample row and sized to end at the block's last valid sample, and
+// a zero scalar offset: the range check (per-lane offset vs size) then zeroes
+// every sample past the block and every invalid chunk, and all offsets stay
+// below 16 * tstride.  The fields are copies of the job's (kept in registers:
+// the barriers' memory clobber would otherwise make the compiler reload them
+// from the kernel arguments every stage).
 template <class E, int STG>
-__device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsigned char *lds) {
+struct RgLoader {
   typedef RgCfg<E, STG> C;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int ninput = 2 * J.nstation;
-  const bool diag = (T.rb == T.cb);
-  const int nstage = (J.ntime + STG - 1) / STG;
+  const unsigned char *row_base, *col_base;
+  long long load_step;   // bytes between loads i and i+1
+  long long stage_step;  // bytes between stages
+  int ntime, ts, row_bytes, voff0, voff1;
 
-  // Staging: thread -> (chunk, first sample row); load i covers sample
-  // srow + kPass i of the stage.  Each load has its own buffer resource, based at
-  // its first sample row and sized to end at the block's last valid sample, and
-  // a zero scalar offset: the range check (per-lane offset vs size) then zeroes
-  // every sample past the block and every invalid chunk, and all offsets stay
-  // below 16 * tstride.
-  //
-  // Job fields live in registers: the barriers' memory clobber would otherwise
-  // make the compiler reload them from the kernel arguments every stage.
-  const int ntime = J.ntime;
-  const int ts = (int)J.tstride;  // < 2^24: the host keeps 128 * tstride < 2^31
-  const int chunk = tid % E::kChunks, srow = tid / E::kChunks;
-  // a sample row of one tile side holds 128 inputs: E::kRow / 128 bytes per input
-  const unsigned char *in_f =
-      (const unsigned char *)J.input + (long long)T.f * ninput * (E::kRow / XG_TILE);
-  const unsigned char *row_base = in_f + T.rb * E::kRow;
-  const unsigned char *col_base = in_f + T.cb * E::kRow;
-  const long long load_step = (long long)C::kPass * ts;  // bytes between loads i and i+1
-  const long long stage_step = (long long)STG * ts;  // bytes between stages
-  // bytes of a sample row that a valid chunk can touch (<= tstride)
-  const int row_bytes = ts < E::kRow ? ts : E::kRow;
-  // chunk covers inputs blk*128 + kInputs*chunk and the next kInputs - 1
-  // (ninput is a multiple of kInputs here); an invalid chunk gets an offset
-  // past any resource's size and reads zeros
-  const int voff = srow * ts + 16 * chunk;
-  const int voff0 = (T.rb * XG_TILE + E::kInputs * chunk < ninput) ? voff : 0x7fffff00;
-  // diagonal tiles read their columns from the row side; the column loads then
-  // fall outside the buffer and return zeros (no branch in the stage loop)
-  const int voff1 = (!diag && T.cb * XG_TILE + E::kInputs * chunk < ninput) ? voff : 0x7fffff00;
-  const int wpos = srow * E::kRow + (E::slot(chunk, srow) << 4);  // LDS write offset
+  __device__ __forceinline__ RgLoader(const XgJob &J, const TileRef &T) {
+    const int tid = threadIdx.x;
+    const int ninput = 2 * J.nstation;
+    const bool diag = (T.rb == T.cb);
+    ntime = J.ntime;
+    ts = (int)J.tstride;  // < 2^24: the host keeps 128 * tstride < 2^31
+    const int chunk = tid % E::kChunks, srow = tid / E::kChunks;
+    // a sample row of one tile side holds 128 inputs: E::kRow / 128 bytes per input
+    const unsigned char *in_f =
+        (const unsigned char *)J.input + (long long)T.f * ninput * (E::kRow / XG_TILE);
+    row_base = in_f + T.rb * E::kRow;
+    col_base = in_f + T.cb * E::kRow;
+    load_step = (long long)C::kPass * ts;
+    stage_step = (long long)STG * ts;
+    // bytes of a sample row that a valid chunk can touch (<= tstride)
+    row_bytes = ts < E::kRow ? ts : E::kRow;
+    // chunk covers inputs blk*128 + kInputs*chunk and the next kInputs - 1
+    // (ninput is a multiple of kInputs here); an invalid chunk gets an offset
+    // past any resource's size and reads zeros
+    const int voff = srow * ts + 16 * chunk;
+    voff0 = (T.rb * XG_TILE + E::kInputs * chunk < ninput) ? voff : 0x7fffff00;
+    // diagonal tiles read their columns from the row side; the column loads
+    // then fall outside the buffer and return zeros (no branch in the stage loop)
+    voff1 = (!diag && T.cb * XG_TILE + E::kInputs * chunk < ninput) ? voff : 0x7fffff00;
+  }
 
-  // two staging register sets (a: even stages, b: odd stages)
-  v4i ra0[C::kLoads], ra1[C::kLoads], rb0[C::kLoads], rb1[C::kLoads];
-  auto gload = [&](v4i *r0, v4i *r1, int st) {
+  // stage st of both tile sides into (r0, r1); past the block: zeros
+  __device__ __forceinline__ void gload(v4i *r0, v4i *r1, int st) const {
     const long long sbase = st * stage_step;
 #pragma unroll
     for (int i = 0; i < C::kLoads; ++i) {
@@ -875,7 +906,22 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
       r0[i] = __builtin_amdgcn_raw_buffer_load_b128(rs0, voff0, 0, 0);
       r1[i] = __builtin_amdgcn_raw_buffer_load_b128(rs1, voff1, 0, 0);
     }
-  };
+  }
+};
+
+// MFMA pipeline of one tile.  On entry the staging registers hold stages 0
+// (set a) and 1 (set b), or have their loads in flight; on exit acc holds the
+// tile's real Gram block and the staging registers hold loads past the block.
+template <class E, int STG>
+__device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, const RgLoader<E, STG> &G,
+                                        unsigned char *lds, v4i *ra0, v4i *ra1, v4i *rb0,
+                                        v4i *rb1, typename E::Acc (&acc)[4][4]) {
+  typedef RgCfg<E, STG> C;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool diag = (T.rb == T.cb);
+  const int nstage = (J.ntime + STG - 1) / STG;
+  const int chunk = tid % E::kChunks, srow = tid / E::kChunks;
+  const int wpos = srow * E::kRow + (E::slot(chunk, srow) << 4);  // LDS write offset
   auto lwrite = [&](const v4i *r0, const v4i *r1, int buf) {
     unsigned char *b = lds + buf * C::kBuf;
 #pragma unroll
@@ -885,17 +931,12 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
     }
   };
 
-  // On a diagonal tile the wave above the diagonal (wr < wc) duplicates its
-  // mirror; it runs the same MFMAs anyway (one wave per SIMD, so skipping them
-  // saves no time) and only skips the stores.
   const int wr = wave >> 1, wc = wave & 1;
-  const bool active = !(diag && wr < wc);
   int aoff[4], boff[4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) aoff[a] = E::lane_off(4 * wr + a, lane);
 #pragma unroll
   for (int b = 0; b < 4; ++b) boff[b] = E::lane_off(4 * wc + b, lane) + (diag ? 0 : C::kSide);
-  typename E::Acc acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -947,20 +988,19 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
   // issues the loads of stage s+4 into the same registers, so a global load
   // has two whole stages to land.  Even stages use register set a, odd ones
   // set b (the stage loop is unrolled by two; a block with an odd number of
-  // stages runs one more stage of zeros).  Prologue: stages 0 and 1 in LDS,
-  // stages 2 and 3 in flight, kstep 0 operands in set 0.
+  // stages runs one more stage of zeros).  Prologue: stages 0 and 1 from the
+  // staging registers into LDS, stages 2 and 3 in flight, kstep 0 operands in
+  // set 0.
   //
   // The stage body has no branches (a branch around the MFMAs makes the
   // compiler copy the accumulators): past the last stage the global loads have
   // an empty range and read zeros without touching memory, and the staging
   // writes and operand reads go to the idle LDS buffer.
-  gload(ra0, ra1, 0);
   lwrite(ra0, ra1, 0);
-  gload(ra0, ra1, 1);
-  lwrite(ra0, ra1, 1);
-  gload(ra0, ra1, 2);
+  G.gload(ra0, ra1, 2);
   __builtin_amdgcn_sched_barrier(0);  // set a's loads first: they are waited for first
-  gload(rb0, rb1, 3);
+  lwrite(rb0, rb1, 1);
+  G.gload(rb0, rb1, 3);
   rg_barrier();
   lread(lds, A0, B0);
   auto stage = [&](int s, const unsigned char *cur, const unsigned char *nxt, v4i *r0, v4i *r1,
@@ -978,7 +1018,7 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
     // this stage's buffer and issues the global loads of stage s+4
     auto refill = [&] {
       lwrite(r0, r1, buf);
-      gload(r0, r1, s + 4);
+      G.gload(r0, r1, s + 4);
     };
     if constexpr ((C::kSteps - 1) & 1)
       kstep(yes(), A1, B1, A0, B0, nxt, refill);
@@ -989,17 +1029,38 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, unsign
     stage(s, lds, lds + C::kBuf, ra0, ra1, 0);
     stage(s + 1, lds + C::kBuf, lds, rb0, rb1, 1);
   }
-  if (active) rg_epilogue(J, T, acc, wr, wc, lane);
 }
 
+// Persistent loop of the register-staged real-Gram kernels.  The first two
+// stages of the next tile are loaded into the staging registers while the
+// current tile's output is stored, so a tile starts with its data on chip.
 template <class E, int STG>
 __device__ __forceinline__ void rg_run(const XgLaunch &L, unsigned char *lds) {
+  typedef RgCfg<E, STG> C;
   const int xcd = blockIdx.x % XG_NXCD;
   const int wl = blockIdx.x / XG_NXCD;
   TileRef T;
-  for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
-    rg_tile<E, STG>(L.jobs[T.job], T, lds);
+  if (!xg_tile(L, xcd, wl, T)) return;
+  v4i ra0[C::kLoads], ra1[C::kLoads], rb0[C::kLoads], rb1[C::kLoads];
+  RgLoader<E, STG> G(L.jobs[T.job], T);
+  G.gload(ra0, ra1, 0);
+  G.gload(rb0, rb1, 1);
+  for (int k = wl;; k += L.wg_per_xcd) {
+    typename E::Acc acc[4][4];
+    rg_tile<E, STG>(L.jobs[T.job], T, G, lds, ra0, ra1, rb0, rb1, acc);
+    TileRef N;
+    const bool more = xg_tile(L, xcd, k + L.wg_per_xcd, N);
+    if (more) {
+      G = RgLoader<E, STG>(L.jobs[N.job], N);
+      G.gload(ra0, ra1, 0);
+      G.gload(rb0, rb1, 1);
+    }
+    // (a diagonal tile's wave above the diagonal runs the same MFMAs as its
+    // mirror: one wave per SIMD, so skipping them would save no time)
+    rg_finish(L.jobs[T.job], T, acc);
     rg_barrier();  // every wave is done with this tile's LDS buffers
+    if (!more) break;
+    T = N;
   }
 }
 
@@ -1105,7 +1166,6 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
   };
 
   const int wr = wave >> 1, wc = wave & 1;
-  const bool active = !(diag && wr < wc);
   int aoff[4], boff[4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) aoff[a] = rg_lane_off(8 * wr + 2 * a, lane);
@@ -1196,7 +1256,7 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
   // DMAs of stages past the block (zero-filled) may still be in flight: drain
   // them before the next tile reuses the buffers
   rgd_barrier<0>();
-  if (active) rg_epilogue(J, T, acc, wr, wc, lane);
+  rg_finish(J, T, acc);
 }
 
 template <int STG, int NBUF>
