@@ -49,6 +49,10 @@ typedef struct XGPUAmdConfigStruct {
   unsigned int matrix_order;  /* TRIANGULAR_ORDER, REAL_IMAG_TRIANGULAR_ORDER or
                                  REGISTER_TILE_TRIANGULAR_ORDER */
   int device;
+  unsigned int complex_block_size;  /* input layout: 0 or 1 = plain [t][f][s][p]; 32 = the
+                                       COMPLEX_BLOCK_SIZE=32 layout (per 32-entry block, the
+                                       real parts of 16 stations' (X, Y), then their
+                                       imaginary parts); int8 input, nstation % 16 == 0 */
 } XGPUAmdConfig;
 
 typedef struct XGPUAmdEngineStruct XGPUAmdEngine;

@@ -34,6 +34,7 @@ def lib() -> C.CDLL:
             "oracle_reorder": [I, I, P, P, I, I, LL],
             "oracle_regtile_from_tri": [I, P, P, I, I, LL],
             "oracle_swizzle": [P, P, I, I, I],
+            "oracle_unblock32": [P, P, C.c_size_t],
             "oracle_extract": [P, P, I, I],
             "oracle_random_legacy": [P, C.c_size_t, C.c_uint],
             "oracle_fill_uniform": [P, C.c_size_t, C.c_uint64, I, I],
@@ -132,6 +133,22 @@ def swizzle(x, ns, nf, nt) -> np.ndarray:
     out = np.empty_like(x)
     lib().oracle_swizzle(_p(out), _p(x), ns, nf, nt)
     return out
+
+
+def unblock32(x: np.ndarray) -> np.ndarray:
+    """COMPLEX_BLOCK_SIZE=32 int8 input -> plain [t][f][s][p][re,im] input."""
+    x = np.ascontiguousarray(x, dtype=np.int8).reshape(-1)
+    if x.size % 64:
+        raise ValueError("block-32 input is a whole number of 64-byte blocks")
+    out = np.empty_like(x)
+    lib().oracle_unblock32(_p(out), _p(x), x.size)
+    return out
+
+
+def block32(x: np.ndarray) -> np.ndarray:
+    """Inverse of unblock32: plain input -> COMPLEX_BLOCK_SIZE=32 layout."""
+    x = np.ascontiguousarray(x, dtype=np.int8).reshape(-1, 32, 2)
+    return np.ascontiguousarray(x.transpose(0, 2, 1)).reshape(-1)
 
 
 def extract(tri: np.ndarray, ns: int, nf: int) -> np.ndarray:

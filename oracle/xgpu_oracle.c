@@ -269,6 +269,14 @@ void oracle_swizzle(int8_t *out, const int8_t *in, int ns, int nf, int nt) {
                 in[((((size_t)t * nf + f) * ns + s) * 2 + p) * 2 + c];
 }
 
+void oracle_unblock32(int8_t *out, const int8_t *in, size_t nbytes) {
+  for (size_t b = 0; b < nbytes; b += 64)
+    for (int q = 0; q < 32; ++q) {
+      out[b + 2 * q] = in[b + q];
+      out[b + 2 * q + 1] = in[b + 32 + q];
+    }
+}
+
 void oracle_extract(float *full, const float *tri, int ns, int nf) {
   const long long nb = (long long)ns * (ns + 1) / 2;
   for (int f = 0; f < nf; ++f)

@@ -22,6 +22,9 @@
  *   oracle_regtile_from_tri   inverse of oracle_reorder (round trip) and the
  *                             layout formula of the reference kernel writer
  *   oracle_swizzle            golden vectors of the reference xgpuSwizzleInput
+ *   oracle_unblock32          golden vectors of the reference xgpuOmpXengine
+ *                             built -DCOMPLEX_BLOCK_SIZE=32 (float and DP4A),
+ *                             through oracle_xengine_f32 / _i64 of its output
  *   oracle_extract            golden vectors of the reference xgpuExtractMatrix
  *   oracle_random_legacy      golden vectors of the reference xgpuRandomComplex
  *   oracle_fill_uniform       numpy restatement of the same integer formula
@@ -69,6 +72,14 @@ void oracle_regtile_from_tri(int elem_bytes, void *dst, const void *tri, int ns,
 
 /* dp4a input corner turn (reference src/cpu_util.c:235-255). */
 void oracle_swizzle(int8_t *out, const int8_t *in, int ns, int nf, int nt);
+
+/* COMPLEX_BLOCK_SIZE = 32 input -> plain [t][f][s][p][re,im] input.  The
+ * flat ComplexInput array is cut into blocks of 32 entries (64 bytes); entry
+ * q of a block in the plain layout (station q/2, polarization q%2 of the
+ * block's 16 stations) has its real part at byte q of the block and its
+ * imaginary part at byte 32 + q (reference src/omp_xengine.c:64-78).
+ * nbytes is a multiple of 64. */
+void oracle_unblock32(int8_t *out, const int8_t *in, size_t nbytes);
 
 /* Packed triangular -> full Hermitian matrix, float complex
  * (reference src/cpu_util.c:258-279). */

@@ -163,6 +163,17 @@ def main():
                      "omp_sum_re": float(out[0::2].astype(np.float64).sum()),
                      "omp_max": float(np.abs(out).max())}
 
+    # --- 7. COMPLEX_BLOCK_SIZE=32 input layout (float and DP4A builds) -------
+    for tag, (ns, nf, nt), seed, dtype in (
+            ("ns16_nf2_nt64_cb32", (16, 2, 64), 13, np.float32),
+            ("ns32_nf3_nt256_cb32_dp4a", (32, 3, 256), 17, np.int32)):
+        L = ref(tag)
+        x = oracle.fill_uniform(nt * nf * ns * 4, seed, -128, 127)  # block-32 bytes
+        out = omp(L, x, ns, nf, dtype)
+        np.savez_compressed(os.path.join(HERE, f"ref_{tag}.npz"), omp=out)
+        meta[tag] = {"ns": ns, "nf": nf, "nt": nt, "x_seed": seed, "x_range": [-128, 127],
+                     "x_sha256": sha(x), "layout": "COMPLEX_BLOCK_SIZE=32"}
+
     with open(os.path.join(HERE, "ref_meta.json"), "w") as fh:
         json.dump(meta, fh, indent=1, sort_keys=True)
     print("fixtures written to", HERE)

@@ -42,7 +42,8 @@ def test_xgpu_h_declares_the_reference_api():
 
 
 @pytest.mark.parametrize("variant", ["libxgpu", "libxgpu_dp4a", "libxgpu_ns512_dp4a",
-                                     "libxgpu_ns16", "libxgpu_ns16_ri", "libxgpu_ns16_tri"])
+                                     "libxgpu_ns16", "libxgpu_ns16_ri", "libxgpu_ns16_tri",
+                                     "libxgpu_ns16_cb32", "libxgpu_ns32_cb32_dp4a"])
 def test_drop_in_library_exports_every_xgpu_h_symbol(variant):
     have = exported(_lib.lib_path(variant + ".so"))
     missing = [n for n in declared_functions("xgpu.h") if n not in have]
@@ -97,6 +98,15 @@ def test_xgpuinfo_sizes(variant, ns, nf, nt, ntp, order, compute):
     assert (i.shared_atomic_size, i.complex_block_size) == (4, 1)
 
 
+@pytest.mark.parametrize("variant,ns,compute", [("libxgpu_ns16_cb32", 16, X.XGPU_FLOAT32),
+                                                ("libxgpu_ns32_cb32_dp4a", 32, X.XGPU_INT8)])
+def test_block32_builds_report_their_layout(variant, ns, compute):
+    i = sizing(variant)
+    assert (i.nstation, i.complex_block_size, i.compute_type) == (ns, 32, compute)
+    r = X.info(ns, 2, compute=compute, complex_block_size=32)
+    assert r.complex_block_size == 32 and X.info(ns, 2).complex_block_size == 1
+
+
 def test_runtime_info_matches_build_time_info():
     for variant, ns, nf, nt, ntp, compute in (("libxgpu", 256, 10, 1024, 128, X.XGPU_FLOAT32),
                                               ("libxgpu_ns512_dp4a", 512, 64, 8192, 256, X.XGPU_INT8)):
@@ -111,7 +121,10 @@ def test_runtime_info_rejects_bad_configs():
                dict(nstation=6, nfrequency=1),          # register tile needs nstation % 4
                dict(nstation=16, nfrequency=0),
                dict(nstation=16, nfrequency=1, ntime=100, ntimepipe=33),
-               dict(nstation=16, nfrequency=1, compute=9)):
+               dict(nstation=16, nfrequency=1, compute=9),
+               dict(nstation=16, nfrequency=1, complex_block_size=16),
+               dict(nstation=24, nfrequency=1, complex_block_size=32),  # needs nstation % 16
+               dict(nstation=16, nfrequency=1, compute=X.XGPU_FLOAT16, complex_block_size=32)):
         with pytest.raises(X.XgpuError) as e:
             X.info(**kw)
         assert e.value.code == _lib.XGPU_INVALID_CONFIG
@@ -192,6 +205,21 @@ def test_omp_xengine_float_matches_reference():
         x = np.ascontiguousarray(g[key_in])
         lib.xgpuOmpXengine(out.ctypes.data, x.ctypes.data)
         assert np.array_equal(out, g[key_out])
+
+
+@pytest.mark.parametrize("variant,tag,dtype", [
+    ("libxgpu_ns16_cb32", "ns16_nf2_nt64_cb32", np.float32),
+    ("libxgpu_ns32_cb32_dp4a", "ns32_nf3_nt256_cb32_dp4a", np.int32)])
+def test_omp_xengine_block32_matches_reference(meta, variant, tag, dtype):
+    """COMPLEX_BLOCK_SIZE=32 builds read the blocked input like the reference
+    built the same way (golden output of its xgpuOmpXengine)."""
+    m = meta[tag]
+    ns, nf, nt = m["ns"], m["nf"], m["nt"]
+    x = oracle.fill_uniform(nt * nf * ns * 4, m["x_seed"], *m["x_range"])
+    ref = golden(f"ref_{tag}.npz")["omp"]
+    out = np.zeros_like(ref)
+    X.variant(variant).xgpuOmpXengine(out.ctypes.data, x.ctypes.data)
+    assert out.dtype == dtype and np.array_equal(out, ref)
 
 
 def test_omp_xengine_dp4a_is_exact():

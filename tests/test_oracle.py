@@ -243,6 +243,36 @@ def test_swizzle_matches_reference(meta):
     assert meta["ns16_nf2_nt64"]["swizzle_text"] == "Swizzling input\n"
 
 
+@pytest.mark.parametrize("tag", ["ns16_nf2_nt64_cb32", "ns32_nf3_nt256_cb32_dp4a"])
+def test_unblock32_matches_reference_block32_build(meta, tag):
+    """The reference CPU engine built -DCOMPLEX_BLOCK_SIZE=32 (float and DP4A)
+    equals the exact engine on the unblocked input (sums stay below 2^24, so
+    the float build is exact too); swapping the real and imaginary halves of
+    each block, or leaving the input blocked, does not."""
+    m = meta[tag]
+    ns, nf, nt = m["ns"], m["nf"], m["nt"]
+    x = oracle.fill_uniform(nt * nf * ns * 4, m["x_seed"], *m["x_range"])
+    assert sha(x) == m["x_sha256"]
+    ref = golden(f"ref_{tag}.npz")["omp"].astype(np.int64)
+    assert np.array_equal(oracle.xengine_i64(oracle.unblock32(x), ns, nf, nt), ref)
+    swapped = x.reshape(-1, 2, 32)[:, ::-1, :].reshape(-1)
+    assert not np.array_equal(oracle.xengine_i64(oracle.unblock32(swapped), ns, nf, nt), ref)
+    assert not np.array_equal(oracle.xengine_i64(x, ns, nf, nt), ref)
+
+
+def test_unblock32_layout_and_inverse():
+    """Byte q of a 64-byte block is the real part of plain entry q, byte
+    32 + q its imaginary part; block32 inverts unblock32."""
+    x = np.arange(128, dtype=np.int64).astype(np.int8)
+    u = oracle.unblock32(x)
+    for b in (0, 64):
+        for q in range(32):
+            assert u[b + 2 * q] == x[b + q] and u[b + 2 * q + 1] == x[b + 32 + q]
+    y = oracle.fill_uniform(64 * 37, 3, -128, 127)
+    assert np.array_equal(oracle.block32(oracle.unblock32(y)), y)
+    assert np.array_equal(oracle.unblock32(oracle.block32(y)), y)
+
+
 def test_extract_matches_reference():
     g = golden("ref_ns16_nf2_nt64.npz")
     assert np.array_equal(oracle.extract(g["omp_legacy"], 16, 2), g["full"])

@@ -30,18 +30,20 @@ def cmac_per_integration(nstation: int, nfreq: int, ntime: int, npol: int = 2) -
 
 
 def _config(nstation, nfrequency, ntime=0, ntimepipe=0, compute=XGPU_INT8,
-            order=REGISTER_TILE_TRIANGULAR_ORDER, device=0) -> XGPUAmdConfig:
+            order=REGISTER_TILE_TRIANGULAR_ORDER, device=0, complex_block_size=1) -> XGPUAmdConfig:
     c = XGPUAmdConfig()
     c.npol, c.nstation, c.nfrequency = 2, nstation, nfrequency
     c.ntime, c.ntimepipe = ntime, (ntimepipe or ntime)
     c.compute_type, c.matrix_order, c.device = compute, order, device
+    c.complex_block_size = complex_block_size
     return c
 
 
 def info(nstation, nfrequency, ntime=0, ntimepipe=0, compute=XGPU_INT8,
-         order=REGISTER_TILE_TRIANGULAR_ORDER) -> XGPUInfo:
+         order=REGISTER_TILE_TRIANGULAR_ORDER, complex_block_size=1) -> XGPUInfo:
     out = XGPUInfo()
-    cfg = _config(nstation, nfrequency, ntime, ntimepipe, compute, order)
+    cfg = _config(nstation, nfrequency, ntime, ntimepipe, compute, order,
+                  complex_block_size=complex_block_size)
     check(core().xgpuAmdInfo(C.byref(cfg), C.byref(out)))
     return out
 
@@ -94,8 +96,11 @@ class Engine:
     """One subarray's X-engine (runtime-sized), see include/xgpu_amd.h."""
 
     def __init__(self, nstation, nfrequency, ntime=0, ntimepipe=0, compute=XGPU_INT8,
-                 order=REGISTER_TILE_TRIANGULAR_ORDER, device=0):
-        self.cfg = _config(nstation, nfrequency, ntime, ntimepipe, compute, order, device)
+                 order=REGISTER_TILE_TRIANGULAR_ORDER, device=0, complex_block_size=1):
+        """complex_block_size: input layout, 1 = plain [t][f][s][p], 32 = the
+        COMPLEX_BLOCK_SIZE=32 blocked layout (int8 input, nstation % 16 == 0)."""
+        self.cfg = _config(nstation, nfrequency, ntime, ntimepipe, compute, order, device,
+                           complex_block_size)
         self.info = XGPUInfo()
         check(core().xgpuAmdInfo(C.byref(self.cfg), C.byref(self.info)))
         h = C.c_void_p()

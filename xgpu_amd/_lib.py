@@ -66,7 +66,7 @@ class XGPUAmdConfig(C.Structure):
     _fields_ = [
         ("npol", C.c_uint), ("nstation", C.c_uint), ("nfrequency", C.c_uint),
         ("ntime", C.c_uint), ("ntimepipe", C.c_uint), ("compute_type", C.c_uint),
-        ("matrix_order", C.c_uint), ("device", C.c_int),
+        ("matrix_order", C.c_uint), ("device", C.c_int), ("complex_block_size", C.c_uint),
     ]
 
 

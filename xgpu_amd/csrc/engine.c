@@ -72,6 +72,12 @@ int xgpuAmdInfo(const XGPUAmdConfig *c, XGPUInfo *info) {
   if (c->matrix_order == REGISTER_TILE_TRIANGULAR_ORDER && (c->nstation % 4))
     return xg_fail(XGPU_INVALID_CONFIG,
                    "REGISTER_TILE order needs nstation %% 4 == 0 (got %u)", c->nstation);
+  if (c->complex_block_size > 1 && c->complex_block_size != 32)
+    return xg_fail(XGPU_INVALID_CONFIG, "complex_block_size must be 0, 1 or 32 (got %u)",
+                   c->complex_block_size);
+  if (c->complex_block_size == 32 && (c->compute_type == XGPU_FLOAT16 || c->nstation % 16))
+    return xg_fail(XGPU_INVALID_CONFIG,
+                   "complex_block_size 32 needs int8 input and nstation %% 16 == 0");
   if (c->ntimepipe && c->ntime % c->ntimepipe)
     return xg_fail(XGPU_INVALID_CONFIG, "ntime (%u) must be a multiple of ntimepipe (%u)",
                    c->ntime, c->ntimepipe);
@@ -94,7 +100,7 @@ int xgpuAmdInfo(const XGPUAmdConfig *c, XGPUInfo *info) {
     info->matLength = info->triLength;
   info->matrix_order = c->matrix_order;
   info->shared_atomic_size = 4;
-  info->complex_block_size = 1;
+  info->complex_block_size = c->complex_block_size == 32 ? 32 : 1;
   return XGPU_OK;
 }
 
@@ -156,6 +162,7 @@ int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
   e->wide_kind = (e->family == XG_KERNEL_I8_MFMA) ? XG_OUT_I64 : XG_OUT_F64;
   e->bound = (e->family == XG_KERNEL_I8_MFMA) ? XG_I32_SAMPLE_BOUND : XG_F32_SPILL_SAMPLES;
   e->cplx_bytes = (e->family == XG_KERNEL_F16_MFMA) ? 4 : 2;
+  e->block32 = (cfg->complex_block_size == 32);
   e->tstride = (long long)cfg->nfrequency * cfg->nstation * 2 * (long long)e->cplx_bytes;
   e->integ_bytes = (size_t)info.matLength * 2 * 4;
   e->cleared = 1;
@@ -208,6 +215,7 @@ void xgpuAmdDestroy(XGPUAmdEngine *e) {
   xgh_free(e->wide);
   xgh_free(e->scratch);
   xgh_free(e->scratch2);
+  xgh_free(e->unblock);
   xgh_stream_destroy(e->stream);
   xgh_stream_destroy(e->copy_stream);
   free(e);
@@ -327,11 +335,30 @@ int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime) {
   return XGPU_OK;
 }
 
-int xgpuAmdCorrelate(XGPUAmdEngine *e, const void *input_d, unsigned int ntime) {
-  if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
-  if (!input_d && ntime) return xg_fail(XGPU_INVALID_CONFIG, "NULL input");
-  if (((uintptr_t)input_d) & 7) return xg_fail(XGPU_INVALID_CONFIG, "input must be 8-byte aligned");
-  HIPCALL(xgh_set_device(e->cfg.device), "set device");
+/* Convert ntime samples of block-32 input into the engine's plain-layout
+ * buffer on `stream`; *plain points at the result. */
+static int unblock_input(XGPUAmdEngine *e, const void *input_d, unsigned ntime, void *stream,
+                         const void **plain) {
+  const size_t bytes = (size_t)ntime * (size_t)e->tstride;
+  if (((uintptr_t)input_d) & 15)
+    return xg_fail(XGPU_INVALID_CONFIG, "block-32 input must be 16-byte aligned");
+  if (bytes > e->unblock_bytes) {
+    if (e->unblock) { /* queued kernels may still read the old buffer */
+      HIPCALL(xgh_device_sync(), "sync");
+      xgh_free(e->unblock);
+      e->unblock = NULL;
+      e->unblock_bytes = 0;
+    }
+    HIPCALL(xgh_malloc(&e->unblock, bytes), "block-32 conversion buffer");
+    e->unblock_bytes = bytes;
+  }
+  HIPCALL(xgh_launch_unblock32(e->unblock, input_d, bytes, stream), "block-32 conversion");
+  *plain = e->unblock;
+  return XGPU_OK;
+}
+
+/* Correlate plain-layout device input (splitting at the integration bound). */
+static int correlate_plain(XGPUAmdEngine *e, const void *input_d, unsigned int ntime) {
   const char *p = (const char *)input_d;
   while (ntime > 0) {
     unsigned room = e->bound - e->samples_in_integ;
@@ -347,6 +374,20 @@ int xgpuAmdCorrelate(XGPUAmdEngine *e, const void *input_d, unsigned int ntime) 
     ntime -= n;
   }
   return XGPU_OK;
+}
+
+int xgpuAmdCorrelate(XGPUAmdEngine *e, const void *input_d, unsigned int ntime) {
+  if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
+  if (!input_d && ntime) return xg_fail(XGPU_INVALID_CONFIG, "NULL input");
+  if (((uintptr_t)input_d) & 7) return xg_fail(XGPU_INVALID_CONFIG, "input must be 8-byte aligned");
+  HIPCALL(xgh_set_device(e->cfg.device), "set device");
+  if (e->block32 && ntime) {
+    const void *plain;
+    int rc = unblock_input(e, input_d, ntime, e->stream, &plain);
+    if (rc) return rc;
+    input_d = plain;
+  }
+  return correlate_plain(e, input_d, ntime);
 }
 
 static int ensure_scratch(XGPUAmdEngine *e, void **buf, size_t bytes) {
@@ -451,7 +492,10 @@ static int host_issue(XGPUAmdEngine *e, const void *input_h, char *dbase, int np
     HIPCALL(xgh_memcpy_h2d_async(dbase + p * chunk, hbase + p * chunk, chunk, e->copy_stream), "H2D");
     HIPCALL(xgh_event_record(e->ev_chunk[p], e->copy_stream), "record");
     HIPCALL(xgh_stream_wait_event(e->stream, e->ev_chunk[p]), "wait");
-    int rc = xgpuAmdCorrelate(e, dbase + p * chunk, ntp);
+    if (e->block32) /* the library's own buffer: convert in place */
+      HIPCALL(xgh_launch_unblock32(dbase + p * chunk, dbase + p * chunk, chunk, e->stream),
+              "block-32 conversion");
+    int rc = correlate_plain(e, dbase + p * chunk, ntp);
     if (rc) return rc;
   }
   return XGPU_OK;
@@ -597,14 +641,19 @@ int xgpuAmdCorrelateBatch(const XGPUAmdBatchItem *items, int nitems, void *strea
       HIPCALL(xgh_event_record(e->ev_aux, e->stream), "record");
       HIPCALL(xgh_stream_wait_event(s, e->ev_aux), "wait");
     }
-    job_fill(&L.jobs[i], e, items[i].input_d, items[i].ntime, !e->cleared, chan);
+    const void *in = items[i].input_d;
+    if (e->block32 && items[i].ntime) {
+      int rc = unblock_input(e, in, items[i].ntime, s, &in);
+      if (rc) return rc;
+    }
+    job_fill(&L.jobs[i], e, in, items[i].ntime, !e->cleared, chan);
     chan += (int)e->cfg.nfrequency;
   }
   L.njobs = nitems;
   /* the real-Gram kernel only when it takes every item */
-  int family = xg_launch_family(e0, items[0].input_d);
+  int family = xg_launch_family(e0, L.jobs[0].input);
   for (int i = 1; i < nitems; ++i)
-    if (xg_launch_family(items[i].engine, items[i].input_d) != family) family = e0->family;
+    if (xg_launch_family(items[i].engine, L.jobs[i].input) != family) family = e0->family;
   xg_plan_launch(&L, e0->cus, wg_per_cu_for(e0, family));
   L.prefetch = (char)e0->prefetch;
   L.stage = rg_stage_code(e0);

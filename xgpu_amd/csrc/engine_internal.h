@@ -46,6 +46,12 @@ struct XGPUAmdEngineStruct {
   void *dev_in[2];
   int parity;
 
+  /* COMPLEX_BLOCK_SIZE=32 input: converted to the plain layout before the
+   * kernels (in place in dev_in on the host path, else into `unblock`) */
+  int block32;
+  void *unblock;
+  size_t unblock_bytes;
+
   /* host-buffer path graphs, keyed by (input pointer, cleared state, buffer) */
   int use_graph;
   struct {

@@ -11,7 +11,10 @@
  *
  * The channel's samples are first gathered station-major so the inner loop
  * walks contiguous memory; the arithmetic and its order per visibility
- * (t ascending) are unchanged.
+ * (t ascending) are unchanged.  With COMPLEX_BLOCK_SIZE 32 the gather also
+ * decodes the blocked input (reference src/omp_xengine.c:64-78): in each
+ * block of 32 entries, byte q holds the real part of plain entry q and byte
+ * 32 + q its imaginary part.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -32,8 +35,16 @@ void xgpuOmpXengine(Complex *matrix_h, ComplexInput *array_h) {
     if (!ch) return;
 #pragma omp parallel for schedule(static)
     for (size_t m = 0; m < nin; ++m)
-      for (size_t t = 0; t < NTIME; ++t)
-        ch[m * NTIME + t] = array_h[(t * NFREQUENCY + f) * nin + m];
+      for (size_t t = 0; t < NTIME; ++t) {
+        const size_t i = (t * NFREQUENCY + f) * nin + m;
+#if COMPLEX_BLOCK_SIZE == 32
+        const signed char *blk = (const signed char *)array_h + 64 * (i / 32);
+        ch[m * NTIME + t].real = blk[i % 32];
+        ch[m * NTIME + t].imag = blk[32 + i % 32];
+#else
+        ch[m * NTIME + t] = array_h[i];
+#endif
+      }
 
 #pragma omp parallel for schedule(dynamic, 16)
     for (long b = 0; b < (long)NBASELINE; ++b) {

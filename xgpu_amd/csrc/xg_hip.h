@@ -88,6 +88,9 @@ int xgh_launch_mfma_peak(int grid, int iters, int *out, void *stream);
  * format: [t/4][f][s][p][c][t%4] */
 int xgh_launch_swizzle(void *dst, const void *src, int nstation, int nfreq,
                        int ntime, void *stream);
+/* COMPLEX_BLOCK_SIZE=32 input -> plain [t][f][s][p] input, in 64-byte blocks
+ * (bytes % 64 == 0, 16-byte aligned pointers; dst may equal src) */
+int xgh_launch_unblock32(void *dst, const void *src, size_t bytes, void *stream);
 
 #ifdef __cplusplus
 }

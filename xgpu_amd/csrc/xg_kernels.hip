@@ -1499,6 +1499,32 @@ __global__ void xg_swizzle_k(uint32_t *dst, const unsigned char *src, int nstati
   }
 }
 
+// COMPLEX_BLOCK_SIZE = 32 input -> plain [t][f][s][p][re,im] (reference
+// src/omp_xengine.c:64-78 reads the same layout).  Each 64-byte block holds
+// the real parts of 32 inputs (16 stations x 2 polarizations) in bytes 0..31
+// and their imaginary parts in bytes 32..63; the plain layout interleaves
+// them.  One thread per block reads all of it before writing, so dst may
+// equal src.
+__global__ void xg_unblock32_k(uint4 *dst, const uint4 *src, long long nblk) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < nblk; b += stride) {
+    const uint4 r0 = src[4 * b], r1 = src[4 * b + 1], i0 = src[4 * b + 2], i1 = src[4 * b + 3];
+    const unsigned re[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+    const unsigned im[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+    uint4 o[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      // bytes (re.b0, im.b0, re.b1, im.b1) and (re.b2, im.b2, re.b3, im.b3)
+      o[j].x = __builtin_amdgcn_perm(im[2 * j], re[2 * j], 0x05010400u);
+      o[j].y = __builtin_amdgcn_perm(im[2 * j], re[2 * j], 0x07030602u);
+      o[j].z = __builtin_amdgcn_perm(im[2 * j + 1], re[2 * j + 1], 0x05010400u);
+      o[j].w = __builtin_amdgcn_perm(im[2 * j + 1], re[2 * j + 1], 0x07030602u);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) dst[4 * b + j] = o[j];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -1656,6 +1682,16 @@ extern "C" int xgh_launch_swizzle(void *dst, const void *src, int nstation, int 
                      (hipStream_t)stream, (uint32_t *)dst, (const unsigned char *)src, nstation,
                      nfreq, ntime);
   return xgh_report(hipGetLastError(), "xg_swizzle launch");
+}
+
+extern "C" int xgh_launch_unblock32(void *dst, const void *src, size_t bytes, void *stream) {
+  if ((bytes % 64) || (((uintptr_t)dst | (uintptr_t)src) & 15))
+    return xgh_report(hipErrorInvalidValue, "xgh_launch_unblock32(alignment)");
+  const long long nblk = (long long)(bytes / 64);
+  if (nblk == 0) return 0;
+  hipLaunchKernelGGL(xg_unblock32_k, dim3(grid_for((size_t)nblk, 256)), dim3(256), 0,
+                     (hipStream_t)stream, (uint4 *)dst, (const uint4 *)src, nblk);
+  return xgh_report(hipGetLastError(), "xg_unblock32 launch");
 }
 
 // ---------------------------------------------------------------------------

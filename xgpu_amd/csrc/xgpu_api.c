@@ -49,7 +49,7 @@ static const XGPUInfo g_info = {
     .triLength = XG_TRI_LENGTH,
     .matrix_order = MATRIX_ORDER,
     .shared_atomic_size = 4,
-    .complex_block_size = 1,
+    .complex_block_size = COMPLEX_BLOCK_SIZE,
 };
 
 typedef struct {
@@ -161,6 +161,7 @@ int xgpuInit(XGPUContext *context, int device_flags) {
 #endif
   cfg.matrix_order = MATRIX_ORDER;
   cfg.device = in->device;
+  cfg.complex_block_size = COMPLEX_BLOCK_SIZE;
   int rc = xgpuAmdCreate(&in->engine, &cfg);
   if (rc) {
     report(rc, "xgpuInit");

@@ -45,6 +45,19 @@
 
 #define NBASELINE ((NSTATION + 1) * (NSTATION / 2))
 
+/* Input layout (reference src/omp_xengine.c:30-34, src/kernel.cuh:212-218):
+ * 1 = plain [t][f][s][p] ComplexInput, 32 = blocks of 32 entries holding the
+ * real parts of 16 stations' (X, Y) followed by their imaginary parts. */
+#ifndef COMPLEX_BLOCK_SIZE
+#define COMPLEX_BLOCK_SIZE 1
+#endif
+#if COMPLEX_BLOCK_SIZE != 1 && COMPLEX_BLOCK_SIZE != 32
+#error COMPLEX_BLOCK_SIZE must be 1 or 32
+#endif
+#if COMPLEX_BLOCK_SIZE == 32 && (NSTATION % 16) != 0
+#error COMPLEX_BLOCK_SIZE 32 needs NSTATION to be a multiple of 16
+#endif
+
 #if defined MATRIX_ORDER_TRIANGULAR
 #define MATRIX_ORDER TRIANGULAR_ORDER
 #elif defined MATRIX_ORDER_REAL_IMAG
