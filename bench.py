@@ -205,7 +205,7 @@ def profile_only(X, which, steps, warmup, kernel=0, stage=0, staging=0):
     else:
         buf.fill_uniform_int8(2, -128, 127)
     eng = X.Engine(ns, nf, compute=comp, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
-    if comp == X.XGPU_INT8:
+    if comp != X.XGPU_FLOAT32:
         eng.set_tuning(kernel=kernel, stage=stage, staging=staging)
     for _ in range(warmup + steps):
         eng.clear()
@@ -234,7 +234,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this X-engine build runs on one GPU; --gpus {args.gpus} is not "
-              "supported (channels shard across GPUs with no communication; see DESIGN.md)",
+              "supported (channels shard across GPUs with no communication)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -342,12 +342,14 @@ def main():
     b16 = X.DeviceBuffer(nt16 * nf16 * ns16 * 8)
     b16.fill_uniform_int8(4, 0, 59)  # finite, positive fp16 bit patterns (exponent <= 14); timing only
     e16 = X.Engine(ns16, nf16, compute=X.XGPU_FLOAT16, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
-    ms16 = e16.time_correlate(b16, nt16, warmup=1, reps=5)
+    ms16 = e16.time_correlate(b16, nt16, warmup=2, reps=10)  # default: real-Gram rgh<32>
+    e16.set_tuning(kernel=1)
+    ms16_cp = e16.time_correlate(b16, nt16, warmup=1, reps=10)  # complex-plane FP16 kernel
     e16.close()
     b16.free()
     cmac16 = X.cmac_per_integration(ns16, nf16, nt16)
     t16 = cmac16 / (ms16 * 1e-3) / 1e12
-    log(f"[bench] fp16 MFMA: {ms16:.3f} ms, {t16:.1f} TCMAC/s")
+    log(f"[bench] fp16 MFMA: {ms16:.3f} ms, {t16:.1f} TCMAC/s (complex-plane kernel {ms16_cp:.3f} ms)")
 
     # on-board MFMA-only ceiling and the clocks the run saw
     peak_meas = X.mfma_peak() / 1e12
@@ -400,7 +402,8 @@ def main():
                        "pct_fp32_peak": round(100 * t32 * 8 / PEAK_FP32_TFLOPS, 1)},
         "int8_over_fp32_plain": round(value / t32, 2),
         "fp16_mfma": {"workload": "NSTATION=256 x 2 pol, NFREQUENCY=128, NTIME=2048, fp16 input",
-                      "tcmac_s": round(t16, 2), "ms": round(ms16, 4),
+                      "kernel": "xg_xengine_rgh<32>", "tcmac_s": round(t16, 2),
+                      "ms": round(ms16, 4), "complex_plane_kernel_ms": round(ms16_cp, 4),
                       "pct_fp16_peak": round(100 * t16 * 8 / PEAK_FP16_TFLOPS, 1)},
         "roofline": {
             "bound": "mfma",
