@@ -35,6 +35,7 @@ def lib() -> C.CDLL:
             "oracle_regtile_from_tri": [I, P, P, I, I, LL],
             "oracle_swizzle": [P, P, I, I, I],
             "oracle_unblock32": [P, P, C.c_size_t],
+            "oracle_fft_f64": [P, P, I, I, I],
             "oracle_extract": [P, P, I, I],
             "oracle_random_legacy": [P, C.c_size_t, C.c_uint],
             "oracle_fill_uniform": [P, C.c_size_t, C.c_uint64, I, I],
@@ -149,6 +150,61 @@ def block32(x: np.ndarray) -> np.ndarray:
     """Inverse of unblock32: plain input -> COMPLEX_BLOCK_SIZE=32 layout."""
     x = np.ascontiguousarray(x, dtype=np.int8).reshape(-1, 32, 2)
     return np.ascontiguousarray(x.transpose(0, 2, 1)).reshape(-1)
+
+
+def fft_f64(x: np.ndarray, n: int, inverse: bool = False) -> np.ndarray:
+    """Double-precision reference FFT (radix-2 Stockham) of complex64 data
+    holding whole transforms of n points; returns complex128."""
+    x = np.ascontiguousarray(x, dtype=np.complex64).reshape(-1)
+    if x.size % n:
+        raise ValueError("whole transforms only")
+    out = np.empty(x.size, dtype=np.complex128)
+    lib().oracle_fft_f64(_p(out), _p(x), n, x.size // n, int(inverse))
+    return out
+
+
+def fft_relative_error_ulps(reference: np.ndarray, result: np.ndarray, n: int) -> float:
+    """Error measure of the reference FFT harness (src/fft/main.cpp:125-143):
+    per transform sum|ref - res|^2 / sum|ref|^2, the maximum over the batch,
+    square-rooted, in units of 2^-23."""
+    ref = np.asarray(reference, dtype=np.complex128).reshape(-1, n)
+    res = np.asarray(result, dtype=np.complex128).reshape(-1, n)
+    err = (np.abs(ref - res) ** 2).sum(axis=1) / (np.abs(ref) ** 2).sum(axis=1)
+    return float(np.sqrt(err.max()) / 1.192092896e-07)
+
+
+def fft_harness_transpose(src: np.ndarray, sz: int, sy: int, sx: int) -> np.ndarray:
+    """The reference FFT harness's record transpose (src/fft/main.cpp:145-165):
+    per record of sx*sy*sz elements, out[sx*(z + y*sz) + x] = in[sx*(z*sy + y) + x]."""
+    src = np.asarray(src).reshape(-1, sz, sy, sx)
+    return np.ascontiguousarray(src.transpose(0, 2, 1, 3)).reshape(-1)
+
+
+def fft_to_prototype(x: np.ndarray, n: int, spectrum: bool = False) -> np.ndarray:
+    """Natural-order transforms -> the layout the reference FFT entry points
+    read or write, as its harness arranges it (src/fft/main.cpp:249-264):
+    n = 8, 16, 64 interleaved within groups; n >= 2048 spectra transposed
+    (spectrum=True: the forward output / inverse input)."""
+    x = np.asarray(x).reshape(-1)
+    if n == 16:
+        return fft_harness_transpose(x, 64, 16, 1)
+    if n < 256:
+        return fft_harness_transpose(x, 512 // n, 8, n // 8)
+    if n >= 2048 and spectrum:
+        return fft_harness_transpose(x, 512, n // 512, 1)
+    return x.copy()
+
+
+def fft_from_prototype(x: np.ndarray, n: int, spectrum: bool = False) -> np.ndarray:
+    """Inverse of fft_to_prototype (the harness's transposes back)."""
+    x = np.asarray(x).reshape(-1)
+    if n == 16:
+        return fft_harness_transpose(x, 16, 64, 1)
+    if n < 256:
+        return fft_harness_transpose(x, 8, 512 // n, n // 8)
+    if n >= 2048 and spectrum:
+        return fft_harness_transpose(x, n // 512, 512, 1)
+    return x.copy()
 
 
 def extract(tri: np.ndarray, ns: int, nf: int) -> np.ndarray:

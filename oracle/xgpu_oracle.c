@@ -359,3 +359,41 @@ int oracle_check(const float *gpu, const float *cpu, int ns, int nf, double tol,
   if (max_err) *max_err = worst;
   return bad;
 }
+
+void oracle_fft_f64(double *out, const float *in, int n, int batch, int inverse) {
+  double *X = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+  double *Y = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+  if (!X || !Y) {
+    free(X);
+    free(Y);
+    return;
+  }
+  const double sign = inverse ? 1.0 : -1.0;
+  for (int b = 0; b < batch; ++b) {
+    const float *src = in + (size_t)b * 2 * n;
+    for (int i = 0; i < 2 * n; ++i) X[i] = src[i];
+    /* pass with span L: Y[2jL + k] = X[jL + k] + w X[jL + n/2 + k],
+     * Y[2jL + L + k] = X[jL + k] - w X[jL + n/2 + k], w = exp(sign i pi k / L) */
+    for (int L = 1; L < n; L *= 2) {
+      for (int k = 0; k < L; ++k) {
+        const double phi = sign * M_PI * k / L;
+        const double wr = cos(phi), wi = sin(phi);
+        for (int j = 0; j < n / (2 * L); ++j) {
+          const double *a = X + 2 * (j * L + k), *c = X + 2 * (j * L + n / 2 + k);
+          const double tr = wr * c[0] - wi * c[1], ti = wr * c[1] + wi * c[0];
+          double *y0 = Y + 2 * (2 * j * L + k), *y1 = Y + 2 * (2 * j * L + L + k);
+          y0[0] = a[0] + tr;
+          y0[1] = a[1] + ti;
+          y1[0] = a[0] - tr;
+          y1[1] = a[1] - ti;
+        }
+      }
+      double *T = X;
+      X = Y;
+      Y = T;
+    }
+    memcpy(out + (size_t)b * 2 * n, X, sizeof(double) * 2 * (size_t)n);
+  }
+  free(X);
+  free(Y);
+}

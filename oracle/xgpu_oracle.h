@@ -30,6 +30,9 @@
  *   oracle_fill_uniform       numpy restatement of the same integer formula
  *   oracle_check              golden text/values of the reference
  *                             xgpuCheckResult on planted errors
+ *   oracle_fft_f64            numpy.fft (a library FFT) on random data,
+ *                             closed forms (impulse, constant, single tone),
+ *                             Parseval, forward/inverse round trip
  */
 #ifndef XGPU_ORACLE_H
 #define XGPU_ORACLE_H
@@ -98,5 +101,13 @@ void oracle_fill_uniform(int8_t *out, size_t n, uint64_t seed, int lo, int hi);
  * in-place rounding the reference applies to non-DP4A fixed-point results. */
 int oracle_check(const float *gpu, const float *cpu, int ns, int nf, double tol, int round_gpu,
                  double *max_err);
+
+/* Double-precision reference FFT of the reference FFT harness: radix-2
+ * Stockham from float input, result in double (reference
+ * src/fft/main.cpp:80-117; the harness rounds it to float).  Forward uses
+ * exp(-2 pi i jk/n), inverse exp(+2 pi i jk/n), unnormalized.  in/out:
+ * batch transforms of n interleaved (re, im) values, natural order; n a
+ * power of two. */
+void oracle_fft_f64(double *out, const float *in, int n, int batch, int inverse);
 
 #endif /* XGPU_ORACLE_H */

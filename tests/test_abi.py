@@ -51,14 +51,32 @@ def test_drop_in_library_exports_every_xgpu_h_symbol(variant):
     X.variant(variant)  # loads, and its dependency on libxgpuamd.so resolves
 
 
+# library that exports each extension header's functions
+HEADER_LIBS = {"xgpu_amd.h": "libxgpuamd.so", "xgpu_serve.h": "libxgpuamd.so",
+               "xgpu_fft.h": "libxgpufft.so"}
+
+
 def test_core_library_exports_every_extension_symbol():
-    have = exported(_lib.lib_path("libxgpuamd.so"))
-    for header in sorted(os.listdir(INCLUDE)):
-        if header == "xgpu.h":
-            continue
+    headers = sorted(h for h in os.listdir(INCLUDE) if h != "xgpu.h")
+    assert headers == sorted(HEADER_LIBS), headers
+    for header in headers:
+        have = exported(_lib.lib_path(HEADER_LIBS[header]))
         missing = [n for n in declared_functions(header) if n not in have]
         assert not missing, (header, missing)
     X.core()
+    _lib.fft_lib()
+
+
+def test_fft_library_declares_the_reference_entry_points():
+    """The FFT prototype's entry points (src/fft/main.cpp:37-55)."""
+    assert declared_functions("xgpu_fft.h") == ["xgpuAmdFft"]
+    sizes = (8, 16, 64, 256, 512, 1024, 2048, 4096, 8192)
+    text = open(os.path.join(INCLUDE, "xgpu_fft.h")).read()
+    for n in sizes:
+        assert f"void FFT{n}(void *work, int batch);" in text
+        assert f"void IFFT{n}(void *work, int batch);" in text
+    have = exported(_lib.lib_path("libxgpufft.so"))
+    assert all(f"FFT{n}" in have and f"IFFT{n}" in have for n in sizes)
 
 
 def test_struct_layouts_match_reference_header():

@@ -223,6 +223,20 @@ def correlate_batch(items, stream=None) -> None:
     check(core().xgpuAmdCorrelateBatch(arr, len(items), stream))
 
 
+FFT_NATURAL, FFT_PROTOTYPE = 0, 1
+FFT_SIZES = (8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192)
+
+
+def fft(work, n: int, batch: int, inverse: bool = False, layout: int = FFT_NATURAL,
+        stream=None) -> None:
+    """In-place batched complex64 FFT of `batch` transforms of n points on the
+    device (include/xgpu_fft.h); inverse is unnormalized."""
+    ptr = work.ptr if isinstance(work, DeviceBuffer) else int(work)
+    rc = _lib.fft_lib().xgpuAmdFft(ptr, n, batch, int(inverse), layout, stream)
+    if rc:
+        raise XgpuError(rc, f"xgpuAmdFft(n={n}, batch={batch}, layout={layout}) returned {rc}")
+
+
 class HostBuffer:
     """Pinned host memory viewed as a numpy array (fast asynchronous copies)."""
 

@@ -167,6 +167,23 @@ def core() -> C.CDLL:
     return _core
 
 
+_fft = None
+
+
+def fft_lib() -> C.CDLL:
+    """The batched FFT library (include/xgpu_fft.h)."""
+    global _fft
+    if _fft is None:
+        lib = _load("libxgpufft.so")
+        P, I = C.c_void_p, C.c_int
+        _sig(lib, "xgpuAmdFft", I, P, I, C.c_longlong, I, I, P)
+        for n in (8, 16, 64, 256, 512, 1024, 2048, 4096, 8192):
+            _sig(lib, f"FFT{n}", None, P, I)
+            _sig(lib, f"IFFT{n}", None, P, I)
+        _fft = lib
+    return _fft
+
+
 def variant(name: str = "libxgpu") -> C.CDLL:
     """A build-time-sized xgpu.h library, e.g. ``libxgpu`` or ``libxgpu_ns512_dp4a``."""
     if name not in _variants:
