@@ -186,6 +186,17 @@ def kernel_name(tun):
 def profile_only(X, which, steps, warmup, kernel=0, stage=0, staging=0):
     """Run one path's workload warmup+steps times (one kernel per step).
     "copy": a plain 1 GiB device-to-device copy per step (counter calibration)."""
+    if which == "fft":
+        import numpy as np
+        buf = X.DeviceBuffer(FFT_ENTRIES * 8)
+        buf.upload(np.zeros(FFT_ENTRIES, np.complex64))
+        for n in X.FFT_SIZES:
+            for _ in range(warmup + steps):
+                X.fft(buf, n, FFT_ENTRIES // n)
+        X.core().xgpuAmdDeviceSynchronize()
+        log(f"[bench] fft: {warmup + steps} transforms of {FFT_ENTRIES} values per size done")
+        buf.free()
+        return
     if which == "copy":
         a, b = X.DeviceBuffer(1 << 30), X.DeviceBuffer(1 << 30)
         a.fill_uniform_int8(1, -128, 127)
@@ -216,6 +227,56 @@ def profile_only(X, which, steps, warmup, kernel=0, stage=0, staging=0):
     buf.free()
 
 
+FFT_ENTRIES = 8 * 1024 * 1024  # complex values per FFT workload (the reference harness's n_entries)
+
+
+def fft_bench(X, reps=20):
+    """Per size: time in place over FFT_ENTRIES complex64 values (forward,
+    natural layout, and the prototype layout of the reference entry points),
+    the harness's Gflop/s (5 N log2 N per transform) and GB/s (one read and
+    one write of every value), and the harness's error measure against
+    numpy's double-precision FFT on 64K values."""
+    import ctypes as C
+    import math
+
+    import numpy as np
+    lib = X._lib.fft_lib()
+    buf = X.DeviceBuffer(FFT_ENTRIES * 8)
+    buf.upload(np.zeros(FFT_ENTRIES, np.complex64))  # finite values; timing only
+    g = np.random.default_rng(1)
+    small = (g.uniform(-1, 1, 1 << 16) + 1j * g.uniform(-1, 1, 1 << 16)).astype(np.complex64)
+    sbuf = X.DeviceBuffer(small.nbytes)
+    out = {"workload": f"{FFT_ENTRIES} complex64 values in place per size, HBM-resident",
+           "sizes": {}}
+    for n in X.FFT_SIZES:
+        batch = FFT_ENTRIES // n
+        res = {}
+        for name, layout in (("natural", X.FFT_NATURAL), ("prototype", X.FFT_PROTOTYPE)):
+            if layout == X.FFT_PROTOTYPE and n in (32, 128):
+                continue
+            ms = C.c_float()
+            rc = lib.xgpuAmdFftTime(buf.ptr, n, batch, 0, layout, reps, C.byref(ms))
+            if rc:
+                res[name] = {"error": rc}
+                continue
+            t = ms.value * 1e-3
+            gb_s = 16.0 * FFT_ENTRIES / t / 1e9
+            res[name] = {"us": round(ms.value * 1e3, 2),
+                         "gflop_s": round(5 * n * math.log2(n) * batch / t / 1e9, 1),
+                         "gb_s": round(gb_s, 1), "pct_hbm_8tb_s": round(100 * gb_s / 8000, 1)}
+        sbuf.upload(small)
+        X.fft(sbuf, n, small.size // n)
+        got = sbuf.download(np.complex64).astype(np.complex128).reshape(-1, n)
+        ref = np.fft.fft(small.astype(np.complex128).reshape(-1, n), axis=1)
+        err = (np.abs(got - ref) ** 2).sum(1) / (np.abs(ref) ** 2).sum(1)
+        res["error_ulps"] = round(float(np.sqrt(err.max()) / 1.192092896e-07), 3)
+        out["sizes"][str(n)] = res
+        log(f"[bench] fft n={n}: {res}")
+    buf.free()
+    sbuf.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,7 +284,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--only", choices=["int8", "fp32", "fp16", "copy"], default=None,
+    ap.add_argument("--only", choices=["int8", "fp32", "fp16", "copy", "fft"], default=None,
                     help="run just this path's workload (for profiling); prints no JSON line")
     ap.add_argument("--kernel", type=int, default=0, choices=[0, 1, 2],
                     help="int8 kernel: 1 complex-plane, 2 real-Gram (0 = default)")
@@ -351,6 +412,10 @@ def main():
     t16 = cmac16 / (ms16 * 1e-3) / 1e12
     log(f"[bench] fp16 MFMA: {ms16:.3f} ms, {t16:.1f} TCMAC/s (complex-plane kernel {ms16_cp:.3f} ms)")
 
+    # batched FFTs (F-engine step), the reference FFT harness's workload:
+    # 8M complex64 entries per size, in place
+    fft = fft_bench(X)
+
     # on-board MFMA-only ceiling and the clocks the run saw
     peak_meas = X.mfma_peak() / 1e12
     log(f"[bench] MFMA-only int8 probe: {peak_meas:.1f} TCMAC/s")
@@ -401,6 +466,7 @@ def main():
         "fp32_plain": {"tcmac_s": round(t32, 3), "ms": round(ms32, 3),
                        "pct_fp32_peak": round(100 * t32 * 8 / PEAK_FP32_TFLOPS, 1)},
         "int8_over_fp32_plain": round(value / t32, 2),
+        "fft": fft,
         "fp16_mfma": {"workload": "NSTATION=256 x 2 pol, NFREQUENCY=128, NTIME=2048, fp16 input",
                       "kernel": "xg_xengine_rgh<32>", "tcmac_s": round(t16, 2),
                       "ms": round(ms16, 4), "complex_plane_kernel_ms": round(ms16_cp, 4),

@@ -33,6 +33,13 @@ extern "C" {
  * stream-aware, error-returning form of the entry points below. */
 int xgpuAmdFft(void *work, int n, long long batch, int inverse, int layout, void *stream);
 
+/* Measurement: average milliseconds of `reps` xgpuAmdFft calls on the
+ * default stream (HIP events), after one warm-up call; work is transformed
+ * reps + 1 times.  No counterpart in the reference (its harness times with
+ * CUDA events, src/fft/main.cpp:14-33). */
+int xgpuAmdFftTime(void *work, int n, long long batch, int inverse, int layout, int reps,
+                   float *ms);
+
 /* The reference entry points (src/fft/main.cpp:37-55; kernels
  * src/fft/FFT<N>.cu, src/fft/IFFT<N>.cu): in place, default stream, the
  * prototype layouts the reference harness transposes to and from

@@ -69,7 +69,7 @@ def test_core_library_exports_every_extension_symbol():
 
 def test_fft_library_declares_the_reference_entry_points():
     """The FFT prototype's entry points (src/fft/main.cpp:37-55)."""
-    assert declared_functions("xgpu_fft.h") == ["xgpuAmdFft"]
+    assert declared_functions("xgpu_fft.h") == ["xgpuAmdFft", "xgpuAmdFftTime"]
     sizes = (8, 16, 64, 256, 512, 1024, 2048, 4096, 8192)
     text = open(os.path.join(INCLUDE, "xgpu_fft.h")).read()
     for n in sizes:

@@ -177,6 +177,7 @@ def fft_lib() -> C.CDLL:
         lib = _load("libxgpufft.so")
         P, I = C.c_void_p, C.c_int
         _sig(lib, "xgpuAmdFft", I, P, I, C.c_longlong, I, I, P)
+        _sig(lib, "xgpuAmdFftTime", I, P, I, C.c_longlong, I, I, I, C.POINTER(C.c_float))
         for n in (8, 16, 64, 256, 512, 1024, 2048, 4096, 8192):
             _sig(lib, f"FFT{n}", None, P, I)
             _sig(lib, f"IFFT{n}", None, P, I)

@@ -6,9 +6,10 @@
 // complex-to-complex transforms of N = 8 .. 8192 points, forward
 // (exp(-2 pi i jk/N)) and unnormalized inverse.
 //
-// Design: one pass over HBM.  A workgroup of 256 threads loads a chunk of
-// max(N, 2048) elements (2048/N whole transforms, or one transform) into LDS
-// (16, 32 or 64 KiB), runs every radix pass there and stores the chunk once,
+// Design: one pass over HBM.  A workgroup (256 threads; 512 for N = 8192)
+// loads a chunk of max(N, 2048) elements (2048/N whole transforms, or one
+// transform) into LDS (16, 32 or 64 KiB), runs every radix pass there and
+// stores the chunk once,
 // so each element is read and written exactly once whatever N is (the
 // reference needs two kernels, hence two HBM round trips, for N >= 2048).
 // The passes are self-sorting Stockham radix-8/4/2 steps,
@@ -96,8 +97,8 @@ __device__ __forceinline__ void dft(float2 *a) {
 }
 
 // Radix of pass p for size N (0 past the last pass).  The product of a
-// plan's radices is N, and every pass has at least 256 butterflies per
-// workgroup.
+// plan's radices is N, and every pass has at least one butterfly per thread
+// of the workgroup.
 __host__ __device__ constexpr int fft_radix(int n, int p) {
   switch (n) {
     case 8: return p == 0 ? 8 : 0;
@@ -117,9 +118,10 @@ __host__ __device__ constexpr int fft_radix(int n, int p) {
 
 template <int N>
 struct FftCfg {
-  static constexpr int kT = N >= 2048 ? 1 : 2048 / N;  // transforms per workgroup
-  static constexpr int kC = kT * N;                     // elements per workgroup
-  static constexpr int kE = kC / 256;                   // elements per thread
+  static constexpr int kT = N >= 2048 ? 1 : 2048 / N;     // transforms per workgroup
+  static constexpr int kC = kT * N;                        // elements per workgroup
+  static constexpr int kThreads = N == 8192 ? 512 : 256;  // more waves where LDS limits
+  static constexpr int kE = kC / kThreads;                 // elements per thread
 };
 
 // One Stockham pass of radix R at span L over the workgroup's transforms in
@@ -128,20 +130,36 @@ struct FftCfg {
 template <int N, int R, int L>
 __device__ __forceinline__ void fft_pass(float2 *X, int tid) {
   typedef FftCfg<N> C;
-  constexpr int kB = C::kC / R / 256;  // butterflies per thread
-  static_assert(kB >= 1 && kB * 256 * R == C::kC, "pass shape");
+  constexpr int kB = C::kC / R / C::kThreads;  // butterflies per thread
+  static_assert(kB >= 1 && kB * C::kThreads * R == C::kC, "pass shape");
   float2 v[kB][R];
   int dst[kB];
 #pragma unroll
   for (int p = 0; p < kB; ++p) {
-    const int b = tid + 256 * p;
+    const int b = tid + C::kThreads * p;
     const int t = b / (N / R), rem = b % (N / R);
     const int k = rem % L, j = rem / L;
     const float2 *src = X + t * N + j * L + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      v[p][r] = src[r * (N / R)];
-      if (L > 1 && r > 0) v[p][r] = cmul(v[p][r], xg_fft_tw[(r * k) * (XG_FFT_TWN / (R * L))]);
+    for (int r = 0; r < R; ++r) v[p][r] = src[r * (N / R)];
+    if (L > 1) {
+      // w^{rk}, w = exp(-2 pi i / RL): w^k, w^2k, w^4k from the table (m < 8192 / R,
+      // so 4m < 8192), the other powers as products of those (at most two roundings)
+      const int m = k * (XG_FFT_TWN / (R * L));
+      float2 w[R];
+      w[1] = xg_fft_tw[m];
+      if constexpr (R >= 4) {
+        w[2] = xg_fft_tw[2 * m];
+        w[3] = cmul(w[1], w[2]);
+      }
+      if constexpr (R == 8) {
+        w[4] = xg_fft_tw[4 * m];
+        w[5] = cmul(w[1], w[4]);
+        w[6] = cmul(w[2], w[4]);
+        w[7] = cmul(w[3], w[4]);
+      }
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[p][r] = cmul(v[p][r], w[r]);
     }
     dst[p] = t * N + j * R * L + k;
   }
@@ -184,14 +202,14 @@ __device__ __forceinline__ int fft_lds_index(int i) {
 }
 
 template <int N, bool INV, int LAYOUT>
-__global__ __launch_bounds__(256) void xg_fft_k(float2 *work, long long nelem) {
+__global__ __launch_bounds__(FftCfg<N>::kThreads) void xg_fft_k(float2 *work, long long nelem) {
   typedef FftCfg<N> C;
   __shared__ float2 X[C::kC];
   const int tid = threadIdx.x;
   const long long base = (long long)blockIdx.x * C::kC;
 #pragma unroll
   for (int e = 0; e < C::kE; ++e) {
-    const int i = tid + 256 * e;
+    const int i = tid + C::kThreads * e;
     float2 v = base + i < nelem ? work[base + i] : make_float2(0.f, 0.f);
     if (INV) v.y = -v.y;
     X[fft_lds_index<N, INV, LAYOUT, true>(i)] = v;
@@ -200,7 +218,7 @@ __global__ __launch_bounds__(256) void xg_fft_k(float2 *work, long long nelem) {
   fft_passes<N, 0, 1>(X, tid);
 #pragma unroll
   for (int e = 0; e < C::kE; ++e) {
-    const int i = tid + 256 * e;
+    const int i = tid + C::kThreads * e;
     if (base + i < nelem) {
       float2 v = X[fft_lds_index<N, INV, LAYOUT, false>(i)];
       if (INV) v.y = -v.y;
@@ -239,8 +257,8 @@ static int fft_launch(void *work, long long batch, hipStream_t s) {
   const long long grid = (nelem + FftCfg<N>::kC - 1) / FftCfg<N>::kC;
   if (grid == 0) return XGPU_FFT_OK;
   if (grid > 0x7fffffffLL) return fft_fail(hipErrorInvalidValue, "batch too large");
-  hipLaunchKernelGGL((xg_fft_k<N, INV, LAYOUT>), dim3((unsigned)grid), dim3(256), 0, s,
-                     (float2 *)work, nelem);
+  hipLaunchKernelGGL((xg_fft_k<N, INV, LAYOUT>), dim3((unsigned)grid), dim3(FftCfg<N>::kThreads),
+                     0, s, (float2 *)work, nelem);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? XGPU_FFT_OK : fft_fail(e, "fft launch");
 }
@@ -283,6 +301,32 @@ extern "C" int xgpuAmdFft(void *work, int n, long long batch, int inverse, int l
                    : fft_dispatch<false, XGPU_FFT_NATURAL>(work, n, batch, s);
   return inverse ? fft_dispatch<true, XGPU_FFT_PROTOTYPE>(work, n, batch, s)
                  : fft_dispatch<false, XGPU_FFT_PROTOTYPE>(work, n, batch, s);
+}
+
+extern "C" int xgpuAmdFftTime(void *work, int n, long long batch, int inverse, int layout,
+                              int reps, float *ms) {
+  if (!ms || reps < 1) return XGPU_FFT_BAD_ARGUMENT;
+  int rc = xgpuAmdFft(work, n, batch, inverse, layout, 0);  // warm-up, and argument checks
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return fft_fail(hipErrorOutOfMemory, "event");
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fft_fail(hipErrorOutOfMemory, "event");
+  }
+  hipError_t e = hipEventRecord(e0, 0);
+  for (int i = 0; i < reps && e == hipSuccess && rc == XGPU_FFT_OK; ++i)
+    rc = xgpuAmdFft(work, n, batch, inverse, layout, 0);
+  if (e == hipSuccess) e = hipEventRecord(e1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float total = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&total, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (e != hipSuccess) return fft_fail(e, "timing");
+  *ms = total / (float)reps;
+  return XGPU_FFT_OK;
 }
 
 // Reference entry points (src/fft/main.cpp:37-55): in place on the default
