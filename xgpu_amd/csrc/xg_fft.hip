@@ -124,11 +124,52 @@ struct FftCfg {
   static constexpr int kE = kC / kThreads;                 // elements per thread
 };
 
-// One Stockham pass of radix R at span L over the workgroup's transforms in
-// LDS: every thread reads its butterflies' inputs, the workgroup syncs, then
-// every thread writes its outputs (in place).
-template <int N, int R, int L>
-__device__ __forceinline__ void fft_pass(float2 *X, int tid) {
+// Chunk offset, in layout LAYOUT, of the workgroup's natural-order element q
+// (transform q / N, element or bin q % N) on the input side (IN = true) or
+// the output side of a transform.
+template <int N, bool INV, int LAYOUT, bool IN>
+__device__ __forceinline__ int fft_gpos(int q) {
+  if (LAYOUT == XGPU_FFT_NATURAL) return q;
+  if (N == 8 || N == 16) {  // groups of 64 transforms: element y of z at z + 64 y
+    const int S = 64 * N, grp = q / S, rem = q % S;
+    return grp * S + rem / N + 64 * (rem % N);
+  }
+  if (N == 64) {  // groups of 8 transforms: element 8a + x of t at 64 a + 8 t + x
+    const int grp = q / 512, rem = q % 512, e = rem % 64;
+    return grp * 512 + 64 * (e / 8) + 8 * (rem / 64) + e % 8;
+  }
+  if (N >= 2048 && (IN == INV)) {  // spectrum side: bin y (N/512) + z at 512 z + y
+    return 512 * (q % (N / 512)) + q / (N / 512);
+  }
+  return q;
+}
+
+// Inverse of fft_gpos: natural-order element held at chunk offset i.
+template <int N, bool INV, int LAYOUT, bool IN>
+__device__ __forceinline__ int fft_natural(int i) {
+  if (LAYOUT == XGPU_FFT_NATURAL) return i;
+  if (N == 8 || N == 16) {
+    const int S = 64 * N, grp = i / S, w = i % S;
+    return (grp * 64 + w % 64) * N + w / 64;
+  }
+  if (N == 64) {
+    const int grp = i / 512, w = i % 512;
+    return (grp * 8 + (w / 8) % 8) * 64 + 8 * (w / 64) + w % 8;
+  }
+  if (N >= 2048 && (IN == INV)) return (i % 512) * (N / 512) + i / 512;
+  return i;
+}
+
+// One Stockham pass of radix R at span L over the workgroup's transforms:
+// every thread reads its butterflies' inputs, the workgroup syncs, then every
+// thread writes its outputs (in place in LDS).  Where a thread's inputs are at
+// least 8 elements apart (N/R >= 8), the first pass reads straight from HBM
+// (FIRST), and where its outputs are (L >= 8), the last pass writes straight
+// to HBM (LAST): consecutive butterflies then touch runs of consecutive
+// elements.  Other sizes stage through LDS with coalesced loops.
+template <int N, int R, int L, bool FIRST, bool LAST, bool INV, int LAYOUT>
+__device__ __forceinline__ void fft_pass(float2 *X, float2 *work, long long base, long long nelem,
+                                         int tid) {
   typedef FftCfg<N> C;
   constexpr int kB = C::kC / R / C::kThreads;  // butterflies per thread
   static_assert(kB >= 1 && kB * C::kThreads * R == C::kC, "pass shape");
@@ -139,9 +180,19 @@ __device__ __forceinline__ void fft_pass(float2 *X, int tid) {
     const int b = tid + C::kThreads * p;
     const int t = b / (N / R), rem = b % (N / R);
     const int k = rem % L, j = rem / L;
-    const float2 *src = X + t * N + j * L + k;
+    const int q0 = t * N + j * L + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[p][r] = src[r * (N / R)];
+    for (int r = 0; r < R; ++r) {
+      const int q = q0 + r * (N / R);
+      if constexpr (FIRST) {
+        const long long g = base + fft_gpos<N, INV, LAYOUT, true>(q);
+        float2 x = g < nelem ? work[g] : make_float2(0.f, 0.f);
+        if (INV) x.y = -x.y;  // inverse = conj(FFT(conj(x)))
+        v[p][r] = x;
+      } else {
+        v[p][r] = X[q];
+      }
+    }
     if (L > 1) {
       // w^{rk}, w = exp(-2 pi i / RL): w^k, w^2k, w^4k from the table (m < 8192 / R,
       // so 4m < 8192), the other powers as products of those (at most two roundings)
@@ -163,66 +214,73 @@ __device__ __forceinline__ void fft_pass(float2 *X, int tid) {
     }
     dst[p] = t * N + j * R * L + k;
   }
-  __syncthreads();
+  if constexpr (!FIRST && !LAST) __syncthreads();  // every read of X before any write
 #pragma unroll
   for (int p = 0; p < kB; ++p) {
     dft<R>(v[p]);
 #pragma unroll
-    for (int s = 0; s < R; ++s) X[dst[p] + s * L] = v[p][s];
+    for (int s = 0; s < R; ++s) {
+      const int q = dst[p] + s * L;
+      if constexpr (LAST) {
+        const long long g = base + fft_gpos<N, INV, LAYOUT, false>(q);
+        if (g < nelem) {
+          float2 y = v[p][s];
+          if (INV) y.y = -y.y;
+          work[g] = y;
+        }
+      } else {
+        X[q] = v[p][s];
+      }
+    }
   }
-  __syncthreads();
+  if constexpr (!LAST) __syncthreads();  // this pass's writes before the next pass's reads
 }
 
-template <int N, int P, int L>
-__device__ __forceinline__ void fft_passes(float2 *X, int tid) {
+// passes of the plan from pass P (span L) on
+template <int N, int P, int L, bool INV, int LAYOUT>
+__device__ __forceinline__ void fft_passes(float2 *X, float2 *work, long long base, long long nelem,
+                                           int tid) {
   constexpr int R = fft_radix(N, P);
   if constexpr (R > 0) {
-    fft_pass<N, R, L>(X, tid);
-    fft_passes<N, P + 1, L * R>(X, tid);
+    constexpr bool FIRST = P == 0 && N / R >= 8;
+    constexpr bool LAST = fft_radix(N, P + 1) == 0 && L >= 8;
+    fft_pass<N, R, L, FIRST, LAST, INV, LAYOUT>(X, work, base, nelem, tid);
+    fft_passes<N, P + 1, L * R, INV, LAYOUT>(X, work, base, nelem, tid);
   }
-}
-
-// LDS (natural) index of the element at chunk offset i, for the load
-// (IN = true) or the store of a transform in layout LAYOUT.
-template <int N, bool INV, int LAYOUT, bool IN>
-__device__ __forceinline__ int fft_lds_index(int i) {
-  if (LAYOUT == XGPU_FFT_NATURAL) return i;
-  if (N == 8 || N == 16) {  // groups of 64 transforms: element y of z at z + 64 y
-    const int S = 64 * N, grp = i / S, w = i % S;
-    return (grp * 64 + w % 64) * N + w / 64;
-  }
-  if (N == 64) {  // groups of 8 transforms: element 8a + x of t at 64 a + 8 t + x
-    const int grp = i / 512, w = i % 512;
-    return (grp * 8 + (w / 8) % 8) * 64 + 8 * (w / 64) + w % 8;
-  }
-  if (N >= 2048 && (IN == INV)) {  // spectrum side: bin y (N/512) + z at 512 z + y
-    return (i % 512) * (N / 512) + i / 512;
-  }
-  return i;
 }
 
 template <int N, bool INV, int LAYOUT>
 __global__ __launch_bounds__(FftCfg<N>::kThreads) void xg_fft_k(float2 *work, long long nelem) {
   typedef FftCfg<N> C;
+  constexpr int R0 = fft_radix(N, 0);
+  constexpr int kPasses = fft_radix(N, 1) == 0 ? 1 : fft_radix(N, 2) == 0 ? 2
+                        : fft_radix(N, 3) == 0 ? 3 : fft_radix(N, 4) == 0 ? 4 : 5;
+  constexpr int RL = fft_radix(N, kPasses - 1);          // radix of the last pass
+  constexpr bool kStageIn = N / R0 < 8;                   // first pass reads LDS
+  constexpr bool kStageOut = N / RL < 8;                  // last pass writes LDS (span N/RL)
   __shared__ float2 X[C::kC];
   const int tid = threadIdx.x;
   const long long base = (long long)blockIdx.x * C::kC;
+  if constexpr (kStageIn) {  // coalesced HBM -> LDS in natural order
 #pragma unroll
-  for (int e = 0; e < C::kE; ++e) {
-    const int i = tid + C::kThreads * e;
-    float2 v = base + i < nelem ? work[base + i] : make_float2(0.f, 0.f);
-    if (INV) v.y = -v.y;
-    X[fft_lds_index<N, INV, LAYOUT, true>(i)] = v;
-  }
-  __syncthreads();
-  fft_passes<N, 0, 1>(X, tid);
-#pragma unroll
-  for (int e = 0; e < C::kE; ++e) {
-    const int i = tid + C::kThreads * e;
-    if (base + i < nelem) {
-      float2 v = X[fft_lds_index<N, INV, LAYOUT, false>(i)];
+    for (int e = 0; e < C::kE; ++e) {
+      const int i = tid + C::kThreads * e;
+      float2 v = base + i < nelem ? work[base + i] : make_float2(0.f, 0.f);
       if (INV) v.y = -v.y;
-      work[base + i] = v;
+      X[fft_natural<N, INV, LAYOUT, true>(i)] = v;
+    }
+    __syncthreads();
+  }
+  fft_passes<N, 0, 1, INV, LAYOUT>(X, work, base, nelem, tid);
+  if constexpr (kStageOut) {  // coalesced LDS -> HBM
+#pragma unroll
+    for (int e = 0; e < C::kE; ++e) {
+      const int i = tid + C::kThreads * e;
+      if (base + i < nelem) {
+        float2 v = X[fft_natural<N, INV, LAYOUT, false>(i)];
+        if (INV) v.y = -v.y;
+        work[base + i] = v;
+      }
     }
   }
 }
