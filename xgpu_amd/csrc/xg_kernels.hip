@@ -696,7 +696,83 @@ __device__ __forceinline__ void store_vis(const XgJob &J, int f, int m, int n, f
   store_flt(J, f, m, n, re, im);
 }
 
-// 32-bit (int32 or float) stores of the three output orders with the index
+// REGISTER_TILE 32-bit stores through LDS.  A register-tile cell (stations
+// i = 2R + rx, j = 2C + ry) holds its four polarization products (element
+// 2 p1 + p2) in 16 consecutive bytes, and the four cells of a lane quad's
+// column block b = 0..3 are consecutive in each quadrant plane (rx, ry).  The
+// quad's lanes hold one product of four cells each; a wave-private LDS
+// transpose (ds_write_b32 x 4, ds_read_b128) gives each lane one whole cell,
+// so one dwordx4 store instruction writes runs of 16 cells (256 bytes) in each
+// of the four planes, against 64-byte runs of 4-byte stores without it.
+// Index formula as in vis_index (reference src/kernel.cuh:49-71).
+#define RG_RT_SCRATCH 1280  // LDS bytes per wave: 16 quad groups x 80 (64 + 16 padding)
+
+template <class Acc, class SW>
+__device__ __forceinline__ void rg_store_regtile(const XgJob &J, const TileRef &T,
+                                                 Acc (&acc)[4][4], int wr, int wc, int lane,
+                                                 const SW &swap, unsigned char *lds) {
+  typedef typename RgElem<Acc>::T E;
+  typedef typename RgElem<Acc>::S S;
+  typedef S S4 __attribute__((ext_vector_type(4)));
+  const bool odd = lane & 1;
+  const int hi = lane >> 5, l31 = lane & 31;
+  const int m = l31 >> 3, ry = (l31 >> 2) & 1;
+  const int e = 2 * (lane & 1) + ((l31 >> 1) & 1);  // this lane's product: 2 p1 + p2
+  const int bq = lane & 3;                          // the cell (column block) this lane stores
+  // quad group g = (rx, ry, m); 80-byte group stride keeps the writes conflict free
+  const int g = (hi * 2 + ry) * 4 + m;
+  unsigned char *scr = lds + (wr * 2 + wc) * RG_RT_SCRATCH;
+  const int wofs = g * 80 + 4 * e, rofs = g * 80 + 16 * bq;
+  const int ns = J.nstation;
+  const int accum = J.accumulate;
+  const long long plane = J.plane;
+  const long long RT = (long long)(ns >> 1) * ((ns >> 1) + 1) / 2;
+  // cell (R, C) of quadrant (rx = hi, ry) of channel f: element 4 [(4 f + 2 ry + rx) RT + R(R+1)/2 + C]
+  S *base = (S *)J.out + 4 * (((long long)T.f * 4 + 2 * ry + hi) * RT);
+  const int C = T.cb * 32 + 16 * wc + 4 * bq + m;
+  const int j = 2 * C + ry;
+  const bool diag = T.rb == T.cb;
+  const bool full = !diag && (T.rb * 64 + 64 <= ns);  // every (i, j) valid, i > j
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      __builtin_amdgcn_sched_barrier(0);
+      E re[4], im[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) rg_combine(acc[a][b], q, odd, swap, re[b], im[b]);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) *(S *)(scr + wofs + 16 * b) = (S)re[b];
+      S4 vre = *(const S4 *)(scr + rofs);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) *(S *)(scr + wofs + 16 * b) = (S)im[b];
+      S4 vim = *(const S4 *)(scr + rofs);
+      const int R = T.rb * 32 + 16 * wr + 4 * a + q;
+      const int i = 2 * R + hi;
+      if (!full) {
+        if (i >= ns || j >= ns) continue;
+        if (i < j) {
+          // quadrant "12" of a diagonal cell is never written by the reference
+          // (kernel.cuh:65-71): zero after a clear
+          if ((i >> 1) != (j >> 1) || accum) continue;
+          vre = (S4)(S)0;
+          vim = (S4)(S)0;
+        }
+      }
+      S *p = base + 4 * ((long long)R * (R + 1) / 2 + C);
+      if (accum) {
+        vre += *(const S4 *)p;
+        vim += *(const S4 *)(p + plane);
+        *(S4 *)p = vre;
+        *(S4 *)(p + plane) = vim;
+      } else {
+        __builtin_nontemporal_store(vre, (S4 *)p);
+        __builtin_nontemporal_store(vim, (S4 *)(p + plane));
+      }
+    }
+}
+
+// 32-bit (int32 or float) stores of the two triangular orders with the index
 // arithmetic split into a per-tile 64-bit base and 32-bit lane, row and
 // column terms.  With m = 2i + p1 (row input) and n = 2j + p2 (column input),
 // this lane's element (a, b, q) has p1 = l & 1, p2 = (l >> 1) & 1 and stations
@@ -718,24 +794,16 @@ __device__ __forceinline__ void rg_store32(const XgJob &J, const TileRef &T, Acc
   const bool full = !diag && (T.rb * 64 + 64 <= ns);  // rb > cb: every (i, j) valid, i > j
   const int accum = J.accumulate;
   const long long plane = J.plane;
+  static_assert(ORDER == XG_ORDER_TRI || ORDER == XG_ORDER_REALIMAG, "register tile: rg_store_regtile");
   S *base;
   unsigned kl;  // lane term
-  if (ORDER == XG_ORDER_REGTILE) {
-    // l = (f*4 + 2ry + rx) RT + R(R+1)/2 + C, element 4l + 2p1 + p2, imag at +plane;
-    // R = i >> 1 = rb*32 + 16wr + 4a + q, rx = l >> 5, C = j >> 1, ry = j & 1
-    const long long RT = (long long)(ns >> 1) * ((ns >> 1) + 1) / 2;
-    base = (S *)J.out + 4 * ((long long)T.f * 4 * RT + T.cb * 32 + 16 * wc);
-    const int ry = (l31 >> 2) & 1;
-    kl = (unsigned)(4 * ((2 * ry + hi) * RT) + 2 * p1 + p2 + 4 * (l31 >> 3));
-  } else {
-    const long long NB = (long long)ns * (ns + 1) / 2;
-    if (ORDER == XG_ORDER_TRI) {  // element 8k + 4p1 + 2p2, imag at +1
-      base = (S *)J.out + 8 * ((long long)T.f * NB + T.cb * 64 + 32 * wc);
-      kl = (unsigned)(4 * p1 + 2 * p2 + 8 * (l31 >> 2));
-    } else {                      // REAL_IMAG: element 4k + 2p1 + p2, imag at +plane
-      base = (S *)J.out + 4 * ((long long)T.f * NB + T.cb * 64 + 32 * wc);
-      kl = (unsigned)(2 * p1 + p2 + 4 * (l31 >> 2));
-    }
+  const long long NB = (long long)ns * (ns + 1) / 2;
+  if (ORDER == XG_ORDER_TRI) {  // element 8k + 4p1 + 2p2, imag at +1
+    base = (S *)J.out + 8 * ((long long)T.f * NB + T.cb * 64 + 32 * wc);
+    kl = (unsigned)(4 * p1 + 2 * p2 + 8 * (l31 >> 2));
+  } else {                      // REAL_IMAG: element 4k + 2p1 + p2, imag at +plane
+    base = (S *)J.out + 4 * ((long long)T.f * NB + T.cb * 64 + 32 * wc);
+    kl = (unsigned)(2 * p1 + p2 + 4 * (l31 >> 2));
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -749,26 +817,10 @@ __device__ __forceinline__ void rg_store32(const XgJob &J, const TileRef &T, Acc
         E re, im;
         rg_combine(acc[a][b], q, odd, swap, re, im);
         const int i = i0 + 8 * a + 2 * q, j = j0 + 8 * b;
-        unsigned off;
-        bool zero = false;
-        if (ORDER == XG_ORDER_REGTILE) {
-          const unsigned R = (unsigned)(T.rb * 32 + 16 * wr + 4 * a + q);
-          off = kl + 16 * b + 2 * R * (R + 1);
-        } else if (ORDER == XG_ORDER_TRI) {
-          off = kl + 64 * b + 4 * (unsigned)i * (unsigned)(i + 1);
-        } else {
-          off = kl + 32 * b + 2 * (unsigned)i * (unsigned)(i + 1);
-        }
-        if (!full) {
-          if (i >= ns || j >= ns) continue;
-          if (i < j) {
-            // register-tile quadrant "12" of a diagonal cell is never written by
-            // the reference (kernel.cuh:65-71): zero after a clear
-            if (ORDER != XG_ORDER_REGTILE || (i >> 1) != (j >> 1) || accum) continue;
-            zero = true;
-          }
-        }
-        if (zero) re = im = 0;
+        const unsigned off = ORDER == XG_ORDER_TRI
+                                 ? kl + 64 * b + 4 * (unsigned)i * (unsigned)(i + 1)
+                                 : kl + 32 * b + 2 * (unsigned)i * (unsigned)(i + 1);
+        if (!full && (i >= ns || j >= ns || i < j)) continue;
         S *p = base + off;
         if (ORDER == XG_ORDER_TRI) {
           S2 w;
@@ -797,10 +849,11 @@ __device__ __forceinline__ void rg_store32(const XgJob &J, const TileRef &T, Acc
 
 template <class Acc, class SW>
 __device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, Acc (&acc)[4][4],
-                                            int wr, int wc, int lane, const SW &swap) {
+                                            int wr, int wc, int lane, const SW &swap,
+                                            unsigned char *lds) {
   if (J.out_kind == RgElem<Acc>::kOut32) {
     if (J.order == XG_ORDER_REGTILE)
-      rg_store32<XG_ORDER_REGTILE>(J, T, acc, wr, wc, lane, swap);
+      rg_store_regtile(J, T, acc, wr, wc, lane, swap, lds);
     else if (J.order == XG_ORDER_TRI)
       rg_store32<XG_ORDER_TRI>(J, T, acc, wr, wc, lane, swap);
     else
@@ -831,13 +884,17 @@ __device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, Ac
 // waits for every store and load issued before it, which would serialize the
 // epilogue's stores.  On a diagonal tile the wave above the diagonal
 // (wr < wc) duplicates its mirror and stores nothing.
+// The register-tile stores use the first RG_RT_SCRATCH x 4 bytes of LDS: every
+// wave must be done with the stage buffers (the caller's barrier).
 template <class Acc>
-__device__ __forceinline__ void rg_finish(const XgJob &J, const TileRef &T, Acc (&acc)[4][4]) {
+__device__ __forceinline__ void rg_finish(const XgJob &J, const TileRef &T, Acc (&acc)[4][4],
+                                          unsigned char *lds) {
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
-  if (!(T.rb == T.cb && wr < wc)) rg_epilogue(J, T, acc, wr, wc, lane, RgBperm{4 * (lane ^ 1)});
+  if (!(T.rb == T.cb && wr < wc))
+    rg_epilogue(J, T, acc, wr, wc, lane, RgBperm{4 * (lane ^ 1)}, lds);
 }
 
 // scheduling-group masks (llvm.amdgcn.sched.group.barrier)
@@ -1057,7 +1114,8 @@ __device__ __forceinline__ void rg_run(const XgLaunch &L, unsigned char *lds) {
     }
     // (a diagonal tile's wave above the diagonal runs the same MFMAs as its
     // mirror: one wave per SIMD, so skipping them would save no time)
-    rg_finish(L.jobs[T.job], T, acc);
+    rg_barrier();  // the last stage's writes into the idle buffer are done
+    rg_finish(L.jobs[T.job], T, acc, lds);
     rg_barrier();  // every wave is done with this tile's LDS buffers
     if (!more) break;
     T = N;
@@ -1256,7 +1314,7 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
   // DMAs of stages past the block (zero-filled) may still be in flight: drain
   // them before the next tile reuses the buffers
   rgd_barrier<0>();
-  rg_finish(J, T, acc);
+  rg_finish(J, T, acc, lds);
 }
 
 template <int STG, int NBUF>
