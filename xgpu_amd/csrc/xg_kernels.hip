@@ -696,43 +696,49 @@ __device__ __forceinline__ void store_vis(const XgJob &J, int f, int m, int n, f
   store_flt(J, f, m, n, re, im);
 }
 
-// REGISTER_TILE 32-bit stores through LDS.  A register-tile cell (stations
-// i = 2R + rx, j = 2C + ry) holds its four polarization products (element
-// 2 p1 + p2) in 16 consecutive bytes, and the four cells of a lane quad's
-// column block b = 0..3 are consecutive in each quadrant plane (rx, ry).  The
-// quad's lanes hold one product of four cells each; a wave-private LDS
-// transpose (ds_write_b32 x 4, ds_read_b128) gives each lane one whole cell,
-// so one dwordx4 store instruction writes runs of 16 cells (256 bytes) in each
-// of the four planes, against 64-byte runs of 4-byte stores without it.
-// Index formula as in vis_index (reference src/kernel.cuh:49-71).
-#define RG_RT_SCRATCH 1280  // LDS bytes per wave: 16 quad groups x 80 (64 + 16 padding)
+// 32-bit (int32 or float) stores of the three output orders, one whole
+// cell per lane.  A cell is a station pair (i, j) with its four polarization
+// products (element 2 p1 + p2): 16 bytes per plane in register-tile and
+// real/imag order, 32 bytes of interleaved (re, im) in triangular order.  The
+// four lanes of a quad (p1 = l & 1, p2 = (l >> 1) & 1) hold one product each
+// of the cells of column blocks b = 0..3 (station j = j0 + 8 b); a
+// wave-private LDS transpose (ds_write_b32, ds_read_b128) gives lane l the
+// whole cell of block b = l & 3.  Cells of consecutive j are consecutive in
+// every order, so each store instruction writes runs of 16 (register tile)
+// or 32 cells instead of 4-byte pieces.  Index formulas as in vis_index
+// (reference src/kernel.cuh:49-117).
+#define RG_CELL_GROUP 144                     // LDS bytes per quad group: 4 x 32 + 16 padding
+#define RG_CELL_SCRATCH (16 * RG_CELL_GROUP)  // per wave (16 quad groups)
 
-template <class Acc, class SW>
-__device__ __forceinline__ void rg_store_regtile(const XgJob &J, const TileRef &T,
-                                                 Acc (&acc)[4][4], int wr, int wc, int lane,
-                                                 const SW &swap, unsigned char *lds) {
+template <int ORDER, class Acc, class SW>
+__device__ __forceinline__ void rg_store_cells(const XgJob &J, const TileRef &T,
+                                               Acc (&acc)[4][4], int wr, int wc, int lane,
+                                               const SW &swap, unsigned char *lds) {
   typedef typename RgElem<Acc>::T E;
   typedef typename RgElem<Acc>::S S;
   typedef S S4 __attribute__((ext_vector_type(4)));
+  constexpr bool TRI = ORDER == XG_ORDER_TRI;
   const bool odd = lane & 1;
-  const int hi = lane >> 5, l31 = lane & 31;
-  const int m = l31 >> 3, ry = (l31 >> 2) & 1;
-  const int e = 2 * (lane & 1) + ((l31 >> 1) & 1);  // this lane's product: 2 p1 + p2
-  const int bq = lane & 3;                          // the cell (column block) this lane stores
-  // quad group g = (rx, ry, m); 80-byte group stride keeps the writes conflict free
-  const int g = (hi * 2 + ry) * 4 + m;
-  unsigned char *scr = lds + (wr * 2 + wc) * RG_RT_SCRATCH;
-  const int wofs = g * 80 + 4 * e, rofs = g * 80 + 16 * bq;
+  const int hi = lane >> 5, mj = (lane & 31) >> 2;   // station offsets: i = i0 + hi, j = j0 + mj
+  const int e = 2 * (lane & 1) + ((lane >> 1) & 1);  // this lane's product: 2 p1 + p2
+  const int bq = lane & 3;                           // the cell (column block) this lane stores
+  // quad group (hi, mj); the 144-byte group stride spreads the writes over the banks
+  unsigned char *scr =
+      lds + (wr * 2 + wc) * RG_CELL_SCRATCH + (hi * 8 + mj) * RG_CELL_GROUP;
   const int ns = J.nstation;
   const int accum = J.accumulate;
   const long long plane = J.plane;
-  const long long RT = (long long)(ns >> 1) * ((ns >> 1) + 1) / 2;
-  // cell (R, C) of quadrant (rx = hi, ry) of channel f: element 4 [(4 f + 2 ry + rx) RT + R(R+1)/2 + C]
-  S *base = (S *)J.out + 4 * (((long long)T.f * 4 + 2 * ry + hi) * RT);
-  const int C = T.cb * 32 + 16 * wc + 4 * bq + m;
-  const int j = 2 * C + ry;
+  const int j = T.cb * 64 + 32 * wc + mj + 8 * bq;
   const bool diag = T.rb == T.cb;
   const bool full = !diag && (T.rb * 64 + 64 <= ns);  // every (i, j) valid, i > j
+  S *base;
+  if (ORDER == XG_ORDER_REGTILE) {  // cell (R, C) of quadrant (i & 1, j & 1) of channel f
+    const long long RT = (long long)(ns >> 1) * ((ns >> 1) + 1) / 2;
+    base = (S *)J.out + 4 * (((long long)T.f * 4 + 2 * (j & 1) + hi) * RT + (j >> 1));
+  } else {                          // cell i(i+1)/2 + j of channel f
+    const long long NB = (long long)ns * (ns + 1) / 2;
+    base = (S *)J.out + (TRI ? 8 : 4) * ((long long)T.f * NB + j);
+  }
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -741,108 +747,54 @@ __device__ __forceinline__ void rg_store_regtile(const XgJob &J, const TileRef &
       E re[4], im[4];
 #pragma unroll
       for (int b = 0; b < 4; ++b) rg_combine(acc[a][b], q, odd, swap, re[b], im[b]);
+      S4 v0, v1;  // TRI: the cell's 8 values; else re and im planes
+      if constexpr (TRI) {
 #pragma unroll
-      for (int b = 0; b < 4; ++b) *(S *)(scr + wofs + 16 * b) = (S)re[b];
-      S4 vre = *(const S4 *)(scr + rofs);
+        for (int b = 0; b < 4; ++b) {
+          *(S *)(scr + 32 * b + 8 * e) = (S)re[b];
+          *(S *)(scr + 32 * b + 8 * e + 4) = (S)im[b];
+        }
+        v0 = *(const S4 *)(scr + 32 * bq);
+        v1 = *(const S4 *)(scr + 32 * bq + 16);
+      } else {
 #pragma unroll
-      for (int b = 0; b < 4; ++b) *(S *)(scr + wofs + 16 * b) = (S)im[b];
-      S4 vim = *(const S4 *)(scr + rofs);
-      const int R = T.rb * 32 + 16 * wr + 4 * a + q;
-      const int i = 2 * R + hi;
+        for (int b = 0; b < 4; ++b) *(S *)(scr + 16 * b + 4 * e) = (S)re[b];
+        v0 = *(const S4 *)(scr + 16 * bq);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) *(S *)(scr + 16 * b + 4 * e) = (S)im[b];
+        v1 = *(const S4 *)(scr + 16 * bq);
+      }
+      const int i = T.rb * 64 + 32 * wr + hi + 8 * a + 2 * q;
       if (!full) {
         if (i >= ns || j >= ns) continue;
         if (i < j) {
-          // quadrant "12" of a diagonal cell is never written by the reference
-          // (kernel.cuh:65-71): zero after a clear
-          if ((i >> 1) != (j >> 1) || accum) continue;
-          vre = (S4)(S)0;
-          vim = (S4)(S)0;
+          // quadrant "12" of a diagonal register-tile cell is never written by
+          // the reference (kernel.cuh:65-71): zero after a clear
+          if (ORDER != XG_ORDER_REGTILE || (i >> 1) != (j >> 1) || accum) continue;
+          v0 = (S4)(S)0;
+          v1 = (S4)(S)0;
         }
       }
-      S *p = base + 4 * ((long long)R * (R + 1) / 2 + C);
-      if (accum) {
-        vre += *(const S4 *)p;
-        vim += *(const S4 *)(p + plane);
-        *(S4 *)p = vre;
-        *(S4 *)(p + plane) = vim;
+      S *p, *p1;
+      if (ORDER == XG_ORDER_REGTILE) {
+        const long long R = i >> 1;
+        p = base + 4 * (R * (R + 1) / 2);
+        p1 = p + plane;
       } else {
-        __builtin_nontemporal_store(vre, (S4 *)p);
-        __builtin_nontemporal_store(vim, (S4 *)(p + plane));
+        const long long rowi = (long long)i * (i + 1) / 2;
+        p = base + (TRI ? 8 : 4) * rowi;
+        p1 = TRI ? p + 4 : p + plane;
       }
-    }
-}
-
-// 32-bit (int32 or float) stores of the two triangular orders with the index
-// arithmetic split into a per-tile 64-bit base and 32-bit lane, row and
-// column terms.  With m = 2i + p1 (row input) and n = 2j + p2 (column input),
-// this lane's element (a, b, q) has p1 = l & 1, p2 = (l >> 1) & 1 and stations
-//   i = rb*64 + 32wr + (l >> 5) + 8a + 2q,   j = cb*64 + 32wc + (l%32 >> 2) + 8b.
-// Index formulas as in vis_index (reference src/kernel.cuh:49-117).
-template <int ORDER, class Acc, class SW>
-__device__ __forceinline__ void rg_store32(const XgJob &J, const TileRef &T, Acc (&acc)[4][4],
-                                           int wr, int wc, int lane, const SW &swap) {
-  typedef typename RgElem<Acc>::T E;
-  typedef typename RgElem<Acc>::S S;
-  typedef S S2 __attribute__((ext_vector_type(2)));
-  const bool odd = lane & 1;
-  const int hi = lane >> 5, l31 = lane & 31;
-  const int p1 = odd, p2 = (l31 >> 1) & 1;
-  const int ns = J.nstation;
-  const int i0 = T.rb * 64 + 32 * wr + hi;
-  const int j0 = T.cb * 64 + 32 * wc + (l31 >> 2);
-  const bool diag = T.rb == T.cb;
-  const bool full = !diag && (T.rb * 64 + 64 <= ns);  // rb > cb: every (i, j) valid, i > j
-  const int accum = J.accumulate;
-  const long long plane = J.plane;
-  static_assert(ORDER == XG_ORDER_TRI || ORDER == XG_ORDER_REALIMAG, "register tile: rg_store_regtile");
-  S *base;
-  unsigned kl;  // lane term
-  const long long NB = (long long)ns * (ns + 1) / 2;
-  if (ORDER == XG_ORDER_TRI) {  // element 8k + 4p1 + 2p2, imag at +1
-    base = (S *)J.out + 8 * ((long long)T.f * NB + T.cb * 64 + 32 * wc);
-    kl = (unsigned)(4 * p1 + 2 * p2 + 8 * (l31 >> 2));
-  } else {                      // REAL_IMAG: element 4k + 2p1 + p2, imag at +plane
-    base = (S *)J.out + 4 * ((long long)T.f * NB + T.cb * 64 + 32 * wc);
-    kl = (unsigned)(2 * p1 + p2 + 4 * (l31 >> 2));
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      // one block at a time: keeps the scheduler from reading every
-      // accumulator into VGPRs up front
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        E re, im;
-        rg_combine(acc[a][b], q, odd, swap, re, im);
-        const int i = i0 + 8 * a + 2 * q, j = j0 + 8 * b;
-        const unsigned off = ORDER == XG_ORDER_TRI
-                                 ? kl + 64 * b + 4 * (unsigned)i * (unsigned)(i + 1)
-                                 : kl + 32 * b + 2 * (unsigned)i * (unsigned)(i + 1);
-        if (!full && (i >= ns || j >= ns || i < j)) continue;
-        S *p = base + off;
-        if (ORDER == XG_ORDER_TRI) {
-          S2 w;
-          w.x = (S)re;
-          w.y = (S)im;
-          if (accum) {
-            const S2 o = *(const S2 *)p;
-            w.x += o.x;
-            w.y += o.y;
-          }
-          __builtin_nontemporal_store(w, (S2 *)p);
-        } else {
-          if (accum) {
-            p[0] += (S)re;
-            p[plane] += (S)im;
-          } else {
-            // streaming stores: the tile's output is not read again by this
-            // launch and should not evict the input the next round reuses
-            __builtin_nontemporal_store((S)re, p);
-            __builtin_nontemporal_store((S)im, p + plane);
-          }
-        }
+      if (accum) {
+        v0 += *(const S4 *)p;
+        v1 += *(const S4 *)p1;
+        *(S4 *)p = v0;
+        *(S4 *)p1 = v1;
+      } else {
+        // streaming stores: the tile's output is not read again by this
+        // launch and should not evict the input the next round reuses
+        __builtin_nontemporal_store(v0, (S4 *)p);
+        __builtin_nontemporal_store(v1, (S4 *)p1);
       }
     }
 }
@@ -853,11 +805,11 @@ __device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, Ac
                                             unsigned char *lds) {
   if (J.out_kind == RgElem<Acc>::kOut32) {
     if (J.order == XG_ORDER_REGTILE)
-      rg_store_regtile(J, T, acc, wr, wc, lane, swap, lds);
+      rg_store_cells<XG_ORDER_REGTILE>(J, T, acc, wr, wc, lane, swap, lds);
     else if (J.order == XG_ORDER_TRI)
-      rg_store32<XG_ORDER_TRI>(J, T, acc, wr, wc, lane, swap);
+      rg_store_cells<XG_ORDER_TRI>(J, T, acc, wr, wc, lane, swap, lds);
     else
-      rg_store32<XG_ORDER_REALIMAG>(J, T, acc, wr, wc, lane, swap);
+      rg_store_cells<XG_ORDER_REALIMAG>(J, T, acc, wr, wc, lane, swap, lds);
     return;
   }
   const bool odd = lane & 1;  // int64 / double output: generic path
@@ -884,7 +836,7 @@ __device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, Ac
 // waits for every store and load issued before it, which would serialize the
 // epilogue's stores.  On a diagonal tile the wave above the diagonal
 // (wr < wc) duplicates its mirror and stores nothing.
-// The register-tile stores use the first RG_RT_SCRATCH x 4 bytes of LDS: every
+// The 32-bit stores use the first RG_CELL_SCRATCH x 4 bytes of LDS: every
 // wave must be done with the stage buffers (the caller's barrier).
 template <class Acc>
 __device__ __forceinline__ void rg_finish(const XgJob &J, const TileRef &T, Acc (&acc)[4][4],
