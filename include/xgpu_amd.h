@@ -148,7 +148,8 @@ typedef struct XGPUAmdTuningStruct {
                       a hipGraph once and replay it, 2 = issue them one by one */
   int kernel;      /* int8 and fp16: 1 = complex-plane kernel (any input), 2 = real-Gram
                       kernel where it applies (16-byte aligned input and, for int8,
-                      nstation % 4 == 0; default) */
+                      nstation % 4 == 0; default); int8: 3 = K-interleaved kernel
+                      (pair tiles of 128 x 256 inputs) where the real-Gram one applies */
   int stage;       /* real-Gram kernel: samples per LDS stage: int8 64 or 128 with
                       register staging, 32 or 64 with DMA staging; fp16 32 or 64
                       (0 = default) */

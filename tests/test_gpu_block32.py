@@ -55,7 +55,7 @@ def run_block32(ns, nf, nt, xb, compute, order, calls=1, **tuning):
     (48, 2, 300, "regtile", 2),     # 96 inputs: one partial tile per side
     (144, 2, 100, "realimag", 2),   # 288 inputs: three tiles per side
 ])
-@pytest.mark.parametrize("kernel,staging,stage", [(1, 0, 0), (2, 1, 64), (2, 2, 32)])
+@pytest.mark.parametrize("kernel,staging,stage", [(1, 0, 0), (2, 1, 64), (2, 2, 32), (3, 0, 0)])
 def test_block32_int8_exact(ns, nf, nt, order, calls, kernel, staging, stage):
     xb = oracle.fill_uniform(nt * nf * ns * 4, 60 + ns, -128, 127)
     got = run_block32(ns, nf, nt, xb, X.XGPU_INT8, ORDERS[order], calls,

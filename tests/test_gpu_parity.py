@@ -53,8 +53,9 @@ def run_i8(ns, nf, nt, x, order=X.REGISTER_TILE_TRIANGULAR_ORDER, calls=1, **tun
 
 
 # (kernel, staging, stage): complex-plane kernel; real-Gram kernel with register
-# staging (64 / 128 samples per stage) and with global->LDS DMA (32 / 64)
-I8_VARIANTS = [(1, 0, 0), (2, 1, 64), (2, 1, 128), (2, 2, 32), (2, 2, 64)]
+# staging (64 / 128 samples per stage) and with global->LDS DMA (32 / 64);
+# K-interleaved kernel (pair tiles)
+I8_VARIANTS = [(1, 0, 0), (2, 1, 64), (2, 1, 128), (2, 2, 32), (2, 2, 64), (3, 0, 0)]
 
 
 @pytest.mark.parametrize("ns,nf,nt,order", [
@@ -71,7 +72,8 @@ I8_VARIANTS = [(1, 0, 0), (2, 1, 64), (2, 1, 128), (2, 2, 32), (2, 2, 64)]
 def test_i8_matches_exact_oracle(ns, nf, nt, order, kernel, staging, stage):
     """kernel 1: complex-plane int8 MFMA kernel; kernel 2: real-Gram kernel
     (used where nstation % 4 == 0, else the engine takes kernel 1) with
-    register (1) or global->LDS DMA (2) staging."""
+    register (1) or global->LDS DMA (2) staging; kernel 3: K-interleaved
+    kernel (same inputs as kernel 2)."""
     if (kernel, staging, stage, ns, nf, nt, order) == (2, 2, 32, 96, 2, 257, "realimag"):
         pytest.skip("this case faulted the GPU twice with a since-reverted change to the DMA "
                     "kernel (operands made scalar with readfirstlane, per-order epilogue "
@@ -83,7 +85,7 @@ def test_i8_matches_exact_oracle(ns, nf, nt, order, kernel, staging, stage):
     assert np.array_equal(to_tri(None, wide, ns, nf, ORDERS[order]), ref)
 
 
-@pytest.mark.parametrize("kernel,staging", [(1, 0), (2, 1), (2, 2)])
+@pytest.mark.parametrize("kernel,staging", [(1, 0), (2, 1), (2, 2), (3, 0)])
 def test_i8_minus_128_everywhere(kernel, staging):
     ns, nf, nt = 64, 2, 4096
     x = np.full(nt * nf * ns * 4, -128, dtype=np.int8)
@@ -105,7 +107,7 @@ def test_i8_int32_bound_block_exact():
     autocorrelation) without a spill, in both int8 kernels."""
     ns, nf, nt = 4, 1, 65535
     x = np.full(nt * nf * ns * 4, -128, dtype=np.int8)
-    for kernel, staging in ((1, 0), (2, 1), (2, 2)):
+    for kernel, staging in ((1, 0), (2, 1), (2, 2), (3, 0)):
         raw, wide = run_i8(ns, nf, nt, x, X.TRIANGULAR_ORDER, kernel=kernel, staging=staging)
         v = raw.reshape(-1, 2).astype(np.int64)
         assert np.all(v[:, 0] == nt * 2 * 128 * 128) and np.all(v[:, 1] == 0), (kernel, staging)
@@ -449,18 +451,18 @@ def test_autotune_keeps_results_exact(tmp_path):
     assert ms > 0 and best["wg_per_cu"] in (1, 2) and best["prefetch"] in (1, 2)
     eng.correlate(buf, nt)
     assert np.array_equal(eng.dump().astype(np.int64), ref)
-    assert best["kernel"] in (1, 2) and best["stage"] in (0, 32, 64, 128)
+    assert best["kernel"] in (1, 2, 3) and best["stage"] in (0, 32, 64, 128)
     for wg in (1, 2):           # every candidate is exact
         for pf in (1, 2):
             eng.set_tuning(wg, pf, kernel=1)
             eng.clear()
             eng.correlate(buf, nt)
             assert np.array_equal(eng.dump().astype(np.int64), ref), (wg, pf)
-    for _, staging, stage in I8_VARIANTS[1:]:
-        eng.set_tuning(kernel=2, staging=staging, stage=stage)
+    for kernel, staging, stage in I8_VARIANTS[1:]:
+        eng.set_tuning(kernel=kernel, staging=staging, stage=stage)
         eng.clear()
         eng.correlate(buf, nt)
-        assert np.array_equal(eng.dump().astype(np.int64), ref), (staging, stage)
+        assert np.array_equal(eng.dump().astype(np.int64), ref), (kernel, staging, stage)
     eng2 = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
     assert eng2.load_tuning(cache, nt)
     assert eng2.get_tuning()["wg_per_cu"] == best["wg_per_cu"]

@@ -25,7 +25,7 @@ class XgJob(C.Structure):
                 ("plane", C.c_longlong), ("nstation", C.c_int), ("nfreq", C.c_int),
                 ("ntime", C.c_int), ("nblk", C.c_int), ("tiles_per_chan", C.c_int),
                 ("order", C.c_int), ("accumulate", C.c_int), ("out_kind", C.c_int),
-                ("chan_base", C.c_int), ("reserved", C.c_int)]
+                ("chan_base", C.c_int), ("pair_tiles", C.c_int)]
 
 
 class XgLaunch(C.Structure):
@@ -115,6 +115,43 @@ def test_tile_order_is_a_permutation_of_the_triangle():
             got.append((v[2].value, v[3].value))
             k += 1
         assert sorted(got) == [(r, c) for r in range(nblk) for c in range(r + 1)], nblk
+
+
+def pair_count(nblk):
+    return sum(nblk - 2 * cp for cp in range((nblk + 1) // 2))
+
+
+def test_pair_tile_order_covers_the_triangle():
+    """Pair tiles (rb, cp) of the K-interleaved kernel: 128 row inputs by 256
+    column inputs with 2 cp <= rb.  Every pair is taken once, and every input
+    pair (m, n) with n <= m + 1 (the station-level lower triangle) lies in
+    exactly one of them."""
+    lib = X.core()
+    fn = lib.xgh_schedule
+    fn.argtypes = [C.POINTER(XgLaunch), C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 4
+    for nblk in list(range(1, 41)) + [64]:
+        L = plan([(nblk * XG_TILE // 2, 1)], cus=8, wg_per_cu=1)
+        L.jobs[0].pair_tiles = 1
+        L.jobs[0].tiles_per_chan = pair_count(nblk)
+        lib.xg_plan_launch(C.byref(L), 8, 1)
+        L.wg_per_xcd = 1
+        v = [C.c_int() for _ in range(4)]
+        got = []
+        k = 0
+        while fn(C.byref(L), 0, k, *[C.byref(a) for a in v]):
+            got.append((v[2].value, v[3].value))
+            k += 1
+        assert sorted(got) == sorted((r, c) for r in range(nblk) for c in range(r // 2 + 1)), nblk
+        if nblk <= 8:
+            ninput = nblk * XG_TILE
+            cover = {}
+            for rb, cp in got:
+                for m in range(rb * XG_TILE, (rb + 1) * XG_TILE):
+                    for n in range(2 * cp * XG_TILE, min((2 * cp + 2) * XG_TILE, m + 2)):
+                        cover[(m, n)] = cover.get((m, n), 0) + 1
+            need = {(m, n) for m in range(ninput) for n in range(min(ninput, m + 2))
+                    if (n >> 1) <= (m >> 1)}
+            assert need <= set(cover) and all(cover[p] == 1 for p in need), nblk
 
 
 def chunk_streams(sched, W):

@@ -102,6 +102,13 @@ int xgpuAmdAutotune(XGPUAmdEngine *e, const void *input_d, unsigned ntime, int w
       t.stage = fam == XG_KERNEL_I8_RG ? v_i8[v][1] : v_f16[v][1];
       cand[ncand++] = t;
     }
+    if (fam == XG_KERNEL_I8_RG) { /* the K-interleaved kernel takes the same inputs */
+      XGPUAmdTuning t = keep;
+      t.kernel = 3;
+      t.staging = 0;
+      t.stage = 0;
+      cand[ncand++] = t;
+    }
   }
   for (int c = 0; c < ncand; ++c) {
     const XGPUAmdTuning t = cand[c];

@@ -122,6 +122,21 @@ static void job_fill(XgJob *J, const XGPUAmdEngine *e, const void *input, unsign
   J->chan_base = chan_base;
 }
 
+/* Number of pair tiles of the K-interleaved kernel: XG_TILE row inputs by
+ * 2 XG_TILE column inputs, (rb, cp) with 2 cp <= rb (xg_pair_count in
+ * xg_kernels.hip). */
+static int pair_count(int nblk) {
+  int n = 0;
+  for (int cp = 0; 2 * cp < nblk; ++cp) n += nblk - 2 * cp;
+  return n;
+}
+
+/* Tile kind of a job for the kernel family that runs it. */
+static void job_tiles(XgJob *J, int family) {
+  J->pair_tiles = (family == XG_KERNEL_I8_KG);
+  J->tiles_per_chan = J->pair_tiles ? pair_count(J->nblk) : J->nblk * (J->nblk + 1) / 2;
+}
+
 void xg_plan_launch(XgLaunch *L, int cus, int wg_per_cu) {
   int maxx = 0;
   for (int x = 0; x < XG_NXCD; ++x) {
@@ -239,7 +254,9 @@ int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
     return xg_fail(XGPU_INVALID_CONFIG, "wg_per_cu must be 0..2");
   if (t->prefetch < 0 || t->prefetch > 2) return xg_fail(XGPU_INVALID_CONFIG, "prefetch must be 0..2");
   if (t->graph < 0 || t->graph > 2) return xg_fail(XGPU_INVALID_CONFIG, "graph must be 0..2");
-  if (t->kernel < 0 || t->kernel > 2) return xg_fail(XGPU_INVALID_CONFIG, "kernel must be 0..2");
+  if (t->kernel < 0 || t->kernel > 3) return xg_fail(XGPU_INVALID_CONFIG, "kernel must be 0..3");
+  if (t->kernel == 3 && e->family != XG_KERNEL_I8_MFMA)
+    return xg_fail(XGPU_INVALID_CONFIG, "kernel 3 (K-interleaved) is int8 only");
   if (t->staging < 0 || t->staging > 2) return xg_fail(XGPU_INVALID_CONFIG, "staging must be 0..2");
   int staging, stage;
   if (e->family == XG_KERNEL_F16_MFMA) {
@@ -304,7 +321,8 @@ int xg_launch_family(const XGPUAmdEngine *e, const void *input_d) {
   if ((uintptr_t)input_d & 15) return e->family;
   if (e->family == XG_KERNEL_I8_MFMA && e->cfg.nstation % 4) return e->family;
   if (128LL * e->tstride >= (1LL << 31) - 4096) return e->family;
-  return e->family == XG_KERNEL_I8_MFMA ? XG_KERNEL_I8_RG : XG_KERNEL_F16_RG;
+  if (e->family == XG_KERNEL_I8_MFMA) return e->kernel_pref == 3 ? XG_KERNEL_I8_KG : XG_KERNEL_I8_RG;
+  return XG_KERNEL_F16_RG;
 }
 
 /* XgLaunch.stage code of the real-Gram kernel variant */
@@ -314,9 +332,12 @@ static char rg_stage_code(const XGPUAmdEngine *e) {
   return (char)(e->stage == 128 ? 1 : 0);
 }
 
-/* one workgroup per CU: the real-Gram kernels hold 512 registers per lane */
+/* one workgroup per CU: the real-Gram and K-interleaved kernels hold 512
+ * registers per lane */
 static int wg_per_cu_for(const XGPUAmdEngine *e, int family) {
-  return (family == XG_KERNEL_I8_RG || family == XG_KERNEL_F16_RG) ? 1 : e->wg_per_cu;
+  return (family == XG_KERNEL_I8_RG || family == XG_KERNEL_F16_RG || family == XG_KERNEL_I8_KG)
+             ? 1
+             : e->wg_per_cu;
 }
 
 int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime) {
@@ -325,6 +346,7 @@ int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime) {
   L.njobs = 1;
   const int family = xg_launch_family(e, input_d);
   job_fill(&L.jobs[0], e, input_d, ntime, !e->cleared, 0);
+  job_tiles(&L.jobs[0], family);
   xg_plan_launch(&L, e->cus, wg_per_cu_for(e, family));
   L.prefetch = (char)e->prefetch;
   L.stage = rg_stage_code(e);
@@ -654,6 +676,7 @@ int xgpuAmdCorrelateBatch(const XGPUAmdBatchItem *items, int nitems, void *strea
   int family = xg_launch_family(e0, L.jobs[0].input);
   for (int i = 1; i < nitems; ++i)
     if (xg_launch_family(items[i].engine, L.jobs[i].input) != family) family = e0->family;
+  for (int i = 0; i < nitems; ++i) job_tiles(&L.jobs[i], family);
   xg_plan_launch(&L, e0->cus, wg_per_cu_for(e0, family));
   L.prefetch = (char)e0->prefetch;
   L.stage = rg_stage_code(e0);

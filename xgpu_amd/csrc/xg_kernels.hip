@@ -95,6 +95,28 @@ __host__ __device__ inline void xg_tile_order(int nblk, int u, int *rb, int *cb)
   *rb = *cb = -1;  // u past the triangle (not reached for u < nblk (nblk+1) / 2)
 }
 
+// Pair tiles (rb, cp): XG_TILE row inputs [XG_TILE rb, +XG_TILE) by 2 XG_TILE
+// column inputs [2 XG_TILE cp, +2 XG_TILE), for 2 cp <= rb: every input pair
+// (m, n) with n <= m + 1 lies in exactly one of them (rb = m / 128,
+// cp = n / 256).  Order: column by column, rows from the diagonal down.
+__host__ __device__ inline int xg_pair_count(int nblk) {
+  int n = 0;
+  for (int cp = 0; 2 * cp < nblk; ++cp) n += nblk - 2 * cp;
+  return n;
+}
+__host__ __device__ inline void xg_pair_order(int nblk, int u, int *rb, int *cp) {
+  for (int c = 0; 2 * c < nblk; ++c) {
+    const int n = nblk - 2 * c;
+    if (u < n) {
+      *rb = 2 * c + u;
+      *cp = c;
+      return;
+    }
+    u -= n;
+  }
+  *rb = *cp = -1;  // u past the last pair tile (not reached for u < xg_pair_count)
+}
+
 // k-th tile of XCD `xcd`'s share (see xg_launch.h).  Returns false past the end.
 // Within a job the XCD's channels are visited in two passes: first, for each
 // channel in turn, the largest multiple W of its tiles that fills whole rounds
@@ -124,7 +146,10 @@ __host__ __device__ inline bool xg_tile(const XgLaunch &L, int xcd, int k, TileR
   const int f0 = (((xcd - J.chan_base) % XG_NXCD) + XG_NXCD) % XG_NXCD;
   t.job = j;
   t.f = f0 + XG_NXCD * ci;
-  xg_tile_order(J.nblk, u, &t.rb, &t.cb);
+  if (J.pair_tiles)
+    xg_pair_order(J.nblk, u, &t.rb, &t.cb);
+  else
+    xg_tile_order(J.nblk, u, &t.rb, &t.cb);
   return true;
 }
 
@@ -1230,6 +1255,19 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
 #pragma unroll
   for (int st = 0; st < NBUF; ++st) dma_stage(st);
   rgd_barrier<(NBUF - 1) * 2 * C::kDma>();
+  if (diag && wr < wc) {
+    // On a diagonal tile this wave's block is the mirror of wave (1, 0)'s and
+    // is never stored: it keeps its share of the DMAs and the stage barriers
+    // (same order, same counts) and issues no MFMAs, which the chip's power
+    // budget gives to the other waves.
+    const int nrun = (C::kSteps & 1) ? (nstage + 1) & ~1 : nstage;
+    for (int s = 0; s < nrun; ++s) {
+      rgd_barrier<(NBUF - 2) * 2 * C::kDma>();
+      dma_stage(s + NBUF);
+    }
+    rgd_barrier<0>();
+    return;  // nothing to store (rg_finish skips this wave on a diagonal tile)
+  }
   lread(lds, A0, B0);
   // One stage whose kstep-0 operands are in operand set P.  The last kstep
   // reads the next stage's kstep 0 into set (P + kSteps) & 1.
@@ -1278,6 +1316,328 @@ void xg_xengine_rgd(XgLaunch L) {
   TileRef T;
   for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
     rgd_tile<STG, NBUF>(L.jobs[T.job], T, lds);
+    rg_barrier();  // every wave is done with this tile's LDS buffers
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K-interleaved int8 kernel (family XG_KERNEL_I8_KG).  The MFMA's K runs over
+// (sample, re/im) pairs: a complex input is one 16-bit element (re byte, im
+// byte), as the input stores it, so a tile needs one accumulator per output
+// value instead of the real-Gram's four per complex product:
+//   Re(m, n) = sum_t re_m re_n + im_m im_n = A . B
+//   Im(m, n) = sum_t im_m re_n - re_m im_n = A'' . B + C_n
+// with A = (re_m, im_m) and B = (re_n, im_n) per sample, A'' = (im_m, ~re_m)
+// (~x = -x - 1 is exact for every int8, where -x is not for -128) and the
+// column sum C_n = sum_t im_n, which the lanes accumulate with dot4 on the B
+// operands.  With one accumulator per output value, a workgroup holds a pair
+// tile of 128 row inputs x 256 column inputs (2 x 4 Re and 2 x 4 Im 32 x 32
+// blocks per wave): 768 input bytes per sample for 131,072 MACs, against 512
+// bytes for 65,536 in the real-Gram kernel.
+//
+// LDS: stages of 32 samples, 4 buffers (3 stages of DMA in flight).  A sample
+// row is 256 bytes (row side) or 512 bytes (column side) of 16-byte chunks
+// (8 inputs); chunk c of a row of sample s sits in slot c ^ ((s & 3) << 2).
+// Operands: ds_read_b64_tr_b16 pairs as in the fp16 kernel (each lane gets 8
+// consecutive samples of one input).  Waves: 2 x 2, wave (wr, wc) owns rows
+// 64 wr .. +63 and columns 128 wc .. +127 of the tile.
+// ---------------------------------------------------------------------------
+#define KG_AROW 256                     // LDS bytes per sample, row side (128 inputs)
+#define KG_BROW 512                     // column side (256 inputs)
+#define KG_STG 32                       // samples per stage
+#define KG_NBUF 4                       // stage buffers
+#define KG_SIDEA (KG_STG * KG_AROW)     // 8 KiB
+#define KG_BUF (KG_SIDEA + KG_STG * KG_BROW)  // 24 KiB per stage
+#define KG_DMA 6                        // DMAs per wave per stage: 2 row-side, 4 column-side
+
+__device__ __forceinline__ int kg_slot(int chunk, int s3) { return chunk ^ (s3 << 2); }
+
+// Per-lane LDS offset of the operand of 32-input block `blk` for samples
+// 0..15 of a kstep, in rows of `row` bytes (the tr16 lane rule of
+// rgh_lane_off, with inputs in place of real rows).
+__device__ __forceinline__ int kg_lane_off(int blk, int lane, int row) {
+  const int g = (lane >> 4) & 1, kg = lane >> 5, k = (lane >> 2) & 3, q = lane & 3;
+  const int chunk = 4 * blk + 2 * g + (q >> 1);
+  return (8 * kg + k) * row + (kg_slot(chunk, k) << 4) + ((q & 1) << 3);
+}
+
+template <int ROW>
+__device__ __forceinline__ v4i kg_operand(const unsigned char *p) {
+  const v4s lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)p);
+  const v4s hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)(p + 4 * ROW));
+  const v2i l2 = __builtin_bit_cast(v2i, lo), h2 = __builtin_bit_cast(v2i, hi);
+  v4i out;
+  out[0] = l2.x;
+  out[1] = l2.y;
+  out[2] = h2.x;
+  out[3] = h2.y;
+  return out;
+}
+
+// (re, im) -> (im, ~re) in every 16-bit element
+__device__ __forceinline__ v4i kg_rotate(v4i a) {
+  v4i o;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    o[i] = (int)(__builtin_amdgcn_perm((unsigned)a[i], (unsigned)a[i], 0x02030001u) ^ 0xFF00FF00u);
+  return o;
+}
+
+// DPP exchange with the neighbouring lane (l ^ 1)
+__device__ __forceinline__ int kg_swap1(int x) {
+  return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, true);  // quad_perm [1, 0, 3, 2]
+}
+
+// 32-bit stores of one pair tile, one whole cell per lane.  For block (a, b)
+// and element group q, lane l holds column input n = 256 cp + 128 wc + 32 b +
+// (l & 31) (station j, polarization p2 = l & 1) against 4 row inputs
+// (stations i0, i0 + 1, both p1), i0 = 64 rb + 32 wr + 16 a + 4 q +
+// 2 (l >> 5).  With its neighbour l ^ 1 (the other p2) it completes cell
+// (i0, j) (even lane) and (i0 + 1, j) (odd lane): 4 products in order
+// 2 p1 + p2.  One code path serves the three orders (only the address and,
+// for triangular order, the re/im interleave differ), so the compiler has
+// no per-order copies of the accumulator reads to hoist.  Index formulas as
+// in vis_index (reference src/kernel.cuh:49-117).
+__device__ __forceinline__ void kg_store_cells(const XgJob &J, const TileRef &T, v16i (&are)[2][4],
+                                               v16i (&aim)[2][4], const int (&corr)[4], int wr,
+                                               int wc, int lane) {
+  typedef unsigned S4 __attribute__((ext_vector_type(4)));
+  const bool odd = lane & 1;
+  const int ns = J.nstation;
+  const bool accum = J.accumulate;
+  const bool regt = J.order == XG_ORDER_REGTILE, tri = J.order == XG_ORDER_TRI;
+  const long long plane = J.plane;
+  const int jl = (lane & 31) >> 1;
+  const int i_lane = T.rb * 64 + 32 * wr + 2 * (lane >> 5) + (odd ? 1 : 0);
+  const bool full = (T.cb * 128 + 127 < T.rb * 64) && (T.rb * 64 + 64 <= ns);
+  const long long RT = (long long)(ns >> 1) * ((ns >> 1) + 1) / 2;
+  const long long NB = (long long)ns * (ns + 1) / 2;
+  unsigned *const out = (unsigned *)J.out;
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        __builtin_amdgcn_sched_barrier(0);
+        int re[4], im[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          re[e] = are[a][b][4 * q + e];
+          im[e] = rg_add(aim[a][b][4 * q + e], corr[b]);
+        }
+        // odd lanes send their e = 0, 1 (row i0), even lanes e = 2, 3 (row i0 + 1)
+        const int r0 = kg_swap1(odd ? re[0] : re[2]), r1 = kg_swap1(odd ? re[1] : re[3]);
+        const int m0 = kg_swap1(odd ? im[0] : im[2]), m1 = kg_swap1(odd ? im[1] : im[3]);
+        S4 cr, ci;
+        if (!odd) {
+          cr = S4{(unsigned)re[0], (unsigned)r0, (unsigned)re[1], (unsigned)r1};
+          ci = S4{(unsigned)im[0], (unsigned)m0, (unsigned)im[1], (unsigned)m1};
+        } else {
+          cr = S4{(unsigned)r0, (unsigned)re[2], (unsigned)r1, (unsigned)re[3]};
+          ci = S4{(unsigned)m0, (unsigned)im[2], (unsigned)m1, (unsigned)im[3]};
+        }
+        const int i = i_lane + 16 * a + 4 * q;
+        const int j = T.cb * 128 + 64 * wc + 16 * b + jl;
+        if (!full) {
+          if (i >= ns || j >= ns) continue;
+          if (i < j) {
+            // quadrant "12" of a diagonal register-tile cell is never written by
+            // the reference (kernel.cuh:65-71): zero after a clear
+            if (!regt || (i >> 1) != (j >> 1) || accum) continue;
+            cr = (S4)0u;
+            ci = (S4)0u;
+          }
+        }
+        long long cell;  // first element of the cell (real plane / interleaved)
+        if (regt)
+          cell = 4 * ((((long long)T.f * 4 + 2 * (j & 1) + (i & 1)) * RT) +
+                      (long long)(i >> 1) * ((i >> 1) + 1) / 2 + (j >> 1));
+        else
+          cell = (tri ? 8 : 4) * ((long long)T.f * NB + (long long)i * (i + 1) / 2 + j);
+        unsigned *const p = out + cell;
+        unsigned *const p1 = tri ? p + 4 : p + plane;
+        S4 v0 = cr, v1 = ci;
+        if (tri) {
+          v0 = S4{cr[0], ci[0], cr[1], ci[1]};
+          v1 = S4{cr[2], ci[2], cr[3], ci[3]};
+        }
+        if (accum) {
+          v0 += *(const S4 *)p;
+          v1 += *(const S4 *)p1;
+        }
+        // streaming stores: the tile's output is not read again by this
+        // launch and should not evict the input the next round reuses
+        __builtin_nontemporal_store(v0, (S4 *)p);
+        __builtin_nontemporal_store(v1, (S4 *)p1);
+      }
+}
+
+// Output stage of a pair tile (int32 output, the only kind the engine gives
+// this kernel); the lane index reaches it through an opaque copy (see
+// rg_finish).  The column correction needs both sample halves of the
+// column's sums (lanes l and l ^ 32).
+__device__ __forceinline__ void kg_finish(const XgJob &J, const TileRef &T, v16i (&are)[2][4],
+                                          v16i (&aim)[2][4], const int (&csum)[4]) {
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  int corr[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+    corr[b] = rg_add(csum[b], __builtin_amdgcn_ds_bpermute(4 * (lane ^ 32), csum[b]));
+  kg_store_cells(J, T, are, aim, corr, wr, wc, lane);
+}
+
+__device__ __forceinline__ void kg_tile(const XgJob &J, const TileRef &T, unsigned char *lds) {
+  // the lane constants below are recomputed per tile (opaque copy of the
+  // thread index) instead of being hoisted out of the tile loop and spilled
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ninput = 2 * J.nstation;
+  const int nstage = (J.ntime + KG_STG - 1) / KG_STG;
+  const int ntime = J.ntime;
+  const int ts = (int)J.tstride;  // < 2^24: the host keeps 128 * tstride < 2^31
+  const unsigned char *in_f = (const unsigned char *)J.input + (long long)T.f * J.nstation * 4;
+  const unsigned long long abase = (unsigned long long)(in_f + T.rb * KG_AROW);
+  const unsigned long long bbase = (unsigned long long)(in_f + T.cb * KG_BROW);
+  const int arow_bytes = ts < KG_AROW ? ts : KG_AROW;
+  const int brow_bytes = ts < KG_BROW ? ts : KG_BROW;
+  const unsigned lds_base = (unsigned)(unsigned long long)lds;
+
+  // DMA lane mapping.  Row side: a DMA covers 4 samples of 256 bytes, lane L
+  // fills slot L & 15 of sample row L >> 4 (sample & 3 == L >> 4).  Column
+  // side: 2 samples of 512 bytes, lane L fills slot L & 31 of sample row
+  // L >> 5; sample & 3 is (L >> 5) or 2 + (L >> 5) for even / odd DMAs.
+  // Chunks past the last input read past the range: zeros.
+  int voa, vob[2];
+  {
+    const int r = lane >> 4, c = kg_slot(lane & 15, r);
+    voa = (T.rb * XG_TILE + 8 * c < ninput) ? r * ts + 16 * c : 0x7fffff00;
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+      const int rr = lane >> 5, cc = kg_slot(lane & 31, (2 * par + rr) & 3);
+      vob[par] = (T.cb * 2 * XG_TILE + 8 * cc < ninput) ? rr * ts + 16 * cc : 0x7fffff00;
+    }
+  }
+  // DMA d of stage st for this wave: d = 0, 1: row-side DMA 2 wave + d;
+  // d = 2..5: column-side DMA 4 wave + d - 2.
+  auto dma = [&](int st, int d) {
+    v4i rs;
+    int voff;
+    unsigned m0 = lds_base + (st & (KG_NBUF - 1)) * KG_BUF;
+    unsigned long long a;
+    if (d < 2) {
+      const int g = 2 * wave + d;
+      const int s0 = st * KG_STG + 4 * g;
+      const int rows = min(ntime - s0, 4);
+      rs.z = max(rows * ts - ts + arow_bytes, 0);
+      a = abase + (unsigned long long)(long long)s0 * ts;
+      voff = voa;
+      m0 += 4 * g * KG_AROW;
+    } else {
+      const int j = 4 * wave + d - 2;
+      const int s0 = st * KG_STG + 2 * j;
+      const int rows = min(ntime - s0, 2);
+      rs.z = max(rows * ts - ts + brow_bytes, 0);
+      a = bbase + (unsigned long long)(long long)s0 * ts;
+      voff = vob[(d - 2) & 1];
+      m0 += KG_SIDEA + 2 * j * KG_BROW;
+    }
+    rs.x = (int)(unsigned)a;
+    rs.y = (int)((a >> 32) & 0xffff);
+    rs.w = 0x00020000;
+    rg_dma16(rs, voff, m0);
+  };
+
+  const int wr = wave >> 1, wc = wave & 1;
+  int aoff[2], boff[4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) aoff[a] = kg_lane_off(2 * wr + a, lane, KG_AROW);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) boff[b] = KG_SIDEA + kg_lane_off(4 * wc + b, lane, KG_BROW);
+  v16i are[2][4], aim[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        are[a][b][e] = 0;
+        aim[a][b][e] = 0;
+      }
+  int csum[4] = {0, 0, 0, 0};
+
+  v4i A0[2], B0[4], A1[2], B1[4];
+  // operands of kstep kk (0: samples 0..15, 1: 16..31) of the stage at base
+  auto lread = [&](const unsigned char *base, int kk, v4i *A, v4i *B) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a) A[a] = kg_operand<KG_AROW>(base + aoff[a] + kk * 16 * KG_AROW);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) B[b] = kg_operand<KG_BROW>(base + boff[b] + kk * 16 * KG_BROW);
+  };
+  // One kstep: 8 Re MFMAs, the next kstep's operand reads, 8 Im MFMAs over
+  // which (FILL) the stage's DMAs are issued.
+  auto kstep = [&](auto fill_c, const v4i *A, const v4i *B, v4i *NA, v4i *NB,
+                   const unsigned char *nbase, int nkk, int st) {
+    constexpr bool FILL = decltype(fill_c)::value;
+    v4i Ar[2];
+    Ar[0] = kg_rotate(A[0]);
+    Ar[1] = kg_rotate(A[1]);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) csum[b] = __builtin_amdgcn_sdot4(B[b][i], 0x01000100, csum[b], false);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      are[i >> 2][i & 3] =
+          __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i >> 2], B[i & 3], are[i >> 2][i & 3], 0, 0, 0);
+    lread(nbase, nkk, NA, NB);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      aim[i >> 2][i & 3] =
+          __builtin_amdgcn_mfma_i32_32x32x32_i8(Ar[i >> 2], B[i & 3], aim[i >> 2][i & 3], 0, 0, 0);
+      if constexpr (FILL) {
+        if (i < KG_DMA) dma(st, i);
+      }
+    }
+  };
+  typedef integral_c<bool, true> yes;
+  typedef integral_c<bool, false> no;
+
+  // prologue: DMAs of stages 0 .. NBUF-1, wait for stage 0
+#pragma unroll
+  for (int st = 0; st < KG_NBUF; ++st)
+#pragma unroll
+    for (int d = 0; d < KG_DMA; ++d) dma(st, d);
+  rgd_barrier<(KG_NBUF - 1) * KG_DMA>();
+  lread(lds, 0, A0, B0);
+  for (int s = 0; s < nstage; ++s) {
+    const unsigned char *cur = lds + (s & (KG_NBUF - 1)) * KG_BUF;
+    const unsigned char *nxt = lds + ((s + 1) & (KG_NBUF - 1)) * KG_BUF;
+    kstep(no(), A0, B0, A1, B1, cur, 1, 0);
+    // stage s+1 has landed (this wave's DMAs; the barrier covers the others'),
+    // and every wave is done reading stage s's buffer, which is refilled with
+    // stage s+NBUF during the last kstep
+    rgd_barrier<(KG_NBUF - 2) * KG_DMA>();
+    kstep(yes(), A1, B1, A0, B0, nxt, 0, s + KG_NBUF);
+  }
+  // DMAs of stages past the block (zero-filled) may still be in flight
+  rgd_barrier<0>();
+  kg_finish(J, T, are, aim, csum);
+}
+
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
+void xg_xengine_kg(XgLaunch L) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[KG_NBUF * KG_BUF];
+  const int xcd = blockIdx.x % XG_NXCD;
+  const int wl = blockIdx.x / XG_NXCD;
+  TileRef T;
+  for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
+    kg_tile(L.jobs[T.job], T, lds);
     rg_barrier();  // every wave is done with this tile's LDS buffers
   }
 }
@@ -1574,6 +1934,9 @@ extern "C" int xgh_launch_xengine(int family, const XgLaunch *L, void *stream) {
         hipLaunchKernelGGL(xg_xengine_rg<128>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       else
         hipLaunchKernelGGL(xg_xengine_rg<64>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      break;
+    case XG_KERNEL_I8_KG:
+      hipLaunchKernelGGL(xg_xengine_kg, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       break;
     case XG_KERNEL_F16_RG:
       if (L->stage == 1)

@@ -10,8 +10,9 @@
  *
  * Work unit: a tile = (job, channel f, row block rb, column block cb) with
  * rb >= cb, where a block is XG_TILE consecutive inputs (input = 2*station +
- * pol).  Tiles are assigned to the 8 XCDs by channel (all tiles of a channel
- * on one XCD, so the channel's input is fetched from HBM into one L2), and
+ * pol); the K-interleaved kernel takes pair tiles instead (XgJob.pair_tiles).
+ * Tiles are assigned to the 8 XCDs by channel (all tiles of a channel on one
+ * XCD, so the channel's input is fetched from HBM into one L2), and
  * each XCD's tiles are dealt round-robin to the persistent workgroups that the
  * hardware places on that XCD (workgroup w runs on XCD w % 8).
  */
@@ -42,6 +43,8 @@
 #define XG_KERNEL_F32_PLAIN 2
 #define XG_KERNEL_I8_RG     3  /* int8 real-Gram MFMA (nstation % 4 == 0) */
 #define XG_KERNEL_F16_RG    4  /* fp16 real-Gram MFMA (16-byte aligned input) */
+#define XG_KERNEL_I8_KG     5  /* int8 MFMA with (re, im) interleaved along K, pair tiles
+                                  (nstation % 4 == 0, 16-byte aligned input) */
 
 typedef struct XgJob {
   const void *input;   /* sample (t=0,f=0,s=0,p=0) of this block */
@@ -52,12 +55,14 @@ typedef struct XgJob {
   int nfreq;
   int ntime;           /* samples in this block */
   int nblk;            /* ceil(2*nstation / XG_TILE) */
-  int tiles_per_chan;  /* nblk*(nblk+1)/2 */
+  int tiles_per_chan;  /* nblk*(nblk+1)/2 (square tiles) */
   int order;           /* XG_ORDER_* */
   int accumulate;      /* 0: store, 1: add into out */
   int out_kind;        /* XG_OUT_* */
   int chan_base;       /* global channel number of f=0 (XCD assignment) */
-  int reserved;
+  int pair_tiles;      /* 0: square tiles (rb, cb), rb >= cb;  1: pair tiles (rb, cp) of
+                          XG_TILE row inputs x 2 XG_TILE column inputs, 2 cp <= rb,
+                          tiles_per_chan = xg_pair_count(nblk) */
 } XgJob;
 
 typedef struct XgLaunch {
