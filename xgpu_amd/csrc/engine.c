@@ -318,6 +318,8 @@ int xg_spill(XGPUAmdEngine *e) {
  * 2^31. */
 int xg_launch_family(const XGPUAmdEngine *e, const void *input_d) {
   if (e->family == XG_KERNEL_F32_PLAIN || e->kernel_pref == 1) return e->family;
+  /* these kernels store 32-bit output only (64-bit dumps come from the spill kernels) */
+  if (e->out_kind != XG_OUT_I32 && e->out_kind != XG_OUT_F32) return e->family;
   if ((uintptr_t)input_d & 15) return e->family;
   if (e->family == XG_KERNEL_I8_MFMA && e->cfg.nstation % 4) return e->family;
   if (128LL * e->tstride >= (1LL << 31) - 4096) return e->family;

@@ -670,13 +670,11 @@ template <>
 struct RgElem<v16i> {
   typedef int T;       // accumulator element
   typedef unsigned S;  // 32-bit output element
-  static constexpr int kOut32 = XG_OUT_I32;
 };
 template <>
 struct RgElem<v16f> {
   typedef float T;
   typedef float S;
-  static constexpr int kOut32 = XG_OUT_F32;
 };
 
 __device__ __forceinline__ int rg_add(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
@@ -714,27 +712,25 @@ __device__ __forceinline__ void rg_combine(const Acc &blk, int q, bool odd, cons
   }
 }
 
-__device__ __forceinline__ void store_vis(const XgJob &J, int f, int m, int n, int re, int im) {
-  store_int(J, f, m, n, re, im);
-}
-__device__ __forceinline__ void store_vis(const XgJob &J, int f, int m, int n, float re, float im) {
-  store_flt(J, f, m, n, re, im);
-}
 
 // 32-bit (int32 or float) stores of the three output orders, one whole
-// cell per lane.  A cell is a station pair (i, j) with its four polarization
-// products (element 2 p1 + p2): 16 bytes per plane in register-tile and
-// real/imag order, 32 bytes of interleaved (re, im) in triangular order.  The
-// four lanes of a quad (p1 = l & 1, p2 = (l >> 1) & 1) hold one product each
-// of the cells of column blocks b = 0..3 (station j = j0 + 8 b); a
-// wave-private LDS transpose (ds_write_b32, ds_read_b128) gives lane l the
-// whole cell of block b = l & 3.  Cells of consecutive j are consecutive in
-// every order, so each store instruction writes runs of 16 (register tile)
-// or 32 cells instead of 4-byte pieces.  Index formulas as in vis_index
-// (reference src/kernel.cuh:49-117).
-#define RG_CELL_GROUP 144                     // LDS bytes per quad group: 4 x 32 + 16 padding
+// cell per lane (the engine gives these kernels 32-bit output only; 64-bit
+// dumps come from the spill / combine kernels).  A cell is a station pair
+// (i, j) with its four polarization products (element 2 p1 + p2): 16 bytes
+// per plane in register-tile and real/imag order, 32 bytes of interleaved
+// (re, im) in triangular order.  The four lanes of a quad (p1 = l & 1,
+// p2 = (l >> 1) & 1) hold one product each of the cells of column blocks
+// b = 0..3 (station j = j0 + 8 b); a wave-private LDS transpose (ds_write_b32,
+// ds_read_b128 per plane) gives lane l the whole cell of block b = l & 3;
+// triangular order interleaves the two planes in registers.  One code path
+// can serve the three orders (ORDER = 0).  Cells of consecutive j are
+// consecutive in every order, so each store instruction writes runs of 16
+// (register tile) or 32 cells instead of 4-byte pieces.  Index formulas as in vis_index (reference
+// src/kernel.cuh:49-117).
+#define RG_CELL_GROUP 144                     // LDS bytes per quad group: 2 x 64 + 16 padding
 #define RG_CELL_SCRATCH (16 * RG_CELL_GROUP)  // per wave (16 quad groups)
 
+// ORDER: an XG_ORDER_* constant, or 0 to read the order from the job.
 template <int ORDER, class Acc, class SW>
 __device__ __forceinline__ void rg_store_cells(const XgJob &J, const TileRef &T,
                                                Acc (&acc)[4][4], int wr, int wc, int lane,
@@ -742,7 +738,8 @@ __device__ __forceinline__ void rg_store_cells(const XgJob &J, const TileRef &T,
   typedef typename RgElem<Acc>::T E;
   typedef typename RgElem<Acc>::S S;
   typedef S S4 __attribute__((ext_vector_type(4)));
-  constexpr bool TRI = ORDER == XG_ORDER_TRI;
+  const int order = ORDER ? ORDER : J.order;
+  const bool tri = order == XG_ORDER_TRI, regt = order == XG_ORDER_REGTILE;
   const bool odd = lane & 1;
   const int hi = lane >> 5, mj = (lane & 31) >> 2;   // station offsets: i = i0 + hi, j = j0 + mj
   const int e = 2 * (lane & 1) + ((lane >> 1) & 1);  // this lane's product: 2 p1 + p2
@@ -757,12 +754,12 @@ __device__ __forceinline__ void rg_store_cells(const XgJob &J, const TileRef &T,
   const bool diag = T.rb == T.cb;
   const bool full = !diag && (T.rb * 64 + 64 <= ns);  // every (i, j) valid, i > j
   S *base;
-  if (ORDER == XG_ORDER_REGTILE) {  // cell (R, C) of quadrant (i & 1, j & 1) of channel f
+  if (regt) {  // cell (R, C) of quadrant (i & 1, j & 1) of channel f
     const long long RT = (long long)(ns >> 1) * ((ns >> 1) + 1) / 2;
     base = (S *)J.out + 4 * (((long long)T.f * 4 + 2 * (j & 1) + hi) * RT + (j >> 1));
-  } else {                          // cell i(i+1)/2 + j of channel f
+  } else {     // cell i(i+1)/2 + j of channel f
     const long long NB = (long long)ns * (ns + 1) / 2;
-    base = (S *)J.out + (TRI ? 8 : 4) * ((long long)T.f * NB + j);
+    base = (S *)J.out + (tri ? 8 : 4) * ((long long)T.f * NB + j);
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -772,22 +769,18 @@ __device__ __forceinline__ void rg_store_cells(const XgJob &J, const TileRef &T,
       E re[4], im[4];
 #pragma unroll
       for (int b = 0; b < 4; ++b) rg_combine(acc[a][b], q, odd, swap, re[b], im[b]);
-      S4 v0, v1;  // TRI: the cell's 8 values; else re and im planes
-      if constexpr (TRI) {
+      // the cell's re and im planes through LDS
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          *(S *)(scr + 32 * b + 8 * e) = (S)re[b];
-          *(S *)(scr + 32 * b + 8 * e + 4) = (S)im[b];
-        }
-        v0 = *(const S4 *)(scr + 32 * bq);
-        v1 = *(const S4 *)(scr + 32 * bq + 16);
-      } else {
-#pragma unroll
-        for (int b = 0; b < 4; ++b) *(S *)(scr + 16 * b + 4 * e) = (S)re[b];
-        v0 = *(const S4 *)(scr + 16 * bq);
-#pragma unroll
-        for (int b = 0; b < 4; ++b) *(S *)(scr + 16 * b + 4 * e) = (S)im[b];
-        v1 = *(const S4 *)(scr + 16 * bq);
+      for (int b = 0; b < 4; ++b) {
+        *(S *)(scr + 16 * b + 4 * e) = (S)re[b];
+        *(S *)(scr + 64 + 16 * b + 4 * e) = (S)im[b];
+      }
+      const S4 cre = *(const S4 *)(scr + 16 * bq);
+      const S4 cim = *(const S4 *)(scr + 64 + 16 * bq);
+      S4 v0 = cre, v1 = cim;
+      if (tri) {  // triangular order interleaves (re, im) by product
+        v0 = S4{cre[0], cim[0], cre[1], cim[1]};
+        v1 = S4{cre[2], cim[2], cre[3], cim[3]};
       }
       const int i = T.rb * 64 + 32 * wr + hi + 8 * a + 2 * q;
       if (!full) {
@@ -795,64 +788,51 @@ __device__ __forceinline__ void rg_store_cells(const XgJob &J, const TileRef &T,
         if (i < j) {
           // quadrant "12" of a diagonal register-tile cell is never written by
           // the reference (kernel.cuh:65-71): zero after a clear
-          if (ORDER != XG_ORDER_REGTILE || (i >> 1) != (j >> 1) || accum) continue;
+          if (!regt || (i >> 1) != (j >> 1) || accum) continue;
           v0 = (S4)(S)0;
           v1 = (S4)(S)0;
         }
       }
       S *p, *p1;
-      if (ORDER == XG_ORDER_REGTILE) {
+      if (regt) {
         const long long R = i >> 1;
         p = base + 4 * (R * (R + 1) / 2);
         p1 = p + plane;
       } else {
         const long long rowi = (long long)i * (i + 1) / 2;
-        p = base + (TRI ? 8 : 4) * rowi;
-        p1 = TRI ? p + 4 : p + plane;
+        p = base + (tri ? 8 : 4) * rowi;
+        p1 = tri ? p + 4 : p + plane;
       }
       if (accum) {
         v0 += *(const S4 *)p;
         v1 += *(const S4 *)p1;
-        *(S4 *)p = v0;
-        *(S4 *)p1 = v1;
-      } else {
-        // streaming stores: the tile's output is not read again by this
-        // launch and should not evict the input the next round reuses
-        __builtin_nontemporal_store(v0, (S4 *)p);
-        __builtin_nontemporal_store(v1, (S4 *)p1);
       }
+      // streaming stores: the tile's output is not read again by this
+      // launch and should not evict the input the next round reuses
+      __builtin_nontemporal_store(v0, (S4 *)p);
+      __builtin_nontemporal_store(v1, (S4 *)p1);
     }
 }
 
-template <class Acc, class SW>
-__device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, Acc (&acc)[4][4],
+// int32 output: one code path for the three orders (no per-order copies of
+// the accumulator reads for the compiler to hoist and spill); float output:
+// one copy per order, which measures faster for the fp16 kernels.
+template <class SW>
+__device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, v16i (&acc)[4][4],
                                             int wr, int wc, int lane, const SW &swap,
                                             unsigned char *lds) {
-  if (J.out_kind == RgElem<Acc>::kOut32) {
-    if (J.order == XG_ORDER_REGTILE)
-      rg_store_cells<XG_ORDER_REGTILE>(J, T, acc, wr, wc, lane, swap, lds);
-    else if (J.order == XG_ORDER_TRI)
-      rg_store_cells<XG_ORDER_TRI>(J, T, acc, wr, wc, lane, swap, lds);
-    else
-      rg_store_cells<XG_ORDER_REALIMAG>(J, T, acc, wr, wc, lane, swap, lds);
-    return;
-  }
-  const bool odd = lane & 1;  // int64 / double output: generic path
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        typename RgElem<Acc>::T re, im;
-        rg_combine(acc[a][b], q, odd, swap, re, im);
-        const int R0 = 128 * wr + 32 * a + 8 * q + 4 * (lane >> 5);  // even real row
-        const int m = T.rb * XG_TILE + (R0 >> 1) + (odd ? 1 : 0);
-        const int n = T.cb * XG_TILE + ((128 * wc + 32 * b + (lane & 31)) >> 1);
-        store_vis(J, T.f, m, n, re, im);
-      }
-    }
+  rg_store_cells<0>(J, T, acc, wr, wc, lane, swap, lds);
+}
+template <class SW>
+__device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, v16f (&acc)[4][4],
+                                            int wr, int wc, int lane, const SW &swap,
+                                            unsigned char *lds) {
+  if (J.order == XG_ORDER_REGTILE)
+    rg_store_cells<XG_ORDER_REGTILE>(J, T, acc, wr, wc, lane, swap, lds);
+  else if (J.order == XG_ORDER_TRI)
+    rg_store_cells<XG_ORDER_TRI>(J, T, acc, wr, wc, lane, swap, lds);
+  else
+    rg_store_cells<XG_ORDER_REALIMAG>(J, T, acc, wr, wc, lane, swap, lds);
 }
 
 // Output stage of a tile.  The lane index reaches the epilogue through an
