@@ -1016,6 +1016,21 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, const 
   lwrite(rb0, rb1, 1);
   G.gload(rb0, rb1, 3);
   rg_barrier();
+  if (diag && wr < wc) {
+    // On a diagonal tile this wave's block is the mirror of wave (1, 0)'s and
+    // is never stored: it does its share of the staging (the same writes,
+    // loads and stage barriers, in the same order) and no operand reads or
+    // MFMAs, which the chip's power budget gives to the other waves.
+    for (int s = 0; s < nstage; s += 2) {
+      rg_barrier();
+      lwrite(ra0, ra1, 0);
+      G.gload(ra0, ra1, s + 4);
+      rg_barrier();
+      lwrite(rb0, rb1, 1);
+      G.gload(rb0, rb1, s + 5);
+    }
+    return;  // nothing to store (rg_finish skips this wave on a diagonal tile)
+  }
   lread(lds, A0, B0);
   auto stage = [&](int s, const unsigned char *cur, const unsigned char *nxt, v4i *r0, v4i *r1,
                    int buf) {
@@ -1069,8 +1084,6 @@ __device__ __forceinline__ void rg_run(const XgLaunch &L, unsigned char *lds) {
       G.gload(ra0, ra1, 0);
       G.gload(rb0, rb1, 1);
     }
-    // (a diagonal tile's wave above the diagonal runs the same MFMAs as its
-    // mirror: one wave per SIMD, so skipping them would save no time)
     rg_barrier();  // the last stage's writes into the idle buffer are done
     rg_finish(L.jobs[T.job], T, acc, lds);
     rg_barrier();  // every wave is done with this tile's LDS buffers
