@@ -448,6 +448,15 @@ def main():
     # on-board MFMA-only ceiling and the clocks the run saw
     peak_meas = X.mfma_peak() / 1e12
     log(f"[bench] MFMA-only int8 probe: {peak_meas:.1f} TCMAC/s")
+    # The MFMA-only ceiling with operands drawn like the headline input
+    # (uniform int8 over the full range): the chip's power management sets the
+    # clock by what the MFMA datapath toggles, so this, not the dense peak, is
+    # what the kernel can reach on this input.
+    ops = X.DeviceBuffer(X.PEAK_OPERAND_BYTES)
+    ops.fill_uniform_int8(5, -128, 127)
+    ceil_uniform = X.mfma_peak_data(ops) / 1e12
+    ops.free()
+    log(f"[bench] MFMA-only int8 ceiling, uniform operands: {ceil_uniform:.1f} TCMAC/s")
     clocks = None
     try:
         r = subprocess.run(["rocm-smi", "--showclocks"], capture_output=True, text=True, timeout=30)
@@ -490,6 +499,9 @@ def main():
         "kernel_ms_median": round(ms_med, 4),
         "kernel_ms_p99": round(ms_p99, 4),
         "mfma_only_probe_tcmac_s": round(peak_meas, 1),
+        "mfma_only_ceiling_uniform_operands_tcmac_s": round(ceil_uniform, 1),
+        "pct_of_mfma_ceiling_uniform_operands": round(100.0 * cmac / (ms_med * 1e-3) / 1e12
+                                                      / ceil_uniform, 1),
         "pct_of_mfma_only_probe": round(100.0 * cmac / (ms_med * 1e-3) / 1e12 / peak_meas, 1),
         "clocks": clocks,
         "fp32_plain": {"tcmac_s": round(t32, 3), "ms": round(ms32, 3),

@@ -201,6 +201,15 @@ def test_engine_without_device_reports_no_device():
     assert "code 7" in r.stdout, r.stdout + r.stderr
 
 
+def test_mfma_peak_data_rejects_bad_arguments():
+    """Argument checks run before any device call (no GPU needed)."""
+    lib = X.core()
+    v = C.c_double()
+    assert lib.xgpuAmdMfmaPeakData(0, None, 3, C.byref(v)) == 6     # XGPU_INVALID_CONFIG
+    assert lib.xgpuAmdMfmaPeakData(0, 4096, 0, C.byref(v)) == 6
+    assert lib.xgpuAmdMfmaPeakData(0, 4096, 3, None) == 6
+
+
 # ---------------------------------------------------------------- CPU functions of xgpu.h
 libc = C.CDLL("libc.so.6")
 

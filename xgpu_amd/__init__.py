@@ -214,6 +214,23 @@ def mfma_peak(device: int = 0, reps: int = 3) -> float:
     return float(v.value)
 
 
+PEAK_OPERAND_BYTES = 65536
+
+
+def mfma_peak_data(operands, device: int = 0, reps: int = 3) -> float:
+    """MFMA-only int8 throughput (complex MACs per second) with operands taken
+    from `operands` (a DeviceBuffer or device pointer holding PEAK_OPERAND_BYTES
+    of int8 values): the MFMA ceiling for input of that distribution."""
+    if isinstance(operands, DeviceBuffer):
+        assert operands.nbytes >= PEAK_OPERAND_BYTES
+        ptr = operands.ptr
+    else:
+        ptr = int(operands)
+    v = C.c_double()
+    check(core().xgpuAmdMfmaPeakData(device, ptr, reps, C.byref(v)))
+    return float(v.value)
+
+
 def correlate_batch(items, stream=None) -> None:
     """Ragged batch in one persistent launch: items = [(engine, input_d, ntime)]."""
     arr = (_lib.XGPUAmdBatchItem * len(items))()
