@@ -443,6 +443,8 @@ def main():
     ms16 = e16.time_correlate(b16, nt16, warmup=100, reps=20)  # default: real-Gram rgh<32>
     e16.set_tuning(kernel=1)
     ms16_cp = e16.time_correlate(b16, nt16, warmup=3, reps=20)  # complex-plane FP16 kernel
+    # FP16 MFMA-only ceiling with operands drawn like this input
+    ceil16 = X.mfma_peak_data(b16, compute=X.XGPU_FLOAT16) / 1e12
     e16.close()
     b16.free()
     cmac16 = X.cmac_per_integration(ns16, nf16, nt16)
@@ -520,7 +522,9 @@ def main():
                                   "(complex Gaussian sigma 16, tiled from 64 MiB)",
                       "kernel": "xg_xengine_rgh<32>", "tcmac_s": round(t16, 2),
                       "ms": round(ms16, 4), "complex_plane_kernel_ms": round(ms16_cp, 4),
-                      "pct_fp16_peak": round(100 * t16 * 8 / PEAK_FP16_TFLOPS, 1)},
+                      "pct_fp16_peak": round(100 * t16 * 8 / PEAK_FP16_TFLOPS, 1),
+                      "mfma_only_ceiling_same_input_tcmac_s": round(ceil16, 1),
+                      "pct_of_mfma_ceiling_same_input": round(100 * t16 / ceil16, 1)},
         "roofline": {
             "bound": "mfma",
             "achieved": round(achieved_tops, 1),

@@ -133,15 +133,17 @@ int xgpuAmdFillUniformInt8(void *ptr_d, size_t nbytes, unsigned long long seed, 
  * with.  Best of `reps` launches after a warm-up. */
 int xgpuAmdMfmaPeak(int device, int reps, double *cmac_per_s);
 /* The same MFMA-only stream with operands taken from `operands_d`, 65536
- * bytes of int8 values in device memory on `device` (fill them with the
- * input distribution of interest).  The operands change every instruction,
+ * bytes in device memory on `device` (fill them with the input distribution
+ * of interest): int8 values for compute_type XGPU_INT8 (int8 MFMA), fp16
+ * values for XGPU_FLOAT16 (fp16 MFMA with fp32 accumulation).  The operands change every instruction,
  * and the chip's power management sets the clock by what the datapath
  * toggles.  So this is the MFMA-only ceiling for input of that
  * distribution: the X-engine's rate on such input is compared with it.  One
  * workgroup per CU and one wave per SIMD, as the X-engine.  Best of `reps`
  * launches after a warm-up.  No counterpart in the reference. */
 #define XGPU_PEAK_OPERAND_BYTES 65536
-int xgpuAmdMfmaPeakData(int device, const void *operands_d, int reps, double *cmac_per_s);
+int xgpuAmdMfmaPeakData(int device, const void *operands_d, unsigned int compute_type, int reps,
+                        double *cmac_per_s);
 
 /* ---- measurement -------------------------------------------------------- */
 /* Run xgpuAmdCorrelate(engine, input_d, ntime) warmup + reps times on the

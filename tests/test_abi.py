@@ -205,9 +205,10 @@ def test_mfma_peak_data_rejects_bad_arguments():
     """Argument checks run before any device call (no GPU needed)."""
     lib = X.core()
     v = C.c_double()
-    assert lib.xgpuAmdMfmaPeakData(0, None, 3, C.byref(v)) == 6     # XGPU_INVALID_CONFIG
-    assert lib.xgpuAmdMfmaPeakData(0, 4096, 0, C.byref(v)) == 6
-    assert lib.xgpuAmdMfmaPeakData(0, 4096, 3, None) == 6
+    assert lib.xgpuAmdMfmaPeakData(0, None, 0, 3, C.byref(v)) == 6     # XGPU_INVALID_CONFIG
+    assert lib.xgpuAmdMfmaPeakData(0, 4096, 0, 0, C.byref(v)) == 6
+    assert lib.xgpuAmdMfmaPeakData(0, 4096, 0, 3, None) == 6
+    assert lib.xgpuAmdMfmaPeakData(0, 4096, 1, 3, C.byref(v)) == 6     # FP32: no MFMA probe
 
 
 # ---------------------------------------------------------------- CPU functions of xgpu.h

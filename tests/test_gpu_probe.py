@@ -27,3 +27,14 @@ def test_mfma_probes_report_rates_below_the_dense_peak():
     ops.free()
     assert 0.1 * DENSE_PEAK_CMAC_S < uni < 1.05 * DENSE_PEAK_CMAC_S
     assert 0.1 * DENSE_PEAK_CMAC_S < three < 1.05 * DENSE_PEAK_CMAC_S
+
+
+def test_fp16_mfma_ceiling_probe():
+    """fp16 operands (Gaussian, sigma 16) on the fp16 MFMA: a positive rate
+    below the dense FP16 peak (2.5 PFLOP/s = 312.5 T complex MACs/s)."""
+    g = (np.random.default_rng(3).standard_normal(X.PEAK_OPERAND_BYTES // 2) * 16).astype(np.float16)
+    ops = X.DeviceBuffer.from_array(g)
+    flush("[probe] fp16 mfma peak, Gaussian operands")
+    rate = X.mfma_peak_data(ops, reps=2, compute=X.XGPU_FLOAT16)
+    ops.free()
+    assert 0.1 * 312.5e12 < rate < 1.05 * 312.5e12

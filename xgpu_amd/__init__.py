@@ -217,17 +217,18 @@ def mfma_peak(device: int = 0, reps: int = 3) -> float:
 PEAK_OPERAND_BYTES = 65536
 
 
-def mfma_peak_data(operands, device: int = 0, reps: int = 3) -> float:
-    """MFMA-only int8 throughput (complex MACs per second) with operands taken
-    from `operands` (a DeviceBuffer or device pointer holding PEAK_OPERAND_BYTES
-    of int8 values): the MFMA ceiling for input of that distribution."""
+def mfma_peak_data(operands, device: int = 0, reps: int = 3, compute: int = XGPU_INT8) -> float:
+    """MFMA-only throughput (complex MACs per second) with operands taken from
+    `operands` (a DeviceBuffer or device pointer holding PEAK_OPERAND_BYTES of
+    int8 values, or fp16 values with compute=XGPU_FLOAT16): the MFMA ceiling
+    for input of that distribution."""
     if isinstance(operands, DeviceBuffer):
         assert operands.nbytes >= PEAK_OPERAND_BYTES
         ptr = operands.ptr
     else:
         ptr = int(operands)
     v = C.c_double()
-    check(core().xgpuAmdMfmaPeakData(device, ptr, reps, C.byref(v)))
+    check(core().xgpuAmdMfmaPeakData(device, ptr, compute, reps, C.byref(v)))
     return float(v.value)
 
 

@@ -146,7 +146,7 @@ def core() -> C.CDLL:
         _sig(lib, "xgpuAmdFillUniformInt8", I, P, SZ, ULL, I, I)
         _sig(lib, "xgpuAmdTimeCorrelate", I, P, P, U, I, I, C.POINTER(C.c_float))
         _sig(lib, "xgpuAmdMfmaPeak", I, I, I, C.POINTER(C.c_double))
-        _sig(lib, "xgpuAmdMfmaPeakData", I, I, P, I, C.POINTER(C.c_double))
+        _sig(lib, "xgpuAmdMfmaPeakData", I, I, P, U, I, C.POINTER(C.c_double))
         _sig(lib, "xgpuAmdSetTuning", I, P, C.POINTER(XGPUAmdTuning))
         _sig(lib, "xgpuAmdGetTuning", I, P, C.POINTER(XGPUAmdTuning))
         _sig(lib, "xgpuAmdAutotune", I, P, P, U, I, I, C.c_char_p, C.POINTER(XGPUAmdTuning),

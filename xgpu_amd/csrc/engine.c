@@ -718,8 +718,12 @@ int xgpuAmdTimeCorrelate(XGPUAmdEngine *e, const void *input_d, unsigned int nti
 /* MFMA-only throughput of the device (int8 32x32x32, 8 chains per wave, 8
  * workgroups of 4 waves per CU), in complex MACs per second (4 int8 MACs
  * each), best of `reps` timed launches after one warm-up. */
-int xgpuAmdMfmaPeakData(int device, const void *operands_d, int reps, double *cmac_per_s) {
+int xgpuAmdMfmaPeakData(int device, const void *operands_d, unsigned compute_type, int reps,
+                        double *cmac_per_s) {
   if (!cmac_per_s || !operands_d || reps < 1) return xg_fail(XGPU_INVALID_CONFIG, "bad arguments");
+  if (compute_type != XGPU_INT8 && compute_type != XGPU_FLOAT16)
+    return xg_fail(XGPU_INVALID_CONFIG, "compute_type must be XGPU_INT8 or XGPU_FLOAT16");
+  const int f16 = compute_type == XGPU_FLOAT16;
   HIPCALL(xgh_set_device(device), "set device");
   int cus = 0;
   HIPCALL(xgh_device_cu_count(device, &cus), "CU count");
@@ -735,15 +739,15 @@ int xgpuAmdMfmaPeakData(int device, const void *operands_d, int reps, double *cm
   for (int r = 0; r <= reps; ++r) {
     float ms = 0.f;
     if (xgh_event_record(e0, stream) ||
-        xgh_launch_mfma_peak_data(grid, iters, operands_d, (int *)out, stream) ||
+        xgh_launch_mfma_peak_data(f16, grid, iters, operands_d, (int *)out, stream) ||
         xgh_event_record(e1, stream) || xgh_event_sync(e1) || xgh_event_elapsed_ms(&ms, e0, e1)) {
       rc = hipfail("mfma peak");
       goto done;
     }
     if (r > 0 && (best == 0.f || ms < best)) best = ms;
   }
-  /* MACs: grid * 4 waves * iters * 8 MFMAs * 32768 */
-  *cmac_per_s = (double)grid * 4 * iters * 8 * 32768.0 / 4.0 / (best * 1e-3);
+  /* MACs: grid * 4 waves * iters * 8 MFMAs * (32768 int8 / 16384 fp16) */
+  *cmac_per_s = (double)grid * 4 * iters * 8 * (f16 ? 16384.0 : 32768.0) / 4.0 / (best * 1e-3);
 done:
   xgh_event_destroy(e0);
   xgh_event_destroy(e1);

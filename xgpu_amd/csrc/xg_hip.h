@@ -84,7 +84,8 @@ int xgh_launch_reorder(int src_order, int elem_bytes, void *dst, const void *src
 /* MFMA-only probe: grid workgroups of 256 threads, each wave `iters` x 8
  * independent int8 32x32x32 MFMAs; out holds grid*256 ints */
 int xgh_launch_mfma_peak(int grid, int iters, int *out, void *stream);
-int xgh_launch_mfma_peak_data(int grid, int iters, const void *ops, int *out, void *stream);
+int xgh_launch_mfma_peak_data(int f16, int grid, int iters, const void *ops, int *out,
+                              void *stream);
 /* device corner-turn to the depth-4 swizzled layout of the dp4a input
  * format: [t/4][f][s][p][c][t%4] */
 int xgh_launch_swizzle(void *dst, const void *src, int nstation, int nfreq,

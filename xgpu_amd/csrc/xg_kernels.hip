@@ -2090,6 +2090,8 @@ __global__ __launch_bounds__(256) void xg_mfma_peak_k(int iters, int *out) {
 // instruction.  One workgroup per CU, one wave per SIMD, as the X-engine.
 // The chip's power management sets the clock by what the datapath toggles,
 // so this is the MFMA-only ceiling for input of that distribution.
+// F16: v_mfma_f32_32x32x16_f16 on the same 16 bytes per lane (8 halves).
+template <bool F16>
 __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
 void xg_mfma_peak_data_k(const v4i *ops, int iters, int *out) {
   const int t = threadIdx.x;
@@ -2099,32 +2101,58 @@ void xg_mfma_peak_data_k(const v4i *ops, int iters, int *out) {
     A[j] = ops[j * 256 + t];
     B[j] = ops[(8 + j) * 256 + t];
   }
-  v16i c[8];
+  if constexpr (F16) {
+    v16f c[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j)
+    for (int j = 0; j < 8; ++j)
 #pragma unroll
-    for (int e = 0; e < 16; ++e) c[j][e] = 0;
-  for (int i = 0; i < iters; i += 8) {
+      for (int e = 0; e < 16; ++e) c[j][e] = 0.f;
+    for (int i = 0; i < iters; i += 8) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r)
+      for (int r = 0; r < 8; ++r)
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
-        c[j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[(r + j) & 7], B[j], c[j], 0, 0, 0);
+        for (int j = 0; j < 8; ++j)
+          c[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(v8h, A[(r + j) & 7]),
+                                                        __builtin_bit_cast(v8h, B[j]), c[j], 0, 0, 0);
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) s += c[j][e];
+    out[(size_t)blockIdx.x * 256 + t] = __float_as_int(s);
+  } else {
+    v16i c[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) c[j][e] = 0;
+    for (int i = 0; i < iters; i += 8) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          c[j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[(r + j) & 7], B[j], c[j], 0, 0, 0);
+    }
+    int s = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) s += c[j][e];
+    out[(size_t)blockIdx.x * 256 + t] = s;
   }
-  int s = 0;
-#pragma unroll
-  for (int j = 0; j < 8; ++j)
-#pragma unroll
-    for (int e = 0; e < 16; ++e) s += c[j][e];
-  out[(size_t)blockIdx.x * 256 + t] = s;
 }
 
 // iters: a multiple of 8; ops: XG_PEAK_OPERAND_BYTES of device memory
-extern "C" int xgh_launch_mfma_peak_data(int grid, int iters, const void *ops, int *out,
+extern "C" int xgh_launch_mfma_peak_data(int f16, int grid, int iters, const void *ops, int *out,
                                          void *stream) {
   if (grid <= 0 || iters <= 0) return 0;
-  hipLaunchKernelGGL(xg_mfma_peak_data_k, dim3(grid), dim3(256), 0, (hipStream_t)stream,
-                     (const v4i *)ops, iters, out);
+  if (f16)
+    hipLaunchKernelGGL(xg_mfma_peak_data_k<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream,
+                       (const v4i *)ops, iters, out);
+  else
+    hipLaunchKernelGGL(xg_mfma_peak_data_k<false>, dim3(grid), dim3(256), 0, (hipStream_t)stream,
+                       (const v4i *)ops, iters, out);
   return xgh_report(hipGetLastError(), "xg_mfma_peak_data_k launch");
 }
 
