@@ -166,7 +166,8 @@ typedef struct XGPUAmdTuningStruct {
                       register staging, 32 or 64 with DMA staging; fp16 32 or 64
                       (0 = default) */
   int staging;     /* int8 real-Gram kernel: 1 = global -> VGPR -> LDS, 2 = global -> LDS
-                      DMA (0 = default); fp16: 0 or 1 (register staging only) */
+                      DMA (0 = default); fp16: 1 = registers (default), 2 = DMA
+                      (stage 32) */
   int reserved[2];
 } XGPUAmdTuning;
 int xgpuAmdSetTuning(XGPUAmdEngine *engine, const XGPUAmdTuning *tuning);

@@ -269,9 +269,9 @@ def test_f32_cfg1_through_xgpu_h():
 
 
 # ------------------------------------------------------------------ FP16 MFMA
-# (kernel, stage): complex-plane kernel; real-Gram kernel with 32 / 64 samples
-# per LDS stage
-F16_VARIANTS = [(1, 0), (2, 32), (2, 64)]
+# (kernel, staging, stage): complex-plane kernel; real-Gram kernel with
+# register staging (32 / 64 samples per LDS stage) and with global->LDS DMA (32)
+F16_VARIANTS = [(1, 0, 0), (2, 1, 32), (2, 1, 64), (2, 2, 32)]
 
 
 def f16_input(nt, nf, ns, seed, sigma=16.0):
@@ -334,10 +334,11 @@ def run_f16(ns, nf, nt, x16, order, calls=1, **tuning):
     (132, 2, 200, "regtile", 2),    # 3 tiles per side, accumulate, ragged edge
     (130, 1, 64, "realimag", 1),    # nstation % 4 == 2
 ])
-@pytest.mark.parametrize("kernel,stage", F16_VARIANTS)
-def test_f16_against_f64_oracle(ns, nf, nt, order, calls, kernel, stage):
+@pytest.mark.parametrize("kernel,staging,stage", F16_VARIANTS)
+def test_f16_against_f64_oracle(ns, nf, nt, order, calls, kernel, staging, stage):
     x16 = f16_input(nt, nf, ns, 4 + ns + nt)
-    got, wide = run_f16(ns, nf, nt, x16, ORDERS[order], calls, kernel=kernel, stage=stage)
+    got, wide = run_f16(ns, nf, nt, x16, ORDERS[order], calls, kernel=kernel, staging=staging,
+                        stage=stage)
     ref = oracle.xengine_f64_f16(x16, ns, nf, nt)
     assert_f16_close(got, ref, ns, nf, nt)
     assert np.array_equal(wide, got.astype(np.float64))
@@ -345,16 +346,17 @@ def test_f16_against_f64_oracle(ns, nf, nt, order, calls, kernel, stage):
 
 def test_f16_variants_agree_cfg2_shape():
     """NSTATION=512 x 2 pol, NFREQUENCY=64 over 1024 samples: the real-Gram
-    kernels (both stage depths) agree with each other bit for bit (same
-    MFMA sequence per accumulator) and with the complex-plane kernel within
-    the fp32 accumulation bound."""
+    kernels (both stage depths, both stagings) agree with each other bit for
+    bit (same MFMA sequence per accumulator) and with the complex-plane kernel
+    within the fp32 accumulation bound."""
     ns, nf, nt = 512, 64, 1024
     x16 = f16_input(nt, nf, ns, 11)
     outs = {}
-    for kernel, stage in F16_VARIANTS:
-        outs[(kernel, stage)], _ = run_f16(ns, nf, nt, x16, X.TRIANGULAR_ORDER, 1,
-                                           kernel=kernel, stage=stage)
-    assert np.array_equal(outs[(2, 32)], outs[(2, 64)])
+    for kernel, staging, stage in F16_VARIANTS:
+        outs[(kernel, staging, stage)], _ = run_f16(ns, nf, nt, x16, X.TRIANGULAR_ORDER, 1,
+                                                    kernel=kernel, staging=staging, stage=stage)
+    assert np.array_equal(outs[(2, 1, 32)], outs[(2, 1, 64)])
+    assert np.array_equal(outs[(2, 1, 32)], outs[(2, 2, 32)])
     chans = [0, 63]
     ref = oracle.xengine_f64_f16(np.ascontiguousarray(x16[:, chans]), ns, len(chans), nt)
     for v, got in outs.items():

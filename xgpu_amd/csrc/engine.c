@@ -260,12 +260,14 @@ int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
   if (t->staging < 0 || t->staging > 2) return xg_fail(XGPU_INVALID_CONFIG, "staging must be 0..2");
   int staging, stage;
   if (e->family == XG_KERNEL_F16_MFMA) {
-    /* the fp16 real-Gram kernel stages through registers only */
-    if (t->staging == 2) return xg_fail(XGPU_INVALID_CONFIG, "fp16 engines have register staging only");
-    if (t->stage != 0 && t->stage != 32 && t->stage != 64)
-      return xg_fail(XGPU_INVALID_CONFIG, "fp16 stage must be 0, 32 or 64");
-    staging = 1;
-    stage = t->stage ? t->stage : XG_DEFAULT_RGH_STAGE;
+    /* fp16 real-Gram kernel: register staging (32 or 64 samples per stage)
+     * or global -> LDS DMA (32) */
+    staging = t->staging ? t->staging : 1;
+    if (staging == 1 && t->stage != 0 && t->stage != 32 && t->stage != 64)
+      return xg_fail(XGPU_INVALID_CONFIG, "fp16 stage must be 0, 32 or 64 with register staging");
+    if (staging == 2 && t->stage != 0 && t->stage != 32)
+      return xg_fail(XGPU_INVALID_CONFIG, "fp16 stage must be 0 or 32 with DMA staging");
+    stage = t->stage ? t->stage : (staging == 2 ? 32 : XG_DEFAULT_RGH_STAGE);
   } else {
     staging = t->staging ? t->staging : XG_DEFAULT_RG_STAGING;
     if (t->stage != 0 && !(staging == 1 && (t->stage == 64 || t->stage == 128)) &&
@@ -329,7 +331,7 @@ int xg_launch_family(const XGPUAmdEngine *e, const void *input_d) {
 
 /* XgLaunch.stage code of the real-Gram kernel variant */
 static char rg_stage_code(const XGPUAmdEngine *e) {
-  if (e->family == XG_KERNEL_F16_MFMA) return (char)(e->stage == 64 ? 1 : 0);
+  if (e->family == XG_KERNEL_F16_MFMA) return (char)(e->staging == 2 ? 2 : e->stage == 64 ? 1 : 0);
   if (e->staging == 2) return (char)(e->stage == 32 ? 4 : 3);
   return (char)(e->stage == 128 ? 1 : 0);
 }

@@ -624,6 +624,11 @@ struct RgInt8 {
   static constexpr int kInputs = 8;    // inputs per chunk
   static constexpr int kK = 32;        // samples per MFMA
   __device__ static int slot(int chunk, int srow) { return rg_slot(chunk, srow & 7); }
+  // the chunk that belongs in `slot` of sample row `srow` (inverse of slot)
+  __device__ static int dma_chunk(int slot, int srow) {
+    const int x = slot ^ (srow & 7);
+    return x ^ ((x & 1) << 3);
+  }
   __device__ static int lane_off(int blk, int lane) { return rg_lane_off(2 * blk, lane); }
   __device__ static v4i operand(const unsigned char *p) { return rg_operand(p); }
   __device__ static v16i mfma(v4i a, v4i b, v16i c) {
@@ -638,6 +643,7 @@ struct RgHalf {
   static constexpr int kInputs = 4;
   static constexpr int kK = 16;
   __device__ static int slot(int chunk, int srow) { return rgh_slot(chunk, srow & 3); }
+  __device__ static int dma_chunk(int slot, int srow) { return slot ^ ((srow & 3) << 2); }
   __device__ static int lane_off(int blk, int lane) { return rgh_lane_off(blk, lane); }
   __device__ static v4i operand(const unsigned char *p) { return rgh_operand(p); }
   __device__ static v16f mfma(v4i a, v4i b, v16f c) {
@@ -1136,18 +1142,21 @@ __device__ __forceinline__ void rgd_barrier() {
   asm volatile("s_waitcnt vmcnt(%0)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
 }
 
-template <int STG, int NBUF>
+// E: element format (RgInt8 or RgHalf).  A 1 KiB wave DMA covers kSpd
+// consecutive sample rows (4 int8 rows of 256 bytes, 2 fp16 rows of 512).
+template <class E, int STG, int NBUF>
 struct RgdCfg {
-  static constexpr int kSide = STG * RG_ROW;
+  static constexpr int kSide = STG * E::kRow;
   static constexpr int kBuf = 2 * kSide;
   static constexpr int kLds = NBUF * kBuf;
-  static constexpr int kDma = STG / 16;      // 1 KiB DMAs per wave per side per stage
-  static constexpr int kSteps = STG / 32;
+  static constexpr int kSpd = 1024 / E::kRow;     // sample rows per DMA
+  static constexpr int kDma = STG / (4 * kSpd);   // DMAs per wave per side per stage
+  static constexpr int kSteps = STG / E::kK;
 };
 
-template <int STG, int NBUF>
+template <class E, int STG, int NBUF>
 __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsigned char *lds) {
-  typedef RgdCfg<STG, NBUF> C;
+  typedef RgdCfg<E, STG, NBUF> C;
   static_assert((NBUF & (NBUF - 1)) == 0 && NBUF >= 2, "NBUF: power of two");
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1156,25 +1165,27 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
   const int nstage = (J.ntime + STG - 1) / STG;
   const int ntime = J.ntime;
   const int ts = (int)J.tstride;  // < 2^24: the host keeps 128 * tstride < 2^31
-  const unsigned char *in_f = (const unsigned char *)J.input + (long long)T.f * J.nstation * 4;
-  const unsigned long long row_base = (unsigned long long)(in_f + T.rb * RG_ROW);
-  const unsigned long long col_base = (unsigned long long)(in_f + T.cb * RG_ROW);
-  const int row_bytes = ts < RG_ROW ? ts : RG_ROW;
+  const unsigned char *in_f =
+      (const unsigned char *)J.input + (long long)T.f * J.nstation * (2 * E::kRow / XG_TILE);
+  const unsigned long long row_base = (unsigned long long)(in_f + T.rb * E::kRow);
+  const unsigned long long col_base = (unsigned long long)(in_f + T.cb * E::kRow);
+  const int row_bytes = ts < E::kRow ? ts : E::kRow;
   const unsigned lds_base = (unsigned)(unsigned long long)lds;
 
-  // DMA lane mapping: a wave DMA covers 4 consecutive samples (16 lanes each);
-  // lane L fills slot L & 15 of sample row L >> 4.  (sample & 7) is the same for
-  // every DMA of this wave: DMA j of a stage starts at sample 16 j + 4 wave.
-  const int s7 = (4 * wave + (lane >> 4)) & 7;
-  const int slot = lane & 15;
-  const int chunk = (slot ^ s7) ^ (((slot ^ s7) & 1) << 3);
-  const int dvoff = (lane >> 4) * ts + 16 * chunk;
-  const int dvoff0 = (T.rb * XG_TILE + 8 * chunk < ninput) ? dvoff : 0x7fffff00;
-  const int dvoff1 = (!diag && T.cb * XG_TILE + 8 * chunk < ninput) ? dvoff : 0x7fffff00;
+  // DMA lane mapping: wave DMA j of a stage is DMA g = 4 j + wave, sample rows
+  // g kSpd .. +kSpd-1; lane L fills slot L % kChunks of row L / kChunks.  The
+  // row's swizzle phase (sample & 7, or & 3 for fp16) is the same for every
+  // DMA of this wave, so each lane loads one fixed chunk.
+  const int srow = lane / E::kChunks, slot = lane % E::kChunks;
+  const int chunk = E::dma_chunk(slot, wave * C::kSpd + srow);
+  const int dvoff = srow * ts + 16 * chunk;
+  const int dvoff0 = (T.rb * XG_TILE + E::kInputs * chunk < ninput) ? dvoff : 0x7fffff00;
+  const int dvoff1 = (!diag && T.cb * XG_TILE + E::kInputs * chunk < ninput) ? dvoff : 0x7fffff00;
   // the DMAs of stage st, in order: side 0 then side 1, j = 0 .. kDma-1
   auto dma = [&](int st, int side, int j) {
-    const int s0 = st * STG + 16 * j + 4 * wave;  // first sample of this DMA
-    const int rows = min(ntime - s0, 4);
+    const int g = 4 * j + wave;
+    const int s0 = st * STG + g * C::kSpd;  // first sample of this DMA
+    const int rows = min(ntime - s0, C::kSpd);
     const int nrec = max(rows * ts - ts + row_bytes, 0);
     const unsigned long long a = (side ? col_base : row_base) + (unsigned long long)(long long)s0 * ts;
     v4i rs;
@@ -1182,8 +1193,7 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
     rs.y = (int)((a >> 32) & 0xffff);
     rs.z = nrec;
     rs.w = 0x00020000;
-    const unsigned m0 = lds_base + (st & (NBUF - 1)) * C::kBuf + side * C::kSide +
-                        (16 * j + 4 * wave) * RG_ROW;
+    const unsigned m0 = lds_base + (st & (NBUF - 1)) * C::kBuf + side * C::kSide + g * 1024;
     rg_dma16(rs, side ? dvoff1 : dvoff0, m0);
   };
   auto dma_stage = [&](int st) {
@@ -1196,10 +1206,10 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
   const int wr = wave >> 1, wc = wave & 1;
   int aoff[4], boff[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) aoff[a] = rg_lane_off(8 * wr + 2 * a, lane);
+  for (int a = 0; a < 4; ++a) aoff[a] = E::lane_off(4 * wr + a, lane);
 #pragma unroll
-  for (int b = 0; b < 4; ++b) boff[b] = rg_lane_off(8 * wc + 2 * b, lane) + (diag ? 0 : C::kSide);
-  v16i acc[4][4];
+  for (int b = 0; b < 4; ++b) boff[b] = E::lane_off(4 * wc + b, lane) + (diag ? 0 : C::kSide);
+  typename E::Acc acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -1210,9 +1220,9 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
   v4i A0[4], B0[4], A1[4], B1[4];
   auto lread = [&](const unsigned char *base, v4i *A, v4i *B) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) A[a] = rg_operand(base + aoff[a]);
+    for (int a = 0; a < 4; ++a) A[a] = E::operand(base + aoff[a]);
 #pragma unroll
-    for (int b = 0; b < 4; ++b) B[b] = rg_operand(base + boff[b]);
+    for (int b = 0; b < 4; ++b) B[b] = E::operand(base + boff[b]);
   };
   // One kstep: 16 MFMAs on (A, B), the 16 transpose reads of the next kstep
   // over the first 8, and (when FILL) the DMAs of stage `st` over the last 8.
@@ -1221,9 +1231,7 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
     constexpr bool FILL = decltype(fill_c)::value;
     constexpr int kTot = 2 * C::kDma;  // DMAs of one stage, spread over MFMAs 8..15
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
-      acc[i >> 2][i & 3] =
-          __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i >> 2], B[i & 3], acc[i >> 2][i & 3], 0, 0, 0);
+    for (int i = 0; i < 8; ++i) acc[i >> 2][i & 3] = E::mfma(A[i >> 2], B[i & 3], acc[i >> 2][i & 3]);
     lread(nbase, NA, NB);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -1232,8 +1240,7 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
     }
 #pragma unroll
     for (int i = 8; i < 16; ++i) {
-      acc[i >> 2][i & 3] =
-          __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i >> 2], B[i & 3], acc[i >> 2][i & 3], 0, 0, 0);
+      acc[i >> 2][i & 3] = E::mfma(A[i >> 2], B[i & 3], acc[i >> 2][i & 3]);
       if constexpr (FILL) {
         // DMAs [(i-8) kTot / 8, (i-7) kTot / 8): every DMA exactly once
 #pragma unroll
@@ -1271,9 +1278,9 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
 #pragma unroll
     for (int ks = 0; ks < C::kSteps - 1; ++ks) {
       if ((P + ks) & 1)
-        kstep(no(), A1, B1, A0, B0, cur + 32 * (ks + 1) * RG_ROW, 0);
+        kstep(no(), A1, B1, A0, B0, cur + E::kK * (ks + 1) * E::kRow, 0);
       else
-        kstep(no(), A0, B0, A1, B1, cur + 32 * (ks + 1) * RG_ROW, 0);
+        kstep(no(), A0, B0, A1, B1, cur + E::kK * (ks + 1) * E::kRow, 0);
     }
     // stage s+1 has landed (this wave's DMAs; the barrier covers the others'),
     // and every wave is done reading stage s's buffer, which is refilled with
@@ -1303,12 +1310,26 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
 template <int STG, int NBUF>
 __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
 void xg_xengine_rgd(XgLaunch L) {
-  __shared__ __attribute__((aligned(16))) unsigned char lds[RgdCfg<STG, NBUF>::kLds];
+  __shared__ __attribute__((aligned(16))) unsigned char lds[RgdCfg<RgInt8, STG, NBUF>::kLds];
   const int xcd = blockIdx.x % XG_NXCD;
   const int wl = blockIdx.x / XG_NXCD;
   TileRef T;
   for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
-    rgd_tile<STG, NBUF>(L.jobs[T.job], T, lds);
+    rgd_tile<RgInt8, STG, NBUF>(L.jobs[T.job], T, lds);
+    rg_barrier();  // every wave is done with this tile's LDS buffers
+  }
+}
+
+// fp16 input, global -> LDS DMA staging (16-byte aligned input)
+template <int STG, int NBUF>
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
+void xg_xengine_rgdh(XgLaunch L) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[RgdCfg<RgHalf, STG, NBUF>::kLds];
+  const int xcd = blockIdx.x % XG_NXCD;
+  const int wl = blockIdx.x / XG_NXCD;
+  TileRef T;
+  for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
+    rgd_tile<RgHalf, STG, NBUF>(L.jobs[T.job], T, lds);
     rg_barrier();  // every wave is done with this tile's LDS buffers
   }
 }
@@ -1932,7 +1953,9 @@ extern "C" int xgh_launch_xengine(int family, const XgLaunch *L, void *stream) {
       hipLaunchKernelGGL(xg_xengine_kg, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       break;
     case XG_KERNEL_F16_RG:
-      if (L->stage == 1)
+      if (L->stage == 2)
+        hipLaunchKernelGGL((xg_xengine_rgdh<32, 4>), dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      else if (L->stage == 1)
         hipLaunchKernelGGL(xg_xengine_rgh<64>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       else
         hipLaunchKernelGGL(xg_xengine_rgh<32>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
