@@ -165,10 +165,10 @@ class Engine:
                    kernel: int = 0, stage: int = 0, staging: int = 0) -> None:
         """wg_per_cu: 1 or 2; prefetch / graph: 1 on, 2 off; kernel (int8 and
         fp16): 1 complex-plane, 2 real-Gram where it applies, 3 (int8)
-        K-interleaved pair-tile kernel where the real-Gram one applies; staging (int8
-        real-Gram): 1 registers, 2 global->LDS DMA; stage: samples per LDS
+        K-interleaved pair-tile kernel where the real-Gram one applies; staging
+        (real-Gram): 1 registers, 2 global->LDS DMA; stage: samples per LDS
         stage (int8: 64 / 128 with register staging, 32 / 64 with DMA; fp16:
-        32 / 64) (0 = default everywhere)."""
+        32 / 64 with register staging, 32 with DMA) (0 = default everywhere)."""
         t = _lib.XGPUAmdTuning()
         t.wg_per_cu, t.prefetch, t.graph = wg_per_cu, prefetch, graph
         t.kernel, t.stage, t.staging = kernel, stage, staging
