@@ -1122,9 +1122,11 @@ void xg_xengine_rgh(XgLaunch L) {
 // [M0 + 16 L, +16), an out-of-range lane writes zeros), so there are no
 // staging registers and no LDS write instructions, and NBUF stage buffers
 // (128 KiB) let each stage's loads be issued NBUF-1 stages before its data is
-// read.  Each lane loads the chunk that belongs in its LDS slot: slot
-// = rg_slot(chunk, sample & 7), i.e. chunk = g(slot ^ (sample & 7)) with
-// g(x) = x ^ ((x & 1) << 3).
+// read.  Each lane loads the chunk that belongs in its LDS slot (E::dma_chunk,
+// the inverse of E::slot): for int8, slot = rg_slot(chunk, sample & 7), i.e.
+// chunk = g(slot ^ (sample & 7)) with g(x) = x ^ ((x & 1) << 3); for fp16,
+// chunk = slot ^ 4 (sample & 3).  Both element formats (int8 xg_xengine_rgd,
+// fp16 xg_xengine_rgdh) share the tile function.
 //
 // The DMAs are issued from inline assembly: the compiler would otherwise
 // wait for every outstanding LDS DMA before each LDS read.  Each wave waits
