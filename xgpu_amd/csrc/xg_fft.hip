@@ -236,16 +236,17 @@ __device__ __forceinline__ void fft_pass(float2 *X, float2 *work, long long base
   if constexpr (!LAST) __syncthreads();  // this pass's writes before the next pass's reads
 }
 
-// passes of the plan from pass P (span L) on
-template <int N, int P, int L, bool INV, int LAYOUT>
+// passes of the plan from pass P (span L) on; FUSE_IN / FUSE_OUT: the first
+// pass may read HBM and the last may write it directly
+template <int N, int P, int L, bool INV, int LAYOUT, bool FUSE_IN, bool FUSE_OUT>
 __device__ __forceinline__ void fft_passes(float2 *X, float2 *work, long long base, long long nelem,
                                            int tid) {
   constexpr int R = fft_radix(N, P);
   if constexpr (R > 0) {
-    constexpr bool FIRST = P == 0 && N / R >= 8;
-    constexpr bool LAST = fft_radix(N, P + 1) == 0 && L >= 8;
+    constexpr bool FIRST = FUSE_IN && P == 0 && N / R >= 8;
+    constexpr bool LAST = FUSE_OUT && fft_radix(N, P + 1) == 0 && L >= 8;
     fft_pass<N, R, L, FIRST, LAST, INV, LAYOUT>(X, work, base, nelem, tid);
-    fft_passes<N, P + 1, L * R, INV, LAYOUT>(X, work, base, nelem, tid);
+    fft_passes<N, P + 1, L * R, INV, LAYOUT, FUSE_IN, FUSE_OUT>(X, work, base, nelem, tid);
   }
 }
 
@@ -256,8 +257,12 @@ __global__ __launch_bounds__(FftCfg<N>::kThreads) void xg_fft_k(float2 *work, lo
   constexpr int kPasses = fft_radix(N, 1) == 0 ? 1 : fft_radix(N, 2) == 0 ? 2
                         : fft_radix(N, 3) == 0 ? 3 : fft_radix(N, 4) == 0 ? 4 : 5;
   constexpr int RL = fft_radix(N, kPasses - 1);          // radix of the last pass
-  constexpr bool kStageIn = N / R0 < 8;                   // first pass reads LDS
-  constexpr bool kStageOut = N / RL < 8;                  // last pass writes LDS (span N/RL)
+  // The spectrum side of the prototype layout at N >= 2048 is transposed
+  // (bin y N/512 + z at 512 z + y): it goes through LDS too, so HBM sees
+  // whole consecutive runs instead of four lines 4 KiB apart per wave access.
+  constexpr bool kProtoSpec = LAYOUT == XGPU_FFT_PROTOTYPE && N >= 2048;
+  constexpr bool kStageIn = N / R0 < 8 || (kProtoSpec && INV);    // first pass reads LDS
+  constexpr bool kStageOut = N / RL < 8 || (kProtoSpec && !INV);  // last pass writes LDS
   __shared__ float2 X[C::kC];
   const int tid = threadIdx.x;
   const long long base = (long long)blockIdx.x * C::kC;
@@ -271,7 +276,7 @@ __global__ __launch_bounds__(FftCfg<N>::kThreads) void xg_fft_k(float2 *work, lo
     }
     __syncthreads();
   }
-  fft_passes<N, 0, 1, INV, LAYOUT>(X, work, base, nelem, tid);
+  fft_passes<N, 0, 1, INV, LAYOUT, !kStageIn, !kStageOut>(X, work, base, nelem, tid);
   if constexpr (kStageOut) {  // coalesced LDS -> HBM
 #pragma unroll
     for (int e = 0; e < C::kE; ++e) {
