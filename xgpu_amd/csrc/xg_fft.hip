@@ -144,6 +144,21 @@ __device__ __forceinline__ int fft_gpos(int q) {
   return q;
 }
 
+// LDS slot of natural-order element q.  The prototype layout of N = 8 and 16
+// interleaves 64 transforms (element y of transform z at z + 64 y), so the
+// staging loops, coalesced on the HBM side, touch LDS with a stride of N
+// elements between lanes: N = 16 hits 2 of the 64 banks.  An XOR of the
+// element index with bits of the transform index spreads them: a block of N
+// float2 spans 2N of the 64 banks, so 32 / N consecutive transforms already
+// start on different banks, and the XOR takes the transform index bits above
+// those.  The slot stays in its N-element block, so every index stays below kC.
+template <int N, int LAYOUT>
+__device__ __forceinline__ int fft_lds(int q) {
+  if constexpr (LAYOUT == XGPU_FFT_PROTOTYPE && (N == 8 || N == 16))
+    return q ^ (((q / N) >> (N == 8 ? 2 : 1)) & (N - 1));
+  return q;
+}
+
 // Inverse of fft_gpos: natural-order element held at chunk offset i.
 template <int N, bool INV, int LAYOUT, bool IN>
 __device__ __forceinline__ int fft_natural(int i) {
@@ -190,7 +205,7 @@ __device__ __forceinline__ void fft_pass(float2 *X, float2 *work, long long base
         if (INV) x.y = -x.y;  // inverse = conj(FFT(conj(x)))
         v[p][r] = x;
       } else {
-        v[p][r] = X[q];
+        v[p][r] = X[fft_lds<N, LAYOUT>(q)];
       }
     }
     if (L > 1) {
@@ -229,7 +244,7 @@ __device__ __forceinline__ void fft_pass(float2 *X, float2 *work, long long base
           work[g] = y;
         }
       } else {
-        X[q] = v[p][s];
+        X[fft_lds<N, LAYOUT>(q)] = v[p][s];
       }
     }
   }
@@ -272,7 +287,7 @@ __global__ __launch_bounds__(FftCfg<N>::kThreads) void xg_fft_k(float2 *work, lo
       const int i = tid + C::kThreads * e;
       float2 v = base + i < nelem ? work[base + i] : make_float2(0.f, 0.f);
       if (INV) v.y = -v.y;
-      X[fft_natural<N, INV, LAYOUT, true>(i)] = v;
+      X[fft_lds<N, LAYOUT>(fft_natural<N, INV, LAYOUT, true>(i))] = v;
     }
     __syncthreads();
   }
@@ -282,7 +297,7 @@ __global__ __launch_bounds__(FftCfg<N>::kThreads) void xg_fft_k(float2 *work, lo
     for (int e = 0; e < C::kE; ++e) {
       const int i = tid + C::kThreads * e;
       if (base + i < nelem) {
-        float2 v = X[fft_natural<N, INV, LAYOUT, false>(i)];
+        float2 v = X[fft_lds<N, LAYOUT>(fft_natural<N, INV, LAYOUT, false>(i))];
         if (INV) v.y = -v.y;
         work[base + i] = v;
       }
