@@ -122,19 +122,10 @@ static void job_fill(XgJob *J, const XGPUAmdEngine *e, const void *input, unsign
   J->chan_base = chan_base;
 }
 
-/* Number of pair tiles of the K-interleaved kernel: XG_TILE row inputs by
- * 2 XG_TILE column inputs, (rb, cp) with 2 cp <= rb (xg_pair_count in
- * xg_kernels.hip). */
-static int pair_count(int nblk) {
-  int n = 0;
-  for (int cp = 0; 2 * cp < nblk; ++cp) n += nblk - 2 * cp;
-  return n;
-}
-
 /* Tile kind of a job for the kernel family that runs it. */
 static void job_tiles(XgJob *J, int family) {
   J->pair_tiles = (family == XG_KERNEL_I8_KG);
-  J->tiles_per_chan = J->pair_tiles ? pair_count(J->nblk) : J->nblk * (J->nblk + 1) / 2;
+  J->tiles_per_chan = J->pair_tiles ? xg_pair_count(J->nblk) : J->nblk * (J->nblk + 1) / 2;
 }
 
 void xg_plan_launch(XgLaunch *L, int cus, int wg_per_cu) {

@@ -99,11 +99,6 @@ __host__ __device__ inline void xg_tile_order(int nblk, int u, int *rb, int *cb)
 // column inputs [2 XG_TILE cp, +2 XG_TILE), for 2 cp <= rb: every input pair
 // (m, n) with n <= m + 1 lies in exactly one of them (rb = m / 128,
 // cp = n / 256).  Order: column by column, rows from the diagonal down.
-__host__ __device__ inline int xg_pair_count(int nblk) {
-  int n = 0;
-  for (int cp = 0; 2 * cp < nblk; ++cp) n += nblk - 2 * cp;
-  return n;
-}
 __host__ __device__ inline void xg_pair_order(int nblk, int u, int *rb, int *cp) {
   for (int c = 0; 2 * c < nblk; ++c) {
     const int n = nblk - 2 * c;

@@ -65,6 +65,15 @@ typedef struct XgJob {
                           tiles_per_chan = xg_pair_count(nblk) */
 } XgJob;
 
+/* Pair tiles per channel of the K-interleaved kernel: (rb, cp) with 2 cp < nblk
+ * and 2 cp <= rb, nblk - 2 cp of them in column cp (the order xg_pair_order in
+ * xg_kernels.hip walks). */
+static inline int xg_pair_count(int nblk) {
+  int n = 0;
+  for (int cp = 0; 2 * cp < nblk; ++cp) n += nblk - 2 * cp;
+  return n;
+}
+
 typedef struct XgLaunch {
   XgJob jobs[XG_MAX_JOBS];
   int njobs;
