@@ -383,15 +383,21 @@ def test_f16_spill_to_f64():
 
 
 # ------------------------------------------------------------------ batching
-def test_batch_ragged_equals_alone():
+@pytest.mark.parametrize("kernel", [0, 3])
+def test_batch_ragged_equals_alone(kernel):
+    """One persistent launch for 4 engines of different shapes (square tiles
+    with the default kernel, pair tiles with kernel 3): each equals its
+    oracle, and a second batch accumulates."""
     shapes = [(64, 3, 256), (96, 2, 512), (200, 1, 300), (16, 5, 64)]
     engines, bufs, refs = [], [], []
     for k, (ns, nf, nt) in enumerate(shapes):
         x = oracle.fill_uniform(nt * nf * ns * 4, 50 + k, -128, 127)
         engines.append(X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER))
+        if kernel:
+            engines[-1].set_tuning(kernel=kernel)
         bufs.append(X.DeviceBuffer.from_array(x))
         refs.append(oracle.xengine_i64(x, ns, nf, nt))
-    flush("[i8] ragged batch of 4")
+    flush(f"[i8] ragged batch of 4, kernel {kernel}")
     X.correlate_batch([(e, b, s[2]) for e, b, s in zip(engines, bufs, shapes)])
     for e, r in zip(engines, refs):
         assert np.array_equal(e.dump().astype(np.int64), r)
