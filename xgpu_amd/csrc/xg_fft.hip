@@ -102,7 +102,7 @@ __device__ __forceinline__ void dft(float2 *a) {
 __host__ __device__ constexpr int fft_radix(int n, int p) {
   switch (n) {
     case 8: return p == 0 ? 8 : 0;
-    case 16: return p < 2 ? 4 : 0;
+    case 16: return p == 0 ? 8 : p == 1 ? 2 : 0;
     case 32: return p == 0 ? 8 : p == 1 ? 4 : 0;
     case 64: return p < 2 ? 8 : 0;
     case 128: return p == 0 ? 8 : p < 3 ? 4 : 0;
@@ -152,10 +152,10 @@ __device__ __forceinline__ int fft_gpos(int q) {
 // float2 spans 2N of the 64 banks, so 32 / N consecutive transforms already
 // start on different banks, and the XOR takes the transform index bits above
 // those.  The slot stays in its N-element block, so every index stays below kC.
+// The radix passes of these small sizes gain from it in natural order too.
 template <int N, int LAYOUT>
 __device__ __forceinline__ int fft_lds(int q) {
-  if constexpr (LAYOUT == XGPU_FFT_PROTOTYPE && (N == 8 || N == 16))
-    return q ^ (((q / N) >> (N == 8 ? 2 : 1)) & (N - 1));
+  if constexpr (N == 8 || N == 16) return q ^ (((q / N) >> (N == 8 ? 2 : 1)) & (N - 1));
   return q;
 }
 
@@ -275,9 +275,11 @@ __global__ __launch_bounds__(FftCfg<N>::kThreads) void xg_fft_k(float2 *work, lo
   // The spectrum side of the prototype layout at N >= 2048 is transposed
   // (bin y N/512 + z at 512 z + y): it goes through LDS too, so HBM sees
   // whole consecutive runs instead of four lines 4 KiB apart per wave access.
+  // N = 8 and 16 stage both sides (their last pass would write 64-byte runs).
   constexpr bool kProtoSpec = LAYOUT == XGPU_FFT_PROTOTYPE && N >= 2048;
-  constexpr bool kStageIn = N / R0 < 8 || (kProtoSpec && INV);    // first pass reads LDS
-  constexpr bool kStageOut = N / RL < 8 || (kProtoSpec && !INV);  // last pass writes LDS
+  constexpr bool kSmall = N <= 16;
+  constexpr bool kStageIn = N / R0 < 8 || (kProtoSpec && INV) || kSmall;    // first pass reads LDS
+  constexpr bool kStageOut = N / RL < 8 || (kProtoSpec && !INV) || kSmall;  // last pass writes LDS
   __shared__ float2 X[C::kC];
   const int tid = threadIdx.x;
   const long long base = (long long)blockIdx.x * C::kC;
