@@ -418,12 +418,31 @@ def main():
         e3 = X.Engine(ns3, nf3, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
         # warm-up past the power-state transient after the host-path phase
         ms3 = e3.time_correlate(b3, nt3, warmup=25, reps=10)
-        e3.close()
         b3.free()
+        # the output dump: the integration (register-tile order, int32) reordered
+        # on the device into interleaved triangular order in HBM, ~1 GB each way
+        out3 = X.DeviceBuffer(2 * e3.tri_len * 4)
+        flags = X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR
+        dts = []
+        for _ in range(6):
+            e3.synchronize()
+            t0 = time.perf_counter()
+            X.check(X.core().xgpuAmdDump(e3.handle, out3.ptr, flags))
+            e3.synchronize()
+            dts.append(time.perf_counter() - t0)
+        out3.free()
+        dump_ms = sorted(dts[1:])[2] * 1e3
+        dump_bytes = (e3.mat_len + e3.tri_len) * 8
+        e3.close()
         c3 = X.cmac_per_integration(ns3, nf3, nt3)
         cfg3 = {"workload": "NSTATION=2048 x 2 pol, NFREQUENCY=16, NTIME=4096, int8",
                 "kernel_ms": round(ms3, 4), "tcmac_s": round(c3 / (ms3 * 1e-3) / 1e12, 1),
-                "pct_int8_mfma_peak": round(100 * c3 * 8 / (ms3 * 1e-3) / 1e12 / PEAK_INT8_TOPS, 1)}
+                "pct_int8_mfma_peak": round(100 * c3 * 8 / (ms3 * 1e-3) / 1e12 / PEAK_INT8_TOPS, 1),
+                "dump": {"what": "device reorder to triangular int32 (read register-tile, write "
+                                 "triangular), wall clock around the call, median of 5",
+                         "ms": round(dump_ms, 4), "bytes": dump_bytes,
+                         "gb_s": round(dump_bytes / (dump_ms * 1e-3) / 1e9, 1),
+                         "pct_hbm_8tb_s": round(100 * dump_bytes / (dump_ms * 1e-3) / 8e12, 1)}}
         log(f"[bench] cfg3: {ms3:.3f} ms, {cfg3['tcmac_s']} TCMAC/s")
     except Exception as exc:
         cfg3 = {"error": str(exc)}

@@ -187,6 +187,17 @@ def test_i8_device_reorder_dump():
         ref = oracle.xengine_i64(x, ns, nf, nt)
         assert np.array_equal(eng.dump(triangular=True).astype(np.int64), ref)
         assert np.array_equal(eng.dump(wide=True, triangular=True), ref)
+        # to device memory: reordered straight into the destination, 16-byte
+        # aligned (vector cell path) and 4 bytes off (scalar path)
+        for wide, dt in ((False, np.int32), (True, np.int64)):
+            n = 2 * eng.tri_len
+            out = X.DeviceBuffer(n * 8 + 16)
+            flags = X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR | (X.XGPU_DUMP_WIDE if wide else 0)
+            for off in (0, 4):
+                X.check(X.core().xgpuAmdDump(eng.handle, out.ptr + off, flags))
+                got = out.download(dt, count=n, offset=off).astype(np.int64)
+                assert np.array_equal(got, ref), (wide, off)
+            out.free()
         eng.close()
 
 

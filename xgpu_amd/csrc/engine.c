@@ -415,7 +415,11 @@ static int ensure_scratch(XGPUAmdEngine *e, void **buf, size_t bytes) {
 
 /* Stage the integration (flags XGPU_DUMP_*) into device memory and return a
  * pointer to it and its size; everything is queued on the engine stream. */
-static int dump_stage(XGPUAmdEngine *e, int flags, const void **out, size_t *bytes) {
+/* Stage the integration for a dump: combine spilled parts, widen, and reorder
+ * to triangular order as the flags ask.  When `direct` is non-NULL (a device
+ * destination), the last pass writes there and *out is set to NULL, so no
+ * extra copy follows; otherwise *out names the staged device buffer. */
+static int dump_stage(XGPUAmdEngine *e, int flags, void *direct, const void **out, size_t *bytes) {
   const int wide = (flags & XGPU_DUMP_WIDE) != 0;
   const int tri = (flags & XGPU_DUMP_TRIANGULAR) && e->cfg.matrix_order != TRIANGULAR_ORDER;
   const size_t n = e->info.matLength * 2; /* scalars in the integration */
@@ -449,12 +453,16 @@ static int dump_stage(XGPUAmdEngine *e, int flags, const void **out, size_t *byt
     val = e->scratch;
   }
   if (tri) {
-    if ((rc = ensure_scratch(e, &e->scratch2, n * 8))) return rc;
-    HIPCALL(xgh_launch_reorder((int)e->cfg.matrix_order, (int)esz, e->scratch2, val,
+    void *target = direct;
+    if (!target) {
+      if ((rc = ensure_scratch(e, &e->scratch2, n * 8))) return rc;
+      target = e->scratch2;
+    }
+    HIPCALL(xgh_launch_reorder((int)e->cfg.matrix_order, (int)esz, target, val,
                                (int)e->cfg.nstation, (int)e->cfg.nfrequency,
                                (long long)e->info.matLength, e->stream),
             "dump reorder");
-    val = e->scratch2;
+    val = direct ? NULL : target;
   }
   *out = val;
   *bytes = (tri ? e->info.triLength * 2 : n) * esz;
@@ -464,8 +472,11 @@ static int dump_stage(XGPUAmdEngine *e, int flags, const void **out, size_t *byt
 int xg_dump_async(XGPUAmdEngine *e, void *dst, int flags) {
   const void *val = NULL;
   size_t bytes = 0;
-  int rc = dump_stage(e, flags, &val, &bytes);
+  const int tri = (flags & XGPU_DUMP_TRIANGULAR) && e->cfg.matrix_order != TRIANGULAR_ORDER;
+  /* A triangular dump to device memory is reordered straight into dst. */
+  int rc = dump_stage(e, flags, (tri && (flags & XGPU_DUMP_DEVICE)) ? dst : NULL, &val, &bytes);
   if (rc) return rc;
+  if (!val) return XGPU_OK;
   if (flags & XGPU_DUMP_DEVICE)
     HIPCALL(xgh_memcpy_d2d_async(dst, val, bytes, e->stream), "dump D2D");
   else

@@ -1862,6 +1862,41 @@ __global__ void xg_reorder_k(int src_order, E *dst, const E *src, int nstation, 
   }
 }
 
+// The same reorder for 32-bit elements, one whole cell per thread: the
+// cell's 4 products are 16 consecutive bytes in each source plane (register
+// tile and real/imag order alike), and 32 interleaved bytes in triangular
+// order, so a thread does the baseline index math once and moves the cell
+// with two 16-byte loads and two 16-byte stores.  src, dst 16-byte aligned
+// and plane % 4 == 0 (checked by the launcher).
+typedef unsigned xg_u4 __attribute__((ext_vector_type(4)));
+__global__ void xg_reorder_cells_k(int src_order, xg_u4 *dst, const unsigned *src, int nstation,
+                                   int nfreq, long long plane) {
+  const long long NB = (long long)nstation * (nstation + 1) / 2;
+  const long long total = (long long)nfreq * NB;
+  const long long P = nstation >> 1;
+  const long long RT = P * (P + 1) / 2;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
+    const int f = (int)(x / NB);
+    const long long b = x - (long long)f * NB;
+    int i = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+    while ((long long)i * (i + 1) / 2 > b) --i;
+    while ((long long)(i + 1) * (i + 2) / 2 <= b) ++i;
+    const int j = (int)(b - (long long)i * (i + 1) / 2);
+    long long s;
+    if (src_order == XG_ORDER_REGTILE) {
+      const int R = i >> 1, rx = i & 1, C = j >> 1, ry = j & 1;
+      s = 4 * (((long long)f * 4 + (2 * ry + rx)) * RT + (long long)R * (R + 1) / 2 + C);
+    } else {
+      s = 4 * x;
+    }
+    const xg_u4 re = *(const xg_u4 *)(src + s);
+    const xg_u4 im = *(const xg_u4 *)(src + s + plane);
+    __builtin_nontemporal_store(xg_u4{re[0], im[0], re[1], im[1]}, dst + 2 * x);
+    __builtin_nontemporal_store(xg_u4{re[2], im[2], re[3], im[3]}, dst + 2 * x + 1);
+  }
+}
+
 // dp4a input corner turn (reference src/cpu_util.c:235-255) on the device:
 // out byte ((((t/4*NF+f)*NS+s)*2+p)*2+c)*4 + t%4 = in byte (((t*NF+f)*NS+s)*2+p)*2+c.
 // One thread per output 32-bit word (4 consecutive samples of one component).
@@ -2047,7 +2082,13 @@ extern "C" int xgh_launch_reorder(int src_order, int elem_bytes, void *dst, cons
   if (src_order != XG_ORDER_REGTILE && src_order != XG_ORDER_REALIMAG)
     return xgh_report(hipErrorInvalidValue, "xgh_launch_reorder(order)");
   const dim3 g(grid_for((size_t)total, 256)), b(256);
-  if (elem_bytes == 4)
+  const bool cells = elem_bytes == 4 && !((uintptr_t)dst & 15) && !((uintptr_t)src & 15) &&
+                     (plane & 3) == 0;
+  if (cells)
+    hipLaunchKernelGGL(xg_reorder_cells_k, dim3(grid_for((size_t)(total / 4), 256)), b, 0,
+                       (hipStream_t)stream, src_order, (xg_u4 *)dst, (const unsigned *)src,
+                       nstation, nfreq, plane);
+  else if (elem_bytes == 4)
     hipLaunchKernelGGL((xg_reorder_k<unsigned>), g, b, 0, (hipStream_t)stream, src_order,
                        (unsigned *)dst, (const unsigned *)src, nstation, nfreq, plane);
   else if (elem_bytes == 8)
