@@ -187,7 +187,8 @@ def kernel_name(tun):
 
 def profile_only(X, which, steps, warmup, kernel=0, stage=0, staging=0):
     """Run one path's workload warmup+steps times (one kernel per step).
-    "copy": a plain 1 GiB device-to-device copy per step (counter calibration)."""
+    "copy": a plain 1 GiB device-to-device copy per step (counter calibration);
+    "dump": one NS=2048, NF=16 integration, then one device triangular dump per step."""
     if which == "fft":
         import numpy as np
         buf = X.DeviceBuffer(FFT_ENTRIES * 8)
@@ -205,6 +206,21 @@ def profile_only(X, which, steps, warmup, kernel=0, stage=0, staging=0):
         for _ in range(warmup + steps):
             X.check(X.core().xgpuAmdMemcpy(b.ptr, a.ptr, 1 << 30, 3))
         log(f"[bench] copy: {warmup + steps} x 1 GiB device copies done")
+        return
+    if which == "dump":
+        ns, nf, nt = 2048, 16, 4096
+        buf = X.DeviceBuffer(ns * nf * nt * 4)
+        buf.fill_uniform_int8(3, -128, 127)
+        eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+        eng.correlate(buf, nt)
+        buf.free()
+        out = X.DeviceBuffer(2 * eng.tri_len * 4)
+        flags = X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR
+        for _ in range(warmup + steps):
+            X.check(X.core().xgpuAmdDump(eng.handle, out.ptr, flags))
+        log(f"[bench] dump: {warmup + steps} triangular int32 dumps of NS={ns} NF={nf} done")
+        out.free()
+        eng.close()
         return
     if which == "fp16":
         ns, nf, nt, comp, nbytes = 256, 128, 2048, X.XGPU_FLOAT16, 256 * 128 * 2048 * 8
@@ -288,10 +304,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--only", choices=["int8", "fp32", "fp16", "copy", "fft"], default=None,
+    ap.add_argument("--only", choices=["int8", "fp32", "fp16", "copy", "fft", "dump"], default=None,
                     help="run just this path's workload (for profiling); prints no JSON line")
-    ap.add_argument("--kernel", type=int, default=0, choices=[0, 1, 2],
-                    help="int8 kernel: 1 complex-plane, 2 real-Gram (0 = default)")
+    ap.add_argument("--kernel", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="int8 kernel: 1 complex-plane, 2 real-Gram, 3 K-interleaved "
+                         "(0 = default)")
     ap.add_argument("--staging", type=int, default=0, choices=[0, 1, 2],
                     help="real-Gram kernel staging: 1 registers, 2 global->LDS DMA (0 = default)")
     ap.add_argument("--stage", type=int, default=0, choices=[0, 32, 64, 128],
