@@ -1862,21 +1862,24 @@ __global__ void xg_reorder_k(int src_order, E *dst, const E *src, int nstation, 
   }
 }
 
-// The same reorder for 32-bit elements, one whole cell per thread: the
-// cell's 4 products are 16 consecutive bytes in each source plane (register
-// tile and real/imag order alike), and 32 interleaved bytes in triangular
-// order, so a thread does the baseline index math once and moves the cell
-// with two 16-byte loads and two 16-byte stores.  src, dst 16-byte aligned
-// and plane % 4 == 0 (checked by the launcher).
+// The same reorder for 32-bit elements, one half cell per thread.  A cell's
+// 4 products are 16 consecutive bytes in each source plane (register tile and
+// real/imag order alike) and 32 interleaved bytes in triangular order; thread
+// y moves half h = y & 1 of cell y >> 1 (products 2h, 2h+1) with two 8-byte
+// loads and one 16-byte store, so a wave's store covers 1 KiB of the
+// destination without gaps.  src, dst 16-byte aligned and plane % 4 == 0
+// (checked by the launcher).
+typedef unsigned xg_u2 __attribute__((ext_vector_type(2)));
 typedef unsigned xg_u4 __attribute__((ext_vector_type(4)));
 __global__ void xg_reorder_cells_k(int src_order, xg_u4 *dst, const unsigned *src, int nstation,
                                    int nfreq, long long plane) {
   const long long NB = (long long)nstation * (nstation + 1) / 2;
-  const long long total = (long long)nfreq * NB;
+  const long long total = 2 * (long long)nfreq * NB;
   const long long P = nstation >> 1;
   const long long RT = P * (P + 1) / 2;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
+  for (long long y = (long long)blockIdx.x * blockDim.x + threadIdx.x; y < total; y += stride) {
+    const long long x = y >> 1;
     const int f = (int)(x / NB);
     const long long b = x - (long long)f * NB;
     int i = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
@@ -1890,10 +1893,10 @@ __global__ void xg_reorder_cells_k(int src_order, xg_u4 *dst, const unsigned *sr
     } else {
       s = 4 * x;
     }
-    const xg_u4 re = *(const xg_u4 *)(src + s);
-    const xg_u4 im = *(const xg_u4 *)(src + s + plane);
-    __builtin_nontemporal_store(xg_u4{re[0], im[0], re[1], im[1]}, dst + 2 * x);
-    __builtin_nontemporal_store(xg_u4{re[2], im[2], re[3], im[3]}, dst + 2 * x + 1);
+    s += 2 * (y & 1);
+    const xg_u2 re = *(const xg_u2 *)(src + s);
+    const xg_u2 im = *(const xg_u2 *)(src + s + plane);
+    __builtin_nontemporal_store(xg_u4{re[0], im[0], re[1], im[1]}, dst + y);
   }
 }
 
@@ -2085,7 +2088,7 @@ extern "C" int xgh_launch_reorder(int src_order, int elem_bytes, void *dst, cons
   const bool cells = elem_bytes == 4 && !((uintptr_t)dst & 15) && !((uintptr_t)src & 15) &&
                      (plane & 3) == 0;
   if (cells)
-    hipLaunchKernelGGL(xg_reorder_cells_k, dim3(grid_for((size_t)(total / 4), 256)), b, 0,
+    hipLaunchKernelGGL(xg_reorder_cells_k, dim3(grid_for((size_t)(total / 2), 256)), b, 0,
                        (hipStream_t)stream, src_order, (xg_u4 *)dst, (const unsigned *)src,
                        nstation, nfreq, plane);
   else if (elem_bytes == 4)
