@@ -374,12 +374,16 @@ static int unblock_input(XGPUAmdEngine *e, const void *input_d, unsigned ntime, 
   return XGPU_OK;
 }
 
-/* Correlate plain-layout device input (splitting at the integration bound). */
+/* Correlate plain-layout device input (splitting at the integration bound).
+ * A call that fits a fresh integration but not the room left spills first
+ * and runs whole: splitting it would cost one more launch, whose sweep over
+ * every tile and read-modify-write of the whole output take as long for one
+ * sample as for hundreds. */
 static int correlate_plain(XGPUAmdEngine *e, const void *input_d, unsigned int ntime) {
   const char *p = (const char *)input_d;
   while (ntime > 0) {
     unsigned room = e->bound - e->samples_in_integ;
-    if (room == 0) {
+    if (room == 0 || (room < ntime && ntime <= e->bound && e->samples_in_integ > 0)) {
       int rc = xg_spill(e);
       if (rc) return rc;
       continue;
