@@ -288,15 +288,16 @@ int xgpuAmdGetTuning(const XGPUAmdEngine *e, XGPUAmdTuning *t) {
   return XGPU_OK;
 }
 
-/* wide += integ; integ = 0 (the int32 / f32 part restarts from zero). */
+/* wide += integ; integ = 0 (the int32 / f32 part restarts from zero).  The
+ * first spill of an integration stores the widened values instead of adding
+ * them to a cleared buffer. */
 int xg_spill(XGPUAmdEngine *e) {
   const size_t n = e->info.matLength * 2;
   if (!e->wide) HIPCALL(xgh_malloc(&e->wide, n * 8), "spill buffer");
-  if (!e->spilled) HIPCALL(xgh_launch_fill_zero(e->wide, n * 8, e->stream), "spill clear");
-  if (e->cleared) {
-    /* nothing integrated since the clear */
+  if (e->cleared) { /* nothing integrated since the clear */
+    if (!e->spilled) HIPCALL(xgh_launch_fill_zero(e->wide, n * 8, e->stream), "spill clear");
   } else {
-    HIPCALL(xgh_launch_spill(e->wide_kind, e->wide, e->integ, n, e->stream), "spill");
+    HIPCALL(xgh_launch_spill(e->wide_kind, e->spilled, e->wide, e->integ, n, e->stream), "spill");
   }
   e->spilled = 1;
   e->cleared = 1; /* integ now logically zero: next kernel stores */
@@ -451,8 +452,7 @@ static int dump_stage(XGPUAmdEngine *e, int flags, void *direct, const void **ou
                 "dump combine");
       }
     } else { /* wide requested, nothing spilled: widen integ */
-      HIPCALL(xgh_launch_fill_zero(e->scratch, n * 8, e->stream), "dump zero");
-      HIPCALL(xgh_launch_spill(e->wide_kind, e->scratch, e->integ, n, e->stream), "dump widen");
+      HIPCALL(xgh_launch_spill(e->wide_kind, 0, e->scratch, e->integ, n, e->stream), "dump widen");
     }
     val = e->scratch;
   }

@@ -69,7 +69,8 @@ int xgh_launch_fill_zero(void *ptr, size_t bytes, void *stream);
 /* plain 16-byte streaming copy (16-byte aligned pointers, bytes % 4 == 0) */
 int xgh_launch_copy(void *dst, const void *src, size_t bytes, void *stream);
 /* dst[i] += src[i]; kind XG_OUT_I64 (int32 -> int64) or XG_OUT_F64 (f32 -> f64) */
-int xgh_launch_spill(int kind, void *dst, const void *src, size_t n, void *stream);
+/* dst (int64 / f64) += src (int32 / f32) when add, else dst = src */
+int xgh_launch_spill(int kind, int add, void *dst, const void *src, size_t n, void *stream);
 /* dst[i] = spill[i] + cur[i] (int64 or f64), or the low 32 bits / f32 rounding
  * when narrow != 0 (dst is then int32 / f32) */
 int xgh_launch_combine(int kind, int narrow, void *dst, const void *spill,

@@ -1787,11 +1787,12 @@ __global__ void xg_copy_k(uint4 *dst, const uint4 *src, size_t n16, unsigned *dt
   if (t < ntail) dtail[t] = stail[t];
 }
 
-template <typename D, typename S>
+// dst += src (ADD) or dst = src, widening each element.
+template <typename D, typename S, bool ADD>
 __global__ void xg_spill_k(D *dst, const S *src, size_t n) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    dst[i] += (D)src[i];
+    dst[i] = ADD ? dst[i] + (D)src[i] : (D)src[i];
 }
 
 template <typename W, typename S, typename O>
@@ -2023,14 +2024,22 @@ extern "C" int xgh_launch_copy(void *dst, const void *src, size_t bytes, void *s
   return xgh_report(hipGetLastError(), "xg_copy launch");
 }
 
-extern "C" int xgh_launch_spill(int kind, void *dst, const void *src, size_t n, void *stream) {
+extern "C" int xgh_launch_spill(int kind, int add, void *dst, const void *src, size_t n,
+                                void *stream) {
   if (!n) return 0;
-  if (kind == XG_OUT_I64)
-    hipLaunchKernelGGL((xg_spill_k<long long, int>), dim3(grid_for(n, 256)), dim3(256), 0,
-                       (hipStream_t)stream, (long long *)dst, (const int *)src, n);
+  const dim3 g(grid_for(n, 256)), b(256);
+  if (kind == XG_OUT_I64 && add)
+    hipLaunchKernelGGL((xg_spill_k<long long, int, true>), g, b, 0, (hipStream_t)stream,
+                       (long long *)dst, (const int *)src, n);
+  else if (kind == XG_OUT_I64)
+    hipLaunchKernelGGL((xg_spill_k<long long, int, false>), g, b, 0, (hipStream_t)stream,
+                       (long long *)dst, (const int *)src, n);
+  else if (kind == XG_OUT_F64 && add)
+    hipLaunchKernelGGL((xg_spill_k<double, float, true>), g, b, 0, (hipStream_t)stream,
+                       (double *)dst, (const float *)src, n);
   else if (kind == XG_OUT_F64)
-    hipLaunchKernelGGL((xg_spill_k<double, float>), dim3(grid_for(n, 256)), dim3(256), 0,
-                       (hipStream_t)stream, (double *)dst, (const float *)src, n);
+    hipLaunchKernelGGL((xg_spill_k<double, float, false>), g, b, 0, (hipStream_t)stream,
+                       (double *)dst, (const float *)src, n);
   else
     return xgh_report(hipErrorInvalidValue, "xgh_launch_spill(kind)");
   return xgh_report(hipGetLastError(), "xg_spill launch");
