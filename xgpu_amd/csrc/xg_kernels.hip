@@ -1832,8 +1832,11 @@ __global__ void xg_fill_uniform_k(uint32_t *p, size_t nwords, uint64_t key, int 
 
 // Reorder split-plane REGISTER_TILE / REAL_IMAG into interleaved TRIANGULAR
 // (reference src/cpu_util.c:57-132, run on the device).  One thread per
-// triangular visibility (f, i>=j, p1, p2).
-template <typename E>
+// triangular visibility (f, i>=j, p1, p2).  PAIR (64-bit elements, dst
+// 16-byte aligned): the visibility's (re, im) go out as one 16-byte store, so
+// a wave's store covers 1 KiB without gaps.
+typedef unsigned long long xg_u64x2 __attribute__((ext_vector_type(2)));
+template <typename E, bool PAIR = false>
 __global__ void xg_reorder_k(int src_order, E *dst, const E *src, int nstation, int nfreq,
                              long long plane) {
   const long long NB = (long long)nstation * (nstation + 1) / 2;
@@ -1858,8 +1861,12 @@ __global__ void xg_reorder_k(int src_order, E *dst, const E *src, int nstation, 
     } else {
       s = kk * 4 + pp;
     }
-    dst[2 * x] = src[s];
-    dst[2 * x + 1] = src[s + plane];
+    if constexpr (PAIR) {
+      __builtin_nontemporal_store(xg_u64x2{src[s], src[s + plane]}, (xg_u64x2 *)dst + x);
+    } else {
+      dst[2 * x] = src[s];
+      dst[2 * x + 1] = src[s + plane];
+    }
   }
 }
 
@@ -2103,6 +2110,10 @@ extern "C" int xgh_launch_reorder(int src_order, int elem_bytes, void *dst, cons
   else if (elem_bytes == 4)
     hipLaunchKernelGGL((xg_reorder_k<unsigned>), g, b, 0, (hipStream_t)stream, src_order,
                        (unsigned *)dst, (const unsigned *)src, nstation, nfreq, plane);
+  else if (elem_bytes == 8 && !((uintptr_t)dst & 15))
+    hipLaunchKernelGGL((xg_reorder_k<unsigned long long, true>), g, b, 0, (hipStream_t)stream,
+                       src_order, (unsigned long long *)dst, (const unsigned long long *)src,
+                       nstation, nfreq, plane);
   else if (elem_bytes == 8)
     hipLaunchKernelGGL((xg_reorder_k<unsigned long long>), g, b, 0, (hipStream_t)stream, src_order,
                        (unsigned long long *)dst, (const unsigned long long *)src, nstation, nfreq,
