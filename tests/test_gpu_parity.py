@@ -201,6 +201,31 @@ def test_i8_device_reorder_dump():
         eng.close()
 
 
+def test_i8_spilled_wide_triangular_dump():
+    """A spilled register-tile integration dumped wide in triangular order
+    (spill + current part reordered in one pass), to host and to device
+    memory, 16-byte aligned and 8 bytes off; the int32 dump after it."""
+    ns, nf, nt = 8, 2, 40000
+    x = oracle.fill_uniform(nt * nf * ns * 4, 11, -128, 127)
+    flush("[i8] spilled wide triangular dump ns=8 nf=2 3x40000")
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    buf = X.DeviceBuffer.from_array(x)
+    for _ in range(3):
+        eng.correlate(buf, nt)
+    ref = 3 * oracle.xengine_i64(x, ns, nf, nt)
+    assert np.array_equal(eng.dump(wide=True, triangular=True), ref)
+    n = 2 * eng.tri_len
+    out = X.DeviceBuffer(n * 8 + 16)
+    flags = X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR | X.XGPU_DUMP_WIDE
+    for off in (0, 8):
+        X.check(X.core().xgpuAmdDump(eng.handle, out.ptr + off, flags))
+        assert np.array_equal(out.download(np.int64, count=n, offset=off), ref), off
+    out.free()
+    assert np.array_equal(eng.dump(triangular=True).astype(np.int64),
+                          ((ref + 2**31) % 2**32) - 2**31)
+    eng.close()
+
+
 def test_i8_full_size_cfg2_sampled():
     """NS=512 x 2 pol, NF=64, NT=8192: exact on sampled baselines of every
     channel, every autocorrelation against the input power, and the two int8

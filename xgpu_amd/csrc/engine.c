@@ -418,12 +418,11 @@ static int ensure_scratch(XGPUAmdEngine *e, void **buf, size_t bytes) {
   return XGPU_OK;
 }
 
-/* Stage the integration (flags XGPU_DUMP_*) into device memory and return a
- * pointer to it and its size; everything is queued on the engine stream. */
-/* Stage the integration for a dump: combine spilled parts, widen, and reorder
- * to triangular order as the flags ask.  When `direct` is non-NULL (a device
- * destination), the last pass writes there and *out is set to NULL, so no
- * extra copy follows; otherwise *out names the staged device buffer. */
+/* Stage the integration for a dump (flags XGPU_DUMP_*): combine spilled
+ * parts, widen, and reorder to triangular order as the flags ask, all queued
+ * on the engine stream.  When `direct` is non-NULL (a device destination),
+ * the last pass writes there and *out is set to NULL, so no extra copy
+ * follows; otherwise *out names the staged device buffer and *bytes its size. */
 static int dump_stage(XGPUAmdEngine *e, int flags, void *direct, const void **out, size_t *bytes) {
   const int wide = (flags & XGPU_DUMP_WIDE) != 0;
   const int tri = (flags & XGPU_DUMP_TRIANGULAR) && e->cfg.matrix_order != TRIANGULAR_ORDER;
@@ -431,6 +430,20 @@ static int dump_stage(XGPUAmdEngine *e, int flags, void *direct, const void **ou
   const size_t esz = wide ? 8 : 4;
   const void *val = NULL;
   int rc;
+  if (tri && wide && e->spilled && !e->cleared) { /* spill + current part, reordered in one pass */
+    void *target = direct;
+    if (!target) {
+      if ((rc = ensure_scratch(e, &e->scratch2, n * 8))) return rc;
+      target = e->scratch2;
+    }
+    HIPCALL(xgh_launch_reorder_combine((int)e->cfg.matrix_order, e->wide_kind, target, e->wide,
+                                       e->integ, (int)e->cfg.nstation, (int)e->cfg.nfrequency,
+                                       (long long)e->info.matLength, e->stream),
+            "dump reorder combine");
+    *out = direct ? NULL : target;
+    *bytes = e->info.triLength * 2 * esz;
+    return XGPU_OK;
+  }
   if (e->cleared && !e->spilled) {
     if ((rc = ensure_scratch(e, &e->scratch, n * 8))) return rc;
     HIPCALL(xgh_launch_fill_zero(e->scratch, n * esz, e->stream), "dump zero");

@@ -1836,6 +1836,25 @@ __global__ void xg_fill_uniform_k(uint32_t *p, size_t nwords, uint64_t key, int 
 // 16-byte aligned): the visibility's (re, im) go out as one 16-byte store, so
 // a wave's store covers 1 KiB without gaps.
 typedef unsigned long long xg_u64x2 __attribute__((ext_vector_type(2)));
+
+// Source index (real plane) of triangular visibility x = (f, baseline, p1 p2):
+// NB baselines per channel, RT station-pair cells per register-tile quadrant.
+__device__ __forceinline__ long long xg_reorder_src(int src_order, long long x, long long NB,
+                                                    long long RT) {
+  const int pp = (int)(x & 3);
+  const long long kk = x >> 2;
+  if (src_order != XG_ORDER_REGTILE) return kk * 4 + pp;
+  const int f = (int)(kk / NB);
+  const long long b = kk - (long long)f * NB;
+  int i = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+  while ((long long)i * (i + 1) / 2 > b) --i;
+  while ((long long)(i + 1) * (i + 2) / 2 <= b) ++i;
+  const int j = (int)(b - (long long)i * (i + 1) / 2);
+  const int R = i >> 1, rx = i & 1, C = j >> 1, ry = j & 1;
+  const long long l = ((long long)f * 4 + (2 * ry + rx)) * RT + (long long)R * (R + 1) / 2 + C;
+  return l * 4 + pp;
+}
+
 template <typename E, bool PAIR = false>
 __global__ void xg_reorder_k(int src_order, E *dst, const E *src, int nstation, int nfreq,
                              long long plane) {
@@ -1845,27 +1864,37 @@ __global__ void xg_reorder_k(int src_order, E *dst, const E *src, int nstation, 
   const long long RT = P * (P + 1) / 2;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
-    const int pp = (int)(x & 3);
-    const long long kk = x >> 2;
-    const int f = (int)(kk / NB);
-    const long long b = kk - (long long)f * NB;
-    int i = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
-    while ((long long)i * (i + 1) / 2 > b) --i;
-    while ((long long)(i + 1) * (i + 2) / 2 <= b) ++i;
-    const int j = (int)(b - (long long)i * (i + 1) / 2);
-    long long s;
-    if (src_order == XG_ORDER_REGTILE) {
-      const int R = i >> 1, rx = i & 1, C = j >> 1, ry = j & 1;
-      const long long l = ((long long)f * 4 + (2 * ry + rx)) * RT + (long long)R * (R + 1) / 2 + C;
-      s = l * 4 + pp;
-    } else {
-      s = kk * 4 + pp;
-    }
+    const long long s = xg_reorder_src(src_order, x, NB, RT);
     if constexpr (PAIR) {
       __builtin_nontemporal_store(xg_u64x2{src[s], src[s + plane]}, (xg_u64x2 *)dst + x);
     } else {
       dst[2 * x] = src[s];
       dst[2 * x + 1] = src[s + plane];
+    }
+  }
+}
+
+// Wide dump of a spilled integration in one pass: dst (triangular, W) =
+// spill (split planes, W) + cur (split planes, S), one thread per visibility;
+// PAIR as above (dst 16-byte aligned).
+template <typename W, typename S, bool PAIR>
+__global__ void xg_reorder_combine_k(int src_order, W *dst, const W *spill, const S *cur,
+                                     int nstation, int nfreq, long long plane) {
+  const long long NB = (long long)nstation * (nstation + 1) / 2;
+  const long long total = (long long)nfreq * NB * 4;
+  const long long P = nstation >> 1;
+  const long long RT = P * (P + 1) / 2;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
+    const long long s = xg_reorder_src(src_order, x, NB, RT);
+    const W re = spill[s] + (W)cur[s];
+    const W im = spill[s + plane] + (W)cur[s + plane];
+    if constexpr (PAIR) {
+      typedef W W2 __attribute__((ext_vector_type(2)));
+      __builtin_nontemporal_store(W2{re, im}, (W2 *)dst + x);
+    } else {
+      dst[2 * x] = re;
+      dst[2 * x + 1] = im;
     }
   }
 }
@@ -2121,6 +2150,37 @@ extern "C" int xgh_launch_reorder(int src_order, int elem_bytes, void *dst, cons
   else
     return xgh_report(hipErrorInvalidValue, "xgh_launch_reorder(elem)");
   return xgh_report(hipGetLastError(), "xg_reorder launch");
+}
+
+extern "C" int xgh_launch_reorder_combine(int src_order, int kind, void *dst, const void *spill,
+                                          const void *cur, int nstation, int nfreq,
+                                          long long plane, void *stream) {
+  const long long total = (long long)nfreq * nstation * (nstation + 1) / 2 * 4;
+  if (total <= 0) return 0;
+  if (src_order != XG_ORDER_REGTILE && src_order != XG_ORDER_REALIMAG)
+    return xgh_report(hipErrorInvalidValue, "xgh_launch_reorder_combine(order)");
+  const dim3 g(grid_for((size_t)total, 256)), b(256);
+  const bool pair = !((uintptr_t)dst & 15);
+  hipStream_t st = (hipStream_t)stream;
+  if (kind == XG_OUT_I64 && pair)
+    hipLaunchKernelGGL((xg_reorder_combine_k<long long, int, true>), g, b, 0, st, src_order,
+                       (long long *)dst, (const long long *)spill, (const int *)cur, nstation,
+                       nfreq, plane);
+  else if (kind == XG_OUT_I64)
+    hipLaunchKernelGGL((xg_reorder_combine_k<long long, int, false>), g, b, 0, st, src_order,
+                       (long long *)dst, (const long long *)spill, (const int *)cur, nstation,
+                       nfreq, plane);
+  else if (kind == XG_OUT_F64 && pair)
+    hipLaunchKernelGGL((xg_reorder_combine_k<double, float, true>), g, b, 0, st, src_order,
+                       (double *)dst, (const double *)spill, (const float *)cur, nstation, nfreq,
+                       plane);
+  else if (kind == XG_OUT_F64)
+    hipLaunchKernelGGL((xg_reorder_combine_k<double, float, false>), g, b, 0, st, src_order,
+                       (double *)dst, (const double *)spill, (const float *)cur, nstation, nfreq,
+                       plane);
+  else
+    return xgh_report(hipErrorInvalidValue, "xgh_launch_reorder_combine(kind)");
+  return xgh_report(hipGetLastError(), "xg_reorder_combine launch");
 }
 
 extern "C" int xgh_launch_swizzle(void *dst, const void *src, int nstation, int nfreq, int ntime,
