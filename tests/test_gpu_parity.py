@@ -202,9 +202,9 @@ def test_i8_device_reorder_dump():
 
 
 def test_i8_spilled_wide_triangular_dump():
-    """A spilled register-tile integration dumped wide in triangular order
-    (spill + current part reordered in one pass), to host and to device
-    memory, 16-byte aligned and 8 bytes off; the int32 dump after it."""
+    """A spilled register-tile integration dumped in triangular order (spill +
+    current part reordered in one pass), int64 and int32, to host and to
+    device memory, aligned for paired stores and off by one element."""
     ns, nf, nt = 8, 2, 40000
     x = oracle.fill_uniform(nt * nf * ns * 4, 11, -128, 127)
     flush("[i8] spilled wide triangular dump ns=8 nf=2 3x40000")
@@ -220,9 +220,14 @@ def test_i8_spilled_wide_triangular_dump():
     for off in (0, 8):
         X.check(X.core().xgpuAmdDump(eng.handle, out.ptr + off, flags))
         assert np.array_equal(out.download(np.int64, count=n, offset=off), ref), off
+    narrow = ((ref + 2**31) % 2**32) - 2**31
+    flags = X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR
+    for off in (0, 4):
+        X.check(X.core().xgpuAmdDump(eng.handle, out.ptr + off, flags))
+        assert np.array_equal(out.download(np.int32, count=n, offset=off).astype(np.int64),
+                              narrow), off
     out.free()
-    assert np.array_equal(eng.dump(triangular=True).astype(np.int64),
-                          ((ref + 2**31) % 2**32) - 2**31)
+    assert np.array_equal(eng.dump(triangular=True).astype(np.int64), narrow)
     eng.close()
 
 

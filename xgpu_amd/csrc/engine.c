@@ -430,15 +430,16 @@ static int dump_stage(XGPUAmdEngine *e, int flags, void *direct, const void **ou
   const size_t esz = wide ? 8 : 4;
   const void *val = NULL;
   int rc;
-  if (tri && wide && e->spilled && !e->cleared) { /* spill + current part, reordered in one pass */
+  if (tri && e->spilled && !e->cleared) { /* spill + current part, reordered in one pass */
     void *target = direct;
     if (!target) {
       if ((rc = ensure_scratch(e, &e->scratch2, n * 8))) return rc;
       target = e->scratch2;
     }
-    HIPCALL(xgh_launch_reorder_combine((int)e->cfg.matrix_order, e->wide_kind, target, e->wide,
-                                       e->integ, (int)e->cfg.nstation, (int)e->cfg.nfrequency,
-                                       (long long)e->info.matLength, e->stream),
+    HIPCALL(xgh_launch_reorder_combine((int)e->cfg.matrix_order, e->wide_kind, !wide, target,
+                                       e->wide, e->integ, (int)e->cfg.nstation,
+                                       (int)e->cfg.nfrequency, (long long)e->info.matLength,
+                                       e->stream),
             "dump reorder combine");
     *out = direct ? NULL : target;
     *bytes = e->info.triLength * 2 * esz;

@@ -80,9 +80,10 @@ int xgh_launch_fill_uniform_i8(void *ptr, size_t n, uint64_t seed, int lo, int h
                                void *stream);
 /* device reorder of a split-plane REGISTER_TILE or REAL_IMAG buffer into
  * interleaved TRIANGULAR order (elem: 4 = int32/f32, 8 = int64/f64) */
-/* Triangular wide dump of a spilled integration: dst = spill + cur, reordered
- * from split planes (int64 += int32, or double += float, by kind). */
-int xgh_launch_reorder_combine(int src_order, int kind, void *dst, const void *spill,
+/* Triangular dump of a spilled integration: dst = spill + cur, reordered from
+ * split planes (int64 + int32, or double + float, by kind), written wide or,
+ * with narrow, as int32 (modulo 2^32) / float. */
+int xgh_launch_reorder_combine(int src_order, int kind, int narrow, void *dst, const void *spill,
                                const void *cur, int nstation, int nfreq, long long plane,
                                void *stream);
 int xgh_launch_reorder(int src_order, int elem_bytes, void *dst, const void *src,

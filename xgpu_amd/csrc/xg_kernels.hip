@@ -1874,11 +1874,12 @@ __global__ void xg_reorder_k(int src_order, E *dst, const E *src, int nstation, 
   }
 }
 
-// Wide dump of a spilled integration in one pass: dst (triangular, W) =
+// Dump of a spilled integration in one pass: dst (triangular, O) =
 // spill (split planes, W) + cur (split planes, S), one thread per visibility;
-// PAIR as above (dst 16-byte aligned).
-template <typename W, typename S, bool PAIR>
-__global__ void xg_reorder_combine_k(int src_order, W *dst, const W *spill, const S *cur,
+// O is W (wide dump) or S (narrow: the sum modulo 2^32 for int32).  PAIR: dst
+// aligned to 2 * sizeof(O), (re, im) as one store.
+template <typename W, typename S, typename O, bool PAIR>
+__global__ void xg_reorder_combine_k(int src_order, O *dst, const W *spill, const S *cur,
                                      int nstation, int nfreq, long long plane) {
   const long long NB = (long long)nstation * (nstation + 1) / 2;
   const long long total = (long long)nfreq * NB * 4;
@@ -1887,11 +1888,11 @@ __global__ void xg_reorder_combine_k(int src_order, W *dst, const W *spill, cons
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride) {
     const long long s = xg_reorder_src(src_order, x, NB, RT);
-    const W re = spill[s] + (W)cur[s];
-    const W im = spill[s + plane] + (W)cur[s + plane];
+    const O re = (O)(spill[s] + (W)cur[s]);
+    const O im = (O)(spill[s + plane] + (W)cur[s + plane]);
     if constexpr (PAIR) {
-      typedef W W2 __attribute__((ext_vector_type(2)));
-      __builtin_nontemporal_store(W2{re, im}, (W2 *)dst + x);
+      typedef O O2 __attribute__((ext_vector_type(2)));
+      __builtin_nontemporal_store(O2{re, im}, (O2 *)dst + x);
     } else {
       dst[2 * x] = re;
       dst[2 * x + 1] = im;
@@ -2152,32 +2153,37 @@ extern "C" int xgh_launch_reorder(int src_order, int elem_bytes, void *dst, cons
   return xgh_report(hipGetLastError(), "xg_reorder launch");
 }
 
-extern "C" int xgh_launch_reorder_combine(int src_order, int kind, void *dst, const void *spill,
-                                          const void *cur, int nstation, int nfreq,
-                                          long long plane, void *stream) {
+template <typename W, typename S, typename O>
+static void reorder_combine_as(int src_order, void *dst, const void *spill, const void *cur,
+                               int nstation, int nfreq, long long plane, dim3 g, hipStream_t st) {
+  if (!((uintptr_t)dst & (2 * sizeof(O) - 1)))
+    hipLaunchKernelGGL((xg_reorder_combine_k<W, S, O, true>), g, dim3(256), 0, st, src_order,
+                       (O *)dst, (const W *)spill, (const S *)cur, nstation, nfreq, plane);
+  else
+    hipLaunchKernelGGL((xg_reorder_combine_k<W, S, O, false>), g, dim3(256), 0, st, src_order,
+                       (O *)dst, (const W *)spill, (const S *)cur, nstation, nfreq, plane);
+}
+
+extern "C" int xgh_launch_reorder_combine(int src_order, int kind, int narrow, void *dst,
+                                          const void *spill, const void *cur, int nstation,
+                                          int nfreq, long long plane, void *stream) {
   const long long total = (long long)nfreq * nstation * (nstation + 1) / 2 * 4;
   if (total <= 0) return 0;
   if (src_order != XG_ORDER_REGTILE && src_order != XG_ORDER_REALIMAG)
     return xgh_report(hipErrorInvalidValue, "xgh_launch_reorder_combine(order)");
-  const dim3 g(grid_for((size_t)total, 256)), b(256);
-  const bool pair = !((uintptr_t)dst & 15);
+  const dim3 g(grid_for((size_t)total, 256));
   hipStream_t st = (hipStream_t)stream;
-  if (kind == XG_OUT_I64 && pair)
-    hipLaunchKernelGGL((xg_reorder_combine_k<long long, int, true>), g, b, 0, st, src_order,
-                       (long long *)dst, (const long long *)spill, (const int *)cur, nstation,
-                       nfreq, plane);
+  if (kind == XG_OUT_I64 && narrow)
+    reorder_combine_as<long long, int, int>(src_order, dst, spill, cur, nstation, nfreq, plane, g, st);
   else if (kind == XG_OUT_I64)
-    hipLaunchKernelGGL((xg_reorder_combine_k<long long, int, false>), g, b, 0, st, src_order,
-                       (long long *)dst, (const long long *)spill, (const int *)cur, nstation,
-                       nfreq, plane);
-  else if (kind == XG_OUT_F64 && pair)
-    hipLaunchKernelGGL((xg_reorder_combine_k<double, float, true>), g, b, 0, st, src_order,
-                       (double *)dst, (const double *)spill, (const float *)cur, nstation, nfreq,
-                       plane);
+    reorder_combine_as<long long, int, long long>(src_order, dst, spill, cur, nstation, nfreq,
+                                                  plane, g, st);
+  else if (kind == XG_OUT_F64 && narrow)
+    reorder_combine_as<double, float, float>(src_order, dst, spill, cur, nstation, nfreq, plane,
+                                             g, st);
   else if (kind == XG_OUT_F64)
-    hipLaunchKernelGGL((xg_reorder_combine_k<double, float, false>), g, b, 0, st, src_order,
-                       (double *)dst, (const double *)spill, (const float *)cur, nstation, nfreq,
-                       plane);
+    reorder_combine_as<double, float, double>(src_order, dst, spill, cur, nstation, nfreq, plane,
+                                              g, st);
   else
     return xgh_report(hipErrorInvalidValue, "xgh_launch_reorder_combine(kind)");
   return xgh_report(hipGetLastError(), "xg_reorder_combine launch");
