@@ -85,77 +85,134 @@ print(json.dumps({"sec": ts[1], "kind": kind, "cores": len(cpus)}))
     }
 
 
-SERVE_NS = (64, 96, 128, 200, 256, 352, 512)
-SERVE_NT = (256, 512, 1024, 2048, 4096)
-
-
-def serving_bench(X, seed=5, nstreams=8, dumps_per_stream=2, load=0.7):
-    """Serving mix: 8 subarray streams (NS from SERVE_NS, NF in [8,128], block NT
-    from SERVE_NT, K in [4,64] blocks per dump), int8 blocks in pinned host
-    memory.  Phase 1 submits everything at once and measures the sustained
-    block rate; phase 2 replays Poisson arrivals at `load` x that rate and
-    reports block-to-dump latency percentiles."""
+def serving_bench(X, seed=5, dumps_per_stream=4, loads=(0.3, 0.6, 0.9)):
+    """Serving mix (xgpu_amd.workloads.serving_mix: 8 subarray streams, NS from
+    {64 .. 512}, NF in [8, 128], block NT from {256 .. 4096}, K in [4, 64]
+    blocks per dump), one uniform int8 block per stream, submitted
+    dumps_per_stream * K times.  Two sources: pinned host blocks (staged to
+    HBM per job: bound by the host link) and blocks already in HBM (a ring of
+    K copies per stream, so an integration's blocks lie back to back and
+    fuse).  Per source: everything submitted at once (sustained rate), then
+    Poisson arrivals at each `loads` fraction of that rate with block-to-dump
+    latency percentiles.  Every stream's last dump is verified: its
+    autocorrelations against K x the block's input power, and every
+    visibility against the same K blocks correlated by a lone engine."""
     import numpy as np
+    from xgpu_amd import workloads as W
 
+    streams = W.serving_mix(seed)
     rng = np.random.default_rng(seed)
-    streams = []
-    for _ in range(nstreams):
-        ns = int(rng.choice(SERVE_NS))
-        nf = int(rng.integers(8, 129))
-        nt = int(rng.choice(SERVE_NT))
-        k = int(rng.integers(4, 65))
-        streams.append((ns, nf, nt, k))
-    blocks = []
-    for i, (ns, nf, nt, k) in enumerate(streams):
+    host, ring = [], []
+    for ns, nf, nt, k in streams:
         hb = X.HostBuffer(nt * nf * ns * 4)
         hb.array[:] = rng.integers(-127, 128, hb.nbytes, dtype=np.int8)
-        blocks.append(hb)
+        host.append(hb)
+        d = X.DeviceBuffer(k * hb.nbytes)
+        X.check(X.core().xgpuAmdMemcpy(d.ptr, hb.ptr, hb.nbytes, 1))
+        for r in range(1, k):
+            X.check(X.core().xgpuAmdMemcpy(d.ptr + r * hb.nbytes, d.ptr, hb.nbytes, 3))
+        ring.append(d)
 
-    def run(arrival_rate=None):
+    def run(device, arrival_rate=None):
         srv = X.Server(staging_bytes=8 << 30)
-        ids = [srv.open_stream(ns, nf, k, dump_slots=1)[0] for ns, nf, nt, k in streams]
+        opened = [srv.open_stream(ns, nf, k, dump_slots=1) for ns, nf, nt, k in streams]
         order = []
         for i, (ns, nf, nt, k) in enumerate(streams):
             order += [i] * (k * dumps_per_stream)
         order = [order[j] for j in rng.permutation(len(order))]
-        t0 = time.perf_counter()
-        if arrival_rate:
-            gaps = rng.exponential(1.0 / arrival_rate, len(order))
-            t_next = t0
-            for i, g in zip(order, gaps):
-                t_next += g
+        count = [0] * len(streams)
+        gaps = rng.exponential(1.0 / arrival_rate, len(order)) if arrival_rate else None
+        t_next = time.perf_counter()
+        for n, i in enumerate(order):
+            if gaps is not None:
+                t_next += gaps[n]
                 delay = t_next - time.perf_counter()
                 if delay > 0:
                     time.sleep(delay)
-                srv.submit(ids[i], blocks[i], streams[i][2])
-        else:
-            for i in order:
-                srv.submit(ids[i], blocks[i], streams[i][2])
+            nt, k = streams[i][2], streams[i][3]
+            if device:
+                srv.submit_device(opened[i][0], ring[i].ptr + (count[i] % k) * host[i].nbytes, nt)
+            else:
+                srv.submit(opened[i][0], host[i], nt)
+            count[i] += 1
         srv.drain()
         st = srv.stats()
+        dumps = [r.array.copy() for _, r in opened]
         srv.close()
-        return st, len(order)
+        return st, len(order), dumps
 
-    run()  # warm-up: engines, staging, code paths
-    sat, nblocks = run()
-    rate = nblocks / sat["busy_s"]
-    lat, _ = run(arrival_rate=load * rate)
-    for hb in blocks:
+    def verify(dumps):
+        ok = True
+        for i, (ns, nf, nt, k) in enumerate(streams):
+            tri = dumps[i].reshape(nf, ns * (ns + 1) // 2, 4, 2)
+            x = host[i].array.reshape(nt, nf, ns, 2, 2).astype(np.int32)
+            power = k * (x * x).sum(axis=(0, 4), dtype=np.int64)
+            dg = np.arange(ns) * (np.arange(ns) + 1) // 2 + np.arange(ns)
+            ok &= bool(np.array_equal(tri[:, dg][:, :, [0, 3], 0], power))
+            eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
+            eng.correlate(ring[i], k * nt)
+            ok &= bool(np.array_equal(eng.dump(wide=True), dumps[i]))
+            eng.close()
+        return ok
+
+    out = {"streams": [{"nstation": ns, "nfrequency": nf, "ntime": nt, "blocks_per_dump": k}
+                       for ns, nf, nt, k in streams],
+           "dumps": "exact int64, triangular order, copied to host"}
+    for device in (False, True):
+        run(device)  # warm-up: engines, staging, code paths
+        sat, nblocks, dumps = run(device)
+        rate = nblocks / sat["busy_s"]
+        res = {"saturated": {"blocks_per_s": round(rate, 1), "tcmac_s": round(sat["tcmac_per_s"], 3),
+                             "p50_ms": round(sat["p50_ms"], 3), "p99_ms": round(sat["p99_ms"], 3),
+                             "mean_batch": round(sat["mean_batch"], 2),
+                             "mean_fused": round(sat["mean_fused"], 2)},
+               "poisson": []}
+        verified = verify(dumps)
+        for load in loads:
+            lat, _, dumps = run(device, arrival_rate=load * rate)
+            verified &= verify(dumps)
+            res["poisson"].append({"load": load, "offered_blocks_per_s": round(load * rate, 1),
+                                   "tcmac_s": round(lat["tcmac_per_s"], 3),
+                                   "p50_ms": round(lat["p50_ms"], 3), "p99_ms": round(lat["p99_ms"], 3),
+                                   "max_ms": round(lat["max_ms"], 3), "dumps": int(lat["dumps_done"]),
+                                   "mean_batch": round(lat["mean_batch"], 2),
+                                   "mean_fused": round(lat["mean_fused"], 2)})
+        res["verified"] = verified
+        key = "device_blocks" if device else "host_blocks"
+        res["input"] = ("blocks resident in HBM, a ring of K copies per stream (an integration's "
+                        "blocks back to back, fused into jobs)" if device else
+                        "pinned host blocks, uniform int8 [-127,127], staged to HBM per job")
+        out[key] = res
+        log(f"[bench] serving {key}: {res['saturated']} verified={verified}")
+    for hb in host:
         hb.free()
-    return {
-        "streams": [{"nstation": ns, "nfrequency": nf, "ntime": nt, "blocks_per_dump": k}
-                    for ns, nf, nt, k in streams],
-        "input": "pinned host blocks, uniform int8 [-127,127], staged to HBM per block",
-        "saturated": {"blocks_per_s": round(rate, 1), "tcmac_s": round(sat["tcmac_per_s"], 3),
-                      "p50_ms": round(sat["p50_ms"], 3), "p99_ms": round(sat["p99_ms"], 3),
-                      "mean_batch": round(sat["mean_batch"], 2)},
-        "poisson": {"offered_blocks_per_s": round(load * rate, 1), "load": load,
-                    "tcmac_s": round(lat["tcmac_per_s"], 3),
-                    "p50_ms": round(lat["p50_ms"], 3), "p99_ms": round(lat["p99_ms"], 3),
-                    "max_ms": round(lat["max_ms"], 3), "dumps": int(lat["dumps_done"]),
-                    "mean_batch": round(lat["mean_batch"], 2)},
-        "dumps": "exact int64, triangular order, copied to host",
-    }
+    for d in ring:
+        d.free()
+    return out
+
+
+def f64_reference(x, ns, chans):
+    """Visibilities of channels `chans` of input x (nt, nf, ns, 2, 2) in
+    float64, interleaved triangular (nchan, ns(ns+1)/2 * 8): the real Gram
+    G = Y^T Y of the channel's real rows Y (nt x 4 ns; row 2 m + c is input m's
+    re (c = 0) or im (c = 1) part), Re V(m, n) = G[mr, nr] + G[mi, ni],
+    Im V(m, n) = G[mi, nr] - G[mr, ni] (V = sum_t x_m conj(x_n), reference
+    src/omp_xengine.c:24-28).  fp16 and int8 products are exact in float64 and
+    the sums of 2048 of them round at the 2^-53 level."""
+    import numpy as np
+    nt = x.shape[0]
+    i = np.repeat(np.arange(ns), np.arange(1, ns + 1))
+    j = np.concatenate([np.arange(k + 1) for k in range(ns)])
+    out = np.empty((len(chans), ns * (ns + 1) // 2, 2, 2, 2))
+    for c, f in enumerate(chans):
+        y = np.ascontiguousarray(x[:, f]).astype(np.float64).reshape(nt, 4 * ns)
+        g = y.T @ y
+        for p1 in (0, 1):
+            for p2 in (0, 1):
+                m, n = 2 * (2 * i + p1), 2 * (2 * j + p2)
+                out[c, :, p1, p2, 0] = g[m, n] + g[m + 1, n + 1]
+                out[c, :, p1, p2, 1] = g[m + 1, n] - g[m, n + 1]
+    return out.reshape(len(chans), -1)
 
 
 def profiled_traffic(kname):
@@ -464,28 +521,54 @@ def main():
     except Exception as exc:
         cfg3 = {"error": str(exc)}
 
-    # FP16 MFMA path on its own workload (NS=256, NF=128, NT=2048, fp16 input)
-    ns16, nf16, nt16 = 256, 128, 2048
-    b16 = X.DeviceBuffer(nt16 * nf16 * ns16 * 8)
-    # the workload's distribution (i.i.d. complex Gaussian, sigma 16; its small
-    # coherent tone left out), 64 MiB drawn from a seed and tiled over the
-    # block: the kernel is power-bound, so its time depends on the data
+    # FP16 MFMA path on its own workload (NS=256, NF=128, NT=2048, fp16 input
+    # from the workload generator: complex Gaussian sigma 16 plus a delayed tone)
     import numpy as np
-    g16 = (np.random.default_rng(4).standard_normal(1 << 25, dtype=np.float32) * 16).astype(np.float16)
-    for k in range(b16.nbytes // g16.nbytes):
-        b16.upload(g16, k * g16.nbytes)
+    from xgpu_amd import workloads as W
+    ns16, nf16, nt16 = 256, 128, 2048
+    x16 = W.cfg4_input(ns16, nf16, nt16, seed=4)
+    b16 = X.DeviceBuffer.from_array(x16)
     e16 = X.Engine(ns16, nf16, compute=X.XGPU_FLOAT16, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
     # warm-up past the power-state transient (~35 ms of MFMA work)
     ms16 = e16.time_correlate(b16, nt16, warmup=100, reps=20)  # default: real-Gram rgh<32>
+    e16.clear()
+    e16.correlate(b16, nt16)
+    got16 = e16.dump(triangular=True).reshape(nf16, -1)
     e16.set_tuning(kernel=1)
     ms16_cp = e16.time_correlate(b16, nt16, warmup=3, reps=20)  # complex-plane FP16 kernel
-    # FP16 MFMA-only ceiling with operands drawn like this input
+    # FP16 MFMA-only ceiling with operands drawn like this input (its first 64 KiB)
     ceil16 = X.mfma_peak_data(b16, compute=X.XGPU_FLOAT16) / 1e12
     e16.close()
     b16.free()
     cmac16 = X.cmac_per_integration(ns16, nf16, nt16)
     t16 = cmac16 / (ms16 * 1e-3) / 1e12
     log(f"[bench] fp16 MFMA: {ms16:.3f} ms, {t16:.1f} TCMAC/s (complex-plane kernel {ms16_cp:.3f} ms)")
+    chans16 = [int(c) for c in np.linspace(0, nf16 - 1, 8)]
+    err16 = W.vis_error_stats(got16[chans16], f64_reference(x16, ns16, chans16), ns16)
+    del x16
+    # FP16 MFMA against the FP32 plain kernel on the same values: the
+    # generator rounded to int8 (exact in fp16 and in fp32), so both kernels
+    # see identical inputs; both are also compared with the exact result
+    xi8 = W.cfg4_input(ns16, nf16, nt16, seed=4, dtype=np.int8)
+    exact = f64_reference(xi8, ns16, chans16)  # integer sums < 2^53: exact
+    bi8 = X.DeviceBuffer.from_array(xi8)
+    bh = X.DeviceBuffer.from_array(xi8.astype(np.float16))
+    e32 = X.Engine(ns16, nf16, compute=X.XGPU_FLOAT32, order=X.TRIANGULAR_ORDER)
+    e32.correlate(bi8, nt16)
+    got32 = e32.dump().reshape(nf16, -1)[chans16]
+    e32.close()
+    eh = X.Engine(ns16, nf16, compute=X.XGPU_FLOAT16, order=X.TRIANGULAR_ORDER)
+    eh.correlate(bh, nt16)
+    goth = eh.dump().reshape(nf16, -1)[chans16]
+    eh.close()
+    bi8.free()
+    bh.free()
+    vs_plain = {"input": "the workload generator rounded to int8 in [-127, 127] (exact in fp16 "
+                         "and fp32), channels as above",
+                "fp16_mfma_vs_exact": W.vis_error_stats(goth, exact, ns16),
+                "fp32_plain_vs_exact": W.vis_error_stats(got32, exact, ns16),
+                "fp16_mfma_vs_fp32_plain": W.vis_error_stats(goth, got32.astype(np.float64), ns16)}
+    log(f"[bench] fp16 errors vs f64: {err16}; vs fp32 plain: {vs_plain['fp16_mfma_vs_fp32_plain']}")
 
     # batched FFTs (F-engine step), the reference FFT harness's workload:
     # 8M complex64 entries per size, in place
@@ -514,7 +597,8 @@ def main():
     if not args.no_serve:
         try:
             serving = serving_bench(X)
-            log(f"[bench] serving: {serving['saturated']} / {serving['poisson']}")
+            log(f"[bench] serving: host {serving['host_blocks']['poisson']} / "
+                f"device {serving['device_blocks']['poisson']}")
         except Exception as exc:  # reported, not fatal to the headline line
             log(f"[bench] serving failed: {exc}")
             serving = {"error": str(exc)}
@@ -555,12 +639,16 @@ def main():
         "int8_over_fp32_plain": round(value / t32, 2),
         "fft": fft,
         "fp16_mfma": {"workload": "NSTATION=256 x 2 pol, NFREQUENCY=128, NTIME=2048, fp16 input "
-                                  "(complex Gaussian sigma 16, tiled from 64 MiB)",
+                                  "(complex Gaussian sigma 16 plus a tone of amplitude 4 with a "
+                                  "random delay per station, seed 4)",
                       "kernel": "xg_xengine_rgh<32>", "tcmac_s": round(t16, 2),
                       "ms": round(ms16, 4), "complex_plane_kernel_ms": round(ms16_cp, 4),
                       "pct_fp16_peak": round(100 * t16 * 8 / PEAK_FP16_TFLOPS, 1),
                       "mfma_only_ceiling_same_input_tcmac_s": round(ceil16, 1),
-                      "pct_of_mfma_ceiling_same_input": round(100 * t16 / ceil16, 1)},
+                      "pct_of_mfma_ceiling_same_input": round(100 * t16 / ceil16, 1),
+                      "errors_vs_f64": dict(err16, channels=chans16,
+                                            reference="float64 real Gram of the same fp16 input"),
+                      "vs_fp32_plain": vs_plain},
         "roofline": {
             "bound": "mfma",
             "achieved": round(achieved_tops, 1),

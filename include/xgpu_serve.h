@@ -6,7 +6,11 @@
  * correlation blocks as they arrive.  A scheduler thread admits a block when
  * the HBM staging pool can hold its input (admission by footprint), copies it
  * to the device, and packs the oldest admitted block of each stream into one
- * persistent ragged launch (xgpuAmdCorrelateBatch, up to 16 blocks).  While a
+ * persistent ragged launch (xgpuAmdCorrelateBatch, up to 16 streams).  A
+ * stream's consecutive pending blocks of one integration are fused into one
+ * job of that launch (host blocks are staged back to back; device blocks are
+ * fused when they already lie back to back in memory), so one kernel pass
+ * integrates them and updates the stream's integration buffer once.  While a
  * batch computes, the next batch's inputs are copied and finished streams'
  * dumps are copied back, so the work of concurrent requests overlaps on the
  * device.  After blocks_per_dump blocks a stream's integration is dumped
@@ -16,8 +20,8 @@
  * block; p50/p99 are reported with the sustained CMAC/s.
  *
  * Thread safety: all functions may be called from any thread.  A submitted
- * host block must stay valid and unchanged until xgpuServerBlocksDone()
- * reports it done (or xgpuServerDrain returns).
+ * block (host or device) must stay valid and unchanged until
+ * xgpuServerBlocksDone() reports it done (or xgpuServerDrain returns).
  */
 #ifndef XGPU_SERVE_H
 #define XGPU_SERVE_H
@@ -34,7 +38,7 @@ typedef struct XGPUServerStruct XGPUServer;
 typedef struct XGPUServerConfigStruct {
   int device;
   size_t staging_bytes;  /* HBM budget for admitted input blocks (0: 4 GiB) */
-  int max_batch;         /* blocks per launch, 1..16 (0: 16) */
+  int max_batch;         /* jobs (streams) per launch, 1..16 (0: 16) */
   int max_inflight;      /* launches queued on the device at once (0: 2) */
 } XGPUServerConfig;
 
@@ -54,7 +58,8 @@ typedef struct XGPUServerStatsStruct {
   double busy_s;                    /* first submit -> last completion */
   double tcmac_per_s;               /* cmac_done / busy_s / 1e12 */
   double p50_ms, p99_ms, max_ms;    /* block-to-dump latency over dumps */
-  double mean_batch;                /* blocks per launch */
+  double mean_batch;                /* jobs (streams) per launch */
+  double mean_fused;                /* blocks per job (consecutive blocks fused) */
   size_t staging_peak_bytes;        /* high-water mark of admitted inputs */
 } XGPUServerStats;
 
@@ -73,6 +78,13 @@ size_t xgpuServerDumpBytes(XGPUServer *server, int stream_id);
 /* Enqueue one block of ntime samples ([t][f][s][p] order, host memory) for a
  * stream.  Blocks of one stream are integrated in submission order. */
 int xgpuServerSubmit(XGPUServer *server, int stream_id, const void *block_h, unsigned int ntime);
+/* The same for a block already in device memory (8-byte aligned; 16-byte
+ * aligned blocks run on the real-Gram kernels): no staging copy, and no share
+ * of the staging pool.  Consecutive blocks of a stream that follow each other
+ * in memory (block k+1 starts where block k ends, e.g. a ring of blocks in
+ * HBM) are fused into one job. */
+int xgpuServerSubmitDevice(XGPUServer *server, int stream_id, const void *block_d,
+                           unsigned int ntime);
 
 /* Wait until every submitted block and every due dump has completed. */
 int xgpuServerDrain(XGPUServer *server);

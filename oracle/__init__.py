@@ -30,6 +30,7 @@ def lib() -> C.CDLL:
             "oracle_xengine_dp4a_f32acc": [P, P, I, I, I, I],
             "oracle_xengine_i64": [P, P, I, I, I, I],
             "oracle_xengine_i64_pairs": [P, P, I, I, I, P, I, I],
+            "oracle_xengine_i64_channels": [P, P, I, I, I, P, I, I],
             "oracle_xengine_f64_f16": [P, P, I, I, I, I],
             "oracle_reorder": [I, I, P, P, I, I, LL],
             "oracle_regtile_from_tri": [I, P, P, I, I, LL],
@@ -100,6 +101,17 @@ def xengine_i64_pairs(x, ns, nf, nt, fij, nthreads=0) -> np.ndarray:
     fij = np.ascontiguousarray(fij, dtype=np.int32).reshape(-1, 3)
     out = np.empty((fij.shape[0], 4, 2), dtype=np.int64)
     lib().oracle_xengine_i64_pairs(_p(out), _p(x), ns, nf, nt, _p(fij), fij.shape[0], nthreads)
+    return out
+
+
+def xengine_i64_channels(x, ns, nf, nt, chans, nthreads=0) -> np.ndarray:
+    """Exact visibilities of whole channels: (len(chans), ns(ns+1)/2, 4, 2)
+    int64, each channel as xengine_i64 lays it out."""
+    x = _check_input(x, ns, nf, nt, np.int8)
+    ch = np.ascontiguousarray(chans, dtype=np.int32).reshape(-1)
+    assert ((ch >= 0) & (ch < nf)).all()
+    out = np.empty((ch.size, nbaseline(ns), 4, 2), dtype=np.int64)
+    lib().oracle_xengine_i64_channels(_p(out), _p(x), ns, nf, nt, _p(ch), ch.size, nthreads)
     return out
 
 

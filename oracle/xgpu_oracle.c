@@ -96,30 +96,41 @@ void oracle_xengine_dp4a_f32acc(int32_t *out, const int8_t *in, int ns, int nf, 
   }
 }
 
-/* One channel gathered input-major: [m = 2s+p][t][re,im] */
-static int8_t *gather_channel(const int8_t *in, int f, int ns, int nf, int nt) {
+/* One channel gathered input-major in int16 planes: row m = 2s+p holds the
+ * nt real parts, then the nt imaginary parts. */
+static int16_t *gather_channel(const int8_t *in, int f, int ns, int nf, int nt) {
   const size_t nin = (size_t)ns * 2;
-  int8_t *ch = (int8_t *)malloc(nin * nt * 2);
+  int16_t *ch = (int16_t *)malloc(nin * nt * 2 * sizeof(int16_t));
   if (!ch) return NULL;
   for (int t = 0; t < nt; ++t) {
     const int8_t *row = in + in_index(t, f, 0, 0, ns, nf);
     for (size_t m = 0; m < nin; ++m) {
-      ch[(m * nt + t) * 2] = row[m * 2];
-      ch[(m * nt + t) * 2 + 1] = row[m * 2 + 1];
+      ch[m * 2 * nt + t] = row[m * 2];
+      ch[m * 2 * nt + nt + t] = row[m * 2 + 1];
     }
   }
   return ch;
 }
 
-static void exact_pair(int64_t *o, const int8_t *ch, int nt, int s1, int s2) {
+/* Exact sums of one pair.  Each sample adds at most 2 * 128 * 128 = 32768 in
+ * magnitude, so int32 partial sums over 16384 samples cannot overflow; they
+ * are added into int64. */
+#define EXACT_CHUNK 16384
+static void exact_pair(int64_t *o, const int16_t *ch, int nt, int s1, int s2) {
   for (int p1 = 0; p1 < 2; ++p1)
     for (int p2 = 0; p2 < 2; ++p2) {
-      const int8_t *a = ch + (size_t)(2 * s1 + p1) * nt * 2;
-      const int8_t *b = ch + (size_t)(2 * s2 + p2) * nt * 2;
+      const int16_t *ar = ch + (size_t)(2 * s1 + p1) * 2 * nt, *ai = ar + nt;
+      const int16_t *br = ch + (size_t)(2 * s2 + p2) * 2 * nt, *bi = br + nt;
       int64_t re = 0, im = 0;
-      for (int t = 0; t < nt; ++t) {
-        re += (int32_t)a[2 * t] * b[2 * t] + (int32_t)a[2 * t + 1] * b[2 * t + 1];
-        im += (int32_t)a[2 * t + 1] * b[2 * t] - (int32_t)a[2 * t] * b[2 * t + 1];
+      for (int t0 = 0; t0 < nt; t0 += EXACT_CHUNK) {
+        const int t1 = t0 + EXACT_CHUNK < nt ? t0 + EXACT_CHUNK : nt;
+        int32_t r = 0, i = 0;
+        for (int t = t0; t < t1; ++t) {
+          r += (int32_t)ar[t] * br[t] + (int32_t)ai[t] * bi[t];
+          i += (int32_t)ai[t] * br[t] - (int32_t)ar[t] * bi[t];
+        }
+        re += r;
+        im += i;
       }
       o[(p1 * 2 + p2) * 2] = re;
       o[(p1 * 2 + p2) * 2 + 1] = im;
@@ -130,13 +141,32 @@ void oracle_xengine_i64(int64_t *out, const int8_t *in, int ns, int nf, int nt, 
   const long long nb = (long long)(ns + 1) * (ns / 2);
   set_threads(nthreads);
   for (int f = 0; f < nf; ++f) {
-    int8_t *ch = gather_channel(in, f, ns, nf, nt);
+    int16_t *ch = gather_channel(in, f, ns, nf, nt);
     if (!ch) return;
 #pragma omp parallel for schedule(dynamic, 8)
     for (long long k = 0; k < nb; ++k) {
       int s1, s2;
       tri_pair(k, &s1, &s2);
       exact_pair(out + ((size_t)f * nb + k) * 8, ch, nt, s1, s2);
+    }
+    free(ch);
+  }
+}
+
+/* Every visibility of the listed channels: out[c] is channel chans[c]'s
+ * triangle (as oracle_xengine_i64 lays out one channel). */
+void oracle_xengine_i64_channels(int64_t *out, const int8_t *in, int ns, int nf, int nt,
+                                 const int32_t *chans, int nchan, int nthreads) {
+  const long long nb = (long long)(ns + 1) * (ns / 2);
+  set_threads(nthreads);
+  for (int c = 0; c < nchan; ++c) {
+    int16_t *ch = gather_channel(in, chans[c], ns, nf, nt);
+    if (!ch) return;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (long long k = 0; k < nb; ++k) {
+      int s1, s2;
+      tri_pair(k, &s1, &s2);
+      exact_pair(out + ((size_t)c * nb + k) * 8, ch, nt, s1, s2);
     }
     free(ch);
   }

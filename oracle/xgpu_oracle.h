@@ -16,6 +16,7 @@
  *                             input, single station, -128), agreement with
  *                             the reference below 2^24
  *   oracle_xengine_i64_pairs  equals oracle_xengine_i64 on every pair
+ *   oracle_xengine_i64_channels  equals oracle_xengine_i64 on the listed channels
  *   oracle_xengine_f64_f16    numpy complex128 brute force, closed forms
  *   oracle_reorder            golden vectors of the reference xgpuReorderMatrix
  *                             (register tile and real/imag builds)
@@ -57,6 +58,11 @@ void oracle_xengine_i64(int64_t *out, const int8_t *in, int ns, int nf, int nt, 
  * out[(n*4 + p1*2+p2)*2 + {re,im}]. */
 void oracle_xengine_i64_pairs(int64_t *out, const int8_t *in, int ns, int nf, int nt,
                               const int32_t *fij, int npairs, int nthreads);
+
+/* Exact int64 visibilities of whole channels chans[0 .. nchan-1]: out holds
+ * one channel triangle after another, each laid out as in oracle_xengine_i64. */
+void oracle_xengine_i64_channels(int64_t *out, const int8_t *in, int ns, int nf, int nt,
+                                 const int32_t *chans, int nchan, int nthreads);
 
 /* FP64 X-engine of FP16 complex input (uint16 halves re,im), same layout. */
 void oracle_xengine_f64_f16(double *out, const uint16_t *in, int ns, int nf, int nt,

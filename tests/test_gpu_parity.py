@@ -15,6 +15,7 @@ import pytest
 
 import oracle
 import xgpu_amd as X
+from xgpu_amd import workloads as W
 from xgpu_amd import _lib
 from conftest import flush
 
@@ -233,8 +234,9 @@ def test_i8_spilled_wide_triangular_dump():
 
 def test_i8_full_size_cfg2_sampled():
     """NS=512 x 2 pol, NF=64, NT=8192: exact on sampled baselines of every
-    channel, every autocorrelation against the input power, and the two int8
-    kernels (complex-plane and real-Gram) bit-identical on every visibility."""
+    channel and on every visibility of 4 whole channels (int64 oracle), every
+    autocorrelation against the input power, and every int8 kernel variant
+    bit-identical to the default on every visibility."""
     ns, nf, nt = 512, 64, 8192
     flush("[i8] cfg2 full size")
     buf = X.DeviceBuffer(nt * nf * ns * 4)
@@ -251,6 +253,12 @@ def test_i8_full_size_cfg2_sampled():
     ref = oracle.xengine_i64_pairs(x, ns, nf, nt, fij)
     for n, (f, i, j) in enumerate(fij):
         assert np.array_equal(tri[f, i * (i + 1) // 2 + j], ref[n]), (f, i, j)
+    # every visibility of whole channels, the first and the last among them
+    chans = [0, 21, 42, nf - 1]
+    flush(f"[i8] cfg2 whole channels {chans}")
+    whole = oracle.xengine_i64_channels(x, ns, nf, nt, chans)
+    for c, f in enumerate(chans):
+        assert np.array_equal(tri[f], whole[c]), f
     # every autocorrelation: sum_t |x|^2 per input, from the input alone
     xt = x.reshape(nt, nf * ns * 2 * 2)
     power = np.zeros(nf * ns * 2 * 2, dtype=np.int64)
@@ -528,22 +536,15 @@ def test_autotune_keeps_results_exact(tmp_path):
 # ------------------------------------------------------------------ full-size shapes
 def test_i8_cfg3_shape_exact_by_properties():
     """NSTATION=2048 x 2 pol, NFREQUENCY=16, NTIME=4096 (1 GiB of int32 output),
-    input round(N(0, 20^2)) plus a unit tone per station (the cfg3 generator),
-    clipped to [-127, 127].  Exact on sampled baselines of every channel
-    (int64 oracle) and on every autocorrelation (input power), via the device
-    reorder to triangular order."""
+    input round(N(0, 20^2)) plus a tone of amplitude 8 with a random phase per
+    station (the cfg3 generator), clipped to [-127, 127].  Exact, via the
+    device reorder to triangular order, on sampled baselines of every channel
+    and every visibility of 4 whole channels (int64 oracle), and on every
+    autocorrelation (input power)."""
     ns, nf, nt = 2048, 16, 4096
     flush("[i8] cfg3 shape")
-    rng = np.random.default_rng(3)
-    omega, phi = 0.0123, rng.uniform(0, 2 * np.pi, ns)
-    x = np.empty((nt, nf, ns, 2, 2), dtype=np.int8)
-    t = np.arange(nt, dtype=np.float32)[:, None, None]
-    for f in range(nf):  # channel by channel to bound host memory
-        tone = 8 * np.exp(1j * (omega * t + phi[None, None, :]))
-        g = rng.normal(0, 20, (nt, ns, 2, 2)).astype(np.float32)
-        g[..., 0] += tone.real[:, 0, :, None].astype(np.float32)
-        g[..., 1] += tone.imag[:, 0, :, None].astype(np.float32)
-        x[:, f] = np.clip(np.rint(g), -127, 127).astype(np.int8)
+    x = W.cfg3_input(ns, nf, nt, seed=3)
+    rng = np.random.default_rng(33)
     x = x.reshape(-1)
     buf = X.DeviceBuffer.from_array(x)
     eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
@@ -557,6 +558,12 @@ def test_i8_cfg3_shape_exact_by_properties():
     ref = oracle.xengine_i64_pairs(x, ns, nf, nt, fij)
     for n, (f, i, j) in enumerate(fij):
         assert np.array_equal(tri[f, i * (i + 1) // 2 + j], ref[n]), (f, i, j)
+    # every visibility of whole channels, the first and the last among them
+    chans = [0, 5, 10, nf - 1]
+    flush(f"[i8] cfg3 whole channels {chans}")
+    whole = oracle.xengine_i64_channels(x, ns, nf, nt, chans)
+    for c, f in enumerate(chans):
+        assert np.array_equal(tri[f], whole[c]), f
     xt = x.reshape(nt, nf * ns * 4)
     power = np.zeros(nf * ns * 4, dtype=np.int64)
     for t0 in range(0, nt, 256):
@@ -577,44 +584,58 @@ def test_f16_cfg4_shape_errors_vs_f64():
     error |dV_ij| / sqrt(V_ii V_jj) <= (NT/16) 2^-24.  The fp16 products are
     exact in fp32 and each MFMA adds 16 of them to an fp32 accumulator, so the
     accumulation error is at most (NT/16) u sum_t |x_i x_j| with u = 2^-24, and
-    sum_t |x_i x_j| <= sqrt(V_ii V_jj) (Cauchy-Schwarz)."""
+    sum_t |x_i x_j| <= sqrt(V_ii V_jj) (Cauchy-Schwarz).  The measured figures
+    are printed (bench.py reports the same statistics)."""
     ns, nf, nt = 256, 128, 2048
     flush("[f16] cfg4 shape")
-    rng = np.random.default_rng(4)
-    tau, omega = rng.uniform(0, 8, ns), 0.0371
-    t = np.arange(nt)[:, None, None, None]
-    f = np.arange(nf)[None, :, None, None]
-    ph = 2 * np.pi * f * tau[None, None, :, None] / nf + omega * t
-    x = np.empty((nt, nf, ns, 2, 2), dtype=np.float16)
-    for t0 in range(0, nt, 256):  # in slabs to bound host memory
-        sl = slice(t0, t0 + 256)
-        g = rng.normal(0, 16, (256, nf, ns, 2, 2))
-        g[..., 0] += 4 * np.cos(ph[sl])
-        g[..., 1] += 4 * np.sin(ph[sl])
-        x[sl] = g.astype(np.float16)
+    x = W.cfg4_input(ns, nf, nt, seed=4)
     buf = X.DeviceBuffer.from_array(x)
     eng = X.Engine(ns, nf, compute=X.XGPU_FLOAT16, order=X.TRIANGULAR_ORDER)
     eng.correlate(buf, nt)
-    got = eng.dump().reshape(nf, -1, 4, 2).astype(np.float64)
+    got = eng.dump().reshape(nf, -1)
     eng.close()
     buf.free()
     chans = [0, 37, 90, 127]
     ref = oracle.xengine_f64_f16(np.ascontiguousarray(x[:, chans]), ns, len(chans), nt)
-    ref = ref.reshape(len(chans), -1, 4, 2)
-    g = got[chans]
-    dv = np.abs((g[..., 0] - ref[..., 0]) + 1j * (g[..., 1] - ref[..., 1]))
-    mag = np.abs(ref[..., 0] + 1j * ref[..., 1])
-    assert dv.max() <= 1e-5 * mag.max(), (dv.max(), mag.max())
-    rel_rms = np.sqrt(np.mean(dv ** 2)) / np.sqrt(np.mean(mag ** 2))
-    assert rel_rms <= 1e-6, rel_rms
-    i = np.repeat(np.arange(ns), np.arange(1, ns + 1))
-    j = np.concatenate([np.arange(k + 1) for k in range(ns)])
-    di = i * (i + 1) // 2 + i
-    dj = j * (j + 1) // 2 + j
-    # V_ii, V_jj per polarization pair: xx autos of (i, p1) and (j, p2)
-    auto = ref[:, :, [0, 3], 0]  # [c, baseline, (p=0, p=1)]
-    for p1 in (0, 1):
-        for p2 in (0, 1):
-            norm = np.sqrt(auto[:, di, p1] * auto[:, dj, p2])
-            worst = (dv[:, :, 2 * p1 + p2] / norm).max()
-            assert worst <= (nt / 16) * 2.0 ** -24, (p1, p2, worst)
+    st = W.vis_error_stats(got[chans], ref, ns)
+    flush(f"[f16] cfg4 errors vs f64: {st}")
+    assert st["max_abs_err"] <= 1e-5 * st["max_abs_vis"], st
+    assert st["rms_rel_err"] <= 1e-6, st
+    assert st["max_norm_err"] <= (nt / 16) * 2.0 ** -24, st
+
+
+def test_f16_against_fp32_plain_on_int8_valued_cfg4():
+    """The cfg4 generator rounded to int8 (values exact in fp16 and fp32), fed
+    to the FP32 plain kernel as int8 and to the FP16 MFMA kernel as fp16:
+    both against the exact int64 oracle on 4 channels, and against each
+    other.  Sums reach 6.6e7 > 2^24, so both round.  Bounds, u = 2^-24,
+    per polarization pair normalized by sqrt(V_ii V_jj):
+      FP16 MFMA (16 products per fp32 rounding step):  (NT/16) u,
+      FP32 plain (one rounding per FMA, 2 NT FMAs per component): 2 NT u,
+    and their difference at most the sum of the two."""
+    ns, nf, nt = 256, 128, 2048
+    flush("[f16/f32] cfg4 int8-valued")
+    x = W.cfg4_input(ns, nf, nt, seed=4, dtype=np.int8)
+    chans = [0, 64, 127]
+    exact = oracle.xengine_i64_channels(x, ns, nf, nt, chans).reshape(len(chans), -1)
+    b8 = X.DeviceBuffer.from_array(x)
+    bh = X.DeviceBuffer.from_array(x.astype(np.float16))
+    e32 = X.Engine(ns, nf, compute=X.XGPU_FLOAT32, order=X.TRIANGULAR_ORDER)
+    e32.correlate(b8, nt)
+    g32 = e32.dump().reshape(nf, -1)[chans].astype(np.float64)
+    e32.close()
+    e16 = X.Engine(ns, nf, compute=X.XGPU_FLOAT16, order=X.TRIANGULAR_ORDER)
+    e16.correlate(bh, nt)
+    g16 = e16.dump().reshape(nf, -1)[chans].astype(np.float64)
+    e16.close()
+    b8.free()
+    bh.free()
+    s16 = W.vis_error_stats(g16, exact, ns)
+    s32 = W.vis_error_stats(g32, exact, ns)
+    d = W.vis_error_stats(g16, g32, ns)
+    flush(f"[f16/f32] fp16 vs exact {s16}; fp32 plain vs exact {s32}; fp16 vs fp32 plain {d}")
+    u = 2.0 ** -24
+    assert s16["max_norm_err"] <= (nt / 16) * u, s16
+    assert s32["max_norm_err"] <= 2 * nt * u, s32
+    assert d["max_norm_err"] <= (nt / 16 + 2 * nt) * u, d
+    assert s16["rms_rel_err"] <= 1e-6 and s32["rms_rel_err"] <= 1e-5, (s16, s32)

@@ -9,6 +9,7 @@ import pytest
 
 import oracle
 import xgpu_amd as X
+from xgpu_amd import workloads as W
 from conftest import flush
 
 pytestmark = pytest.mark.gpu
@@ -103,3 +104,150 @@ def test_serving_admission_under_small_staging_pool():
         assert np.array_equal(res[k][d], e)
     for hb in blocks[k]:
         hb.free()
+
+
+# ---------------------------------------------------------------------------
+# The serving mix as configured: 8 streams, NSTATION from {64, 96, 128, 200,
+# 256, 352, 512}, NFREQUENCY in [8, 128], block NTIME from {256 .. 4096},
+# K in [4, 64] blocks per dump, one stream integrating past the int32 bound.
+# Each stream cycles through D = min(K, 3) distinct blocks (uniform int8 in
+# [-127, 127]), so its integration is sum_b count_b V(block b): the oracle
+# needs V of D blocks only.  Even streams submit host blocks (staged back to
+# back and fused), odd streams submit blocks from a device ring of 2D slots
+# (slot r holds block r % D; consecutive submissions are adjacent in memory
+# except at the wrap, so they fuse too).
+# ---------------------------------------------------------------------------
+MIX = W.serving_mix(5)
+
+
+class MixStream:
+    def __init__(self, k, ns, nf, nt, K):
+        self.k, self.ns, self.nf, self.nt, self.K = k, ns, nf, nt, K
+        self.D = min(K, 3)
+        self.nbytes = nt * nf * ns * 4
+        self.device = bool(k & 1)
+        self.host = []
+        for b in range(self.D):
+            hb = X.HostBuffer(self.nbytes)
+            hb.array[:] = oracle.fill_uniform(self.nbytes, 7000 + 10 * k + b, -127, 127)
+            self.host.append(hb)
+        self.ring = None
+        if self.device:
+            self.ring = X.DeviceBuffer(2 * self.D * self.nbytes)
+            for r in range(2 * self.D):
+                X.check(X.core().xgpuAmdMemcpy(self.ring.ptr + r * self.nbytes,
+                                               self.host[r % self.D].ptr, self.nbytes, 1))
+
+    def submit(self, srv, sid, i):
+        if self.device:
+            srv.submit_device(sid, self.ring.ptr + (i % (2 * self.D)) * self.nbytes, self.nt)
+        else:
+            srv.submit(sid, self.host[i % self.D], self.nt)
+
+    def counts(self):
+        c = [0] * self.D
+        for i in range(self.K):
+            c[i % self.D] += 1
+        return c
+
+    def free(self):
+        for hb in self.host:
+            hb.free()
+        if self.ring is not None:
+            self.ring.free()
+
+
+def serve_mix(streams, **kw):
+    """One dump per stream; submissions interleaved round-robin."""
+    srv = X.Server(**kw)
+    ids = {st.k: srv.open_stream(st.ns, st.nf, st.K, dump_slots=1) for st in streams}
+    for i in range(max(st.K for st in streams)):
+        for st in streams:
+            if i < st.K:
+                st.submit(srv, ids[st.k][0], i)
+    srv.drain()
+    out = {}
+    for st in streams:
+        sid, ring = ids[st.k]
+        assert srv.dumps_done(sid) == 1 and srv.blocks_done(sid) == st.K, st.k
+        out[st.k] = ring.array[: 2 * oracle.tri_len(st.ns, st.nf)].copy()
+    stats = srv.stats()
+    srv.close()
+    return out, stats
+
+
+def check_stream_dump(st, dump):
+    """All autocorrelations (from the input power) and every visibility of the
+    first and last channel (exact int64 oracle) of one stream's dump."""
+    ns, nf, nt = st.ns, st.nf, st.nt
+    tri = dump.reshape(nf, ns * (ns + 1) // 2, 4, 2)
+    counts = st.counts()
+    power = np.zeros((nf, ns, 2), dtype=np.int64)
+    chans = [0, nf - 1]
+    ref = np.zeros((2, ns * (ns + 1) // 2, 4, 2), dtype=np.int64)
+    for b, hb in enumerate(st.host):
+        x = hb.array.reshape(nt, nf, ns, 2, 2).astype(np.int32)
+        power += counts[b] * (x * x).sum(axis=(0, 4), dtype=np.int64)
+        ref += counts[b] * oracle.xengine_i64_channels(hb.array, ns, nf, nt, chans).reshape(ref.shape)
+    d = np.arange(ns) * (np.arange(ns) + 1) // 2 + np.arange(ns)
+    assert np.array_equal(tri[:, d][:, :, [0, 3], 0], power), st.k
+    assert not tri[:, d][:, :, [0, 3], 1].any(), st.k
+    assert np.array_equal(tri[chans], ref), st.k
+
+
+def test_serving_configured_mix_exact_and_equal_alone():
+    assert any(nt * K > W.I32_SAMPLE_BOUND for _, _, nt, K in MIX)
+    flush(f"[serve] configured mix {MIX}")
+    streams = [MixStream(k, *MIX[k]) for k in range(len(MIX))]
+    try:
+        mixed, stats = serve_mix(streams, staging_bytes=4 << 30)
+        assert stats["blocks_done"] == sum(st.K for st in streams)
+        assert stats["mean_batch"] > 1.0    # several streams share launches
+        for st in streams:
+            flush(f"[serve] check stream {st.k} {MIX[st.k]} device={st.device}")
+            check_stream_dump(st, mixed[st.k])
+        spill = [st for st in streams if st.nt * st.K > W.I32_SAMPLE_BOUND][0]
+        for st in {streams[0], streams[1], spill}:
+            flush(f"[serve] stream {st.k} alone")
+            alone, _ = serve_mix([st], staging_bytes=4 << 30)
+            assert np.array_equal(alone[st.k], mixed[st.k]), st.k
+    finally:
+        for st in streams:
+            st.free()
+
+
+def test_serving_device_blocks_fuse_only_when_adjacent():
+    """While a 128 MiB host block of another stream is staged and correlated
+    (one launch in flight at most), a stream's K device blocks queue up: back
+    to back in memory they fuse into one job, scattered they run as K jobs.
+    The dumps are the same either way."""
+    ns, nf, nt, K = 64, 4, 512, 6
+    nbytes = nt * nf * ns * 4
+    x = oracle.fill_uniform(K * nbytes, 77, -127, 127)
+    ref = sum(oracle.xengine_i64(x[i * nbytes:(i + 1) * nbytes], ns, nf, nt) for i in range(K))
+    flush("[serve] device blocks, adjacent and scattered")
+    adj = X.DeviceBuffer.from_array(x)
+    gap = X.DeviceBuffer(2 * K * nbytes)
+    for i in range(K):
+        gap.upload(x[i * nbytes:(i + 1) * nbytes], 2 * i * nbytes)
+    bns, bnf, bnt = 512, 16, 4096
+    blocker = X.HostBuffer(bnt * bnf * bns * 4)
+    blocker.array[:] = 1
+    for buf, stride, fused in ((adj, nbytes, True), (gap, 2 * nbytes, False)):
+        srv = X.Server(max_inflight=1)
+        bid, _ = srv.open_stream(bns, bnf, 1, dump_slots=1)
+        sid, ring = srv.open_stream(ns, nf, K, dump_slots=1)
+        srv.submit(bid, blocker, bnt)
+        for i in range(K):
+            srv.submit_device(sid, buf.ptr + i * stride, nt)
+        srv.drain()
+        st = srv.stats()
+        assert np.array_equal(ring.array[: ref.size], ref), stride
+        if fused:
+            assert st["mean_fused"] > 1.0, st
+        else:
+            assert st["mean_fused"] == 1.0 and st["launches"] == K + 1, st
+        srv.close()
+    blocker.free()
+    adj.free()
+    gap.free()

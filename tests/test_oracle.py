@@ -111,6 +111,26 @@ def test_xengine_i64_brute_force(ns, nf, nt, seed):
     assert np.array_equal(oracle.xengine_i64(x, ns, nf, nt), xengine_exact(x, ns, nf, nt))
 
 
+def test_xengine_i64_long_blocks_brute_force():
+    """Blocks longer than the oracle's int32 partial-sum chunk (16384
+    samples), at -128 everywhere (the largest per-sample terms) and on
+    uniform data."""
+    ns, nf, nt = 2, 1, 40000
+    x = np.full(nt * nf * ns * 4, -128, dtype=np.int8)
+    v = oracle.xengine_i64(x, ns, nf, nt).reshape(-1, 2)
+    assert np.all(v[:, 0] == nt * 2 * 128 * 128) and np.all(v[:, 1] == 0)
+    x = oracle.fill_uniform(nt * nf * ns * 4, 31, -128, 127)
+    assert np.array_equal(oracle.xengine_i64(x, ns, nf, nt), xengine_exact(x, ns, nf, nt))
+
+
+def test_xengine_i64_channels_equals_full():
+    ns, nf, nt = 20, 5, 300
+    x = oracle.fill_uniform(nt * nf * ns * 4, 32, -128, 127)
+    full = oracle.xengine_i64(x, ns, nf, nt).reshape(nf, -1, 4, 2)
+    for chans in ([0], [4, 1], [2, 2, 3]):
+        assert np.array_equal(oracle.xengine_i64_channels(x, ns, nf, nt, chans), full[chans])
+
+
 def test_xengine_i64_closed_forms():
     ns, nf, nt = 8, 2, 33
     # constant input c = a + bi on every station/pol: V = NT |c|^2 (imag 0)

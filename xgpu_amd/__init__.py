@@ -304,8 +304,14 @@ class Server:
         return sid.value, ring
 
     def submit(self, sid: int, block, ntime: int) -> None:
+        """Host block (HostBuffer or host pointer)."""
         ptr = block.ptr if isinstance(block, HostBuffer) else int(block)
         check(core().xgpuServerSubmit(self.handle, sid, ptr, ntime))
+
+    def submit_device(self, sid: int, block, ntime: int) -> None:
+        """Block already in device memory (DeviceBuffer or device pointer)."""
+        ptr = block.ptr if isinstance(block, DeviceBuffer) else int(block)
+        check(core().xgpuServerSubmitDevice(self.handle, sid, ptr, ntime))
 
     def drain(self) -> None:
         check(core().xgpuServerDrain(self.handle))

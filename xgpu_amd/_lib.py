@@ -96,7 +96,7 @@ class XGPUServerStats(C.Structure):
                 ("cmac_done", C.c_ulonglong), ("busy_s", C.c_double),
                 ("tcmac_per_s", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
                 ("max_ms", C.c_double), ("mean_batch", C.c_double),
-                ("staging_peak_bytes", C.c_size_t)]
+                ("mean_fused", C.c_double), ("staging_peak_bytes", C.c_size_t)]
 
 
 _core = None
@@ -159,6 +159,7 @@ def core() -> C.CDLL:
         _sig(lib, "xgpuServerOpenStream", I, P, C.POINTER(XGPUStreamConfig), P, U, C.POINTER(I))
         _sig(lib, "xgpuServerDumpBytes", SZ, P, I)
         _sig(lib, "xgpuServerSubmit", I, P, I, P, U)
+        _sig(lib, "xgpuServerSubmitDevice", I, P, I, P, U)
         _sig(lib, "xgpuServerDrain", I, P)
         _sig(lib, "xgpuServerBlocksDone", ULL, P, I)
         _sig(lib, "xgpuServerDumpsDone", ULL, P, I)

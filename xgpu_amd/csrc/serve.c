@@ -8,8 +8,15 @@
  *   completion  waits on the batch / dump events in issue order, frees
  *               staging, and records block-to-dump latency from device event
  *               timestamps.
- * Engines are only touched by the scheduler thread; everything shared is
- * under one mutex.
+ * A stream's consecutive pending blocks of one integration are fused into one
+ * job: host blocks are staged back to back, device blocks when they already
+ * lie back to back in memory.  One kernel pass then integrates them all and
+ * read-modify-writes the stream's integration buffer once instead of once per
+ * block (for short blocks of a large subarray that pass is most of the cost).
+ * Engines are only touched by the scheduler thread.  Locking: s->mu guards
+ * the queues, the pool, the streams' counters and the statistics; the event
+ * free list has a lock of its own (ev_mu), because the scheduler takes events
+ * after releasing s->mu while the completion thread returns them.
  */
 #include <pthread.h>
 #include <stdint.h>
@@ -26,15 +33,25 @@
 #define XS_MAX_STREAMS 64
 #define XS_MAX_RANGES 1024
 #define XS_ALIGN 256
+#define XS_MAX_FUSE 64        /* blocks fused into one job */
 
 typedef struct Block {
   int stream;
-  const void *host;
+  const void *src;         /* host block, or device block when on_device */
+  int on_device;
   unsigned ntime;
   double t_arrival;
-  size_t off, bytes;
+  size_t bytes;
   struct Block *next;      /* global arrival order */
 } Block;
+
+/* One job of a batch: nblk consecutive blocks of one stream, fused */
+typedef struct {
+  Block *blk[XS_MAX_FUSE];
+  int nblk;
+  unsigned ntime;
+  size_t off, staged;      /* staging range (staged = 0: device blocks, none) */
+} Job;
 
 typedef struct {
   XGPUAmdEngine *eng;
@@ -54,7 +71,8 @@ typedef struct Record {
   void *event;
   int n;
   int stream[XG_MAX_JOBS];
-  size_t off[XG_MAX_JOBS], bytes[XG_MAX_JOBS];
+  int nblk[XG_MAX_JOBS];   /* blocks fused into job i */
+  size_t off[XG_MAX_JOBS], bytes[XG_MAX_JOBS];  /* staging to release (bytes 0: none) */
   unsigned long long cmac;
   double t_arrival;
   int sid;
@@ -89,6 +107,7 @@ struct XGPUServerStruct {
   void *copy_stream, *compute_stream;
   void *ev_ref;
   double t_ref;
+  pthread_mutex_t ev_mu;        /* guards ev_free / nev_free only */
   void *ev_free[512];
   int nev_free;
 
@@ -98,6 +117,7 @@ struct XGPUServerStruct {
   double *lat;
   size_t nlat, caplat;
   unsigned long long blocks_submitted, blocks_done, dumps_done, launches, launched_blocks, cmac_done;
+  unsigned long long launched_jobs;
   double t_first, t_last;
 };
 
@@ -153,18 +173,24 @@ static void pool_free(XGPUServer *s, size_t off, size_t bytes) {
   }
 }
 
+/* Event free list, under ev_mu: the scheduler takes events outside s->mu
+ * while the completion thread returns them. */
 static void *get_event(XGPUServer *s) {
-  if (s->nev_free > 0) return s->ev_free[--s->nev_free];
   void *e = NULL;
+  pthread_mutex_lock(&s->ev_mu);
+  if (s->nev_free > 0) e = s->ev_free[--s->nev_free];
+  pthread_mutex_unlock(&s->ev_mu);
+  if (e) return e;
   if (xgh_event_create(&e, 1)) return NULL;
   return e;
 }
 
 static void put_event(XGPUServer *s, void *e) {
-  if (s->nev_free < (int)(sizeof s->ev_free / sizeof s->ev_free[0]))
-    s->ev_free[s->nev_free++] = e;
-  else
-    xgh_event_destroy(e);
+  pthread_mutex_lock(&s->ev_mu);
+  const int keep = s->nev_free < (int)(sizeof s->ev_free / sizeof s->ev_free[0]);
+  if (keep) s->ev_free[s->nev_free++] = e;
+  pthread_mutex_unlock(&s->ev_mu);
+  if (!keep) xgh_event_destroy(e);
 }
 
 static void push_record(XGPUServer *s, Record *r) {
@@ -185,39 +211,94 @@ static void set_error(XGPUServer *s, const char *what) {
 }
 
 /* ---- scheduler ----------------------------------------------------------- */
+/* Unlink b, whose predecessor in the pending list is prev (NULL: the head). */
+static void unlink_pending(XGPUServer *s, Block *prev, Block *b) {
+  if (prev) prev->next = b->next; else s->pend_head = b->next;
+  if (s->pend_tail == b) s->pend_tail = prev;
+}
+
+/* Form the job of the stream whose oldest pending block is `first` (with
+ * predecessor `prev`): `first` and the stream's next pending blocks, as long as
+ * they stay in the current integration (the dump after blocks_per_dump blocks
+ * closes it), fit the samples one launch may integrate (the engine's bound),
+ * have the same source kind and, for device blocks, follow each other in
+ * memory.  A host job takes one staging range for all its blocks; when that
+ * does not fit, fewer blocks are taken.  On success the blocks leave the
+ * pending list.  Returns 0 when not even `first` is admitted.  Under s->mu. */
+static int take_job(XGPUServer *s, Block *prev, Block *first, Job *job) {
+  Stream *st = &s->streams[first->stream];
+  Block *cand[XS_MAX_FUSE], *pred[XS_MAX_FUSE];
+  unsigned room = st->cfg.blocks_per_dump - st->in_integ;
+  if (room > XS_MAX_FUSE) room = XS_MAX_FUSE;
+  int n = 0;
+  unsigned ntime = 0;
+  for (Block *p = prev, *b = first; b && (unsigned)n < room; p = b, b = b->next) {
+    if (b->stream != first->stream) continue;
+    if (n > 0) {
+      const Block *last = cand[n - 1];
+      if (b->on_device != first->on_device || ntime + b->ntime > st->eng->bound) break;
+      if (b->on_device && (const char *)b->src != (const char *)last->src + last->bytes) break;
+    }
+    cand[n] = b;
+    pred[n] = p;
+    ntime += b->ntime;
+    ++n;
+  }
+  job->staged = 0;
+  job->off = 0;
+  if (!first->on_device) {
+    for (;;) {
+      size_t bytes = 0;
+      for (int k = 0; k < n; ++k) bytes += cand[k]->bytes;
+      if (pool_alloc(s, bytes, &job->off)) {
+        job->staged = bytes;
+        break;
+      }
+      if (n == 1) return 0;
+      n /= 2;
+    }
+  }
+  job->nblk = n;
+  job->ntime = 0;
+  for (int k = n - 1; k >= 0; --k) { /* later blocks first: earlier ones may be their predecessors */
+    unlink_pending(s, pred[k], cand[k]);
+    job->blk[k] = cand[k];
+    job->ntime += cand[k]->ntime;
+  }
+  return 1;
+}
+
 static void *scheduler_main(void *arg) {
   XGPUServer *s = (XGPUServer *)arg;
   xgh_set_device(s->cfg.device);
-  Block *batch[XG_MAX_JOBS];
+  Job jobs[XG_MAX_JOBS];
   pthread_mutex_lock(&s->mu);
   for (;;) {
     while (!s->stop && !s->error && (!s->pend_head || s->inflight >= s->cfg.max_inflight))
       pthread_cond_wait(&s->cv, &s->mu);
     if (s->error || (s->stop && !s->pend_head)) break;
-    /* Form a batch: the oldest pending block of each stream, one compute
-     * type, in arrival order, as long as the staging pool admits it. */
+    /* Form a batch: per stream, in order of its oldest pending block, one job
+     * of one compute type, as long as the staging pool admits it. */
     int n = 0;
-    int in_batch[XS_MAX_STREAMS] = {0};
     int seen[XS_MAX_STREAMS] = {0};
     unsigned type = 0;
     Block *prev = NULL, *b = s->pend_head;
     while (b && n < s->cfg.max_batch) {
-      Block *next = b->next;
       Stream *st = &s->streams[b->stream];
-      const int first_of_stream = !seen[b->stream];
-      seen[b->stream] = 1;
-      if (first_of_stream && !in_batch[b->stream] && (n == 0 || st->cfg.compute_type == type) &&
-          pool_alloc(s, b->bytes, &b->off)) {
-        if (n == 0) type = st->cfg.compute_type;
-        in_batch[b->stream] = 1;
-        batch[n++] = b;
-        if (prev) prev->next = next; else s->pend_head = next;
-        if (s->pend_tail == b) s->pend_tail = prev;
-        b = next;
-        continue;
+      if (!seen[b->stream] && (n == 0 || st->cfg.compute_type == type)) {
+        seen[b->stream] = 1;
+        if (take_job(s, prev, b, &jobs[n])) {
+          if (n == 0) type = st->cfg.compute_type;
+          ++n;
+          /* b and its fused blocks left the list; prev (never one of them)
+           * now precedes the next block to look at */
+          b = prev ? prev->next : s->pend_head;
+          continue;
+        }
       }
+      seen[b->stream] = 1;
       prev = b;
-      b = next;
+      b = b->next;
     }
     if (n == 0) { /* nothing fits: wait for a completion to free staging */
       if (s->inflight == 0 && s->rec_head == NULL) {
@@ -234,22 +315,34 @@ static void *scheduler_main(void *arg) {
     Record *rb = (Record *)calloc(1, sizeof(Record));
     XGPUAmdBatchItem items[XG_MAX_JOBS];
     Record *dumps[XG_MAX_JOBS];
-    int ndump = 0;
+    int ndump = 0, ncopies = 0;
     int ok = rb != NULL;
     for (int i = 0; ok && i < n; ++i) {
-      Block *blk = batch[i];
-      if (xgh_memcpy_h2d_async(s->pool + blk->off, blk->host, blk->bytes, s->copy_stream)) {
-        xg_fail(XGPU_CUDA_ERROR, "staging copy: %s", xgh_last_error());
-        ok = 0;
+      Job *jb = &jobs[i];
+      items[i].engine = s->streams[jb->blk[0]->stream].eng;
+      items[i].ntime = jb->ntime;
+      if (jb->staged) {
+        size_t at = jb->off;
+        for (int k = 0; ok && k < jb->nblk; ++k) {
+          if (xgh_memcpy_h2d_async(s->pool + at, jb->blk[k]->src, jb->blk[k]->bytes, s->copy_stream)) {
+            xg_fail(XGPU_CUDA_ERROR, "staging copy: %s", xgh_last_error());
+            ok = 0;
+          }
+          at += jb->blk[k]->bytes;
+          ++ncopies;
+        }
+        items[i].input_d = s->pool + jb->off;
+      } else {
+        items[i].input_d = jb->blk[0]->src;
       }
-      items[i].engine = s->streams[blk->stream].eng;
-      items[i].input_d = s->pool + blk->off;
-      items[i].ntime = blk->ntime;
     }
-    void *ev_copy = ok ? get_event(s) : NULL;
-    if (ok && (!ev_copy || xgh_event_record(ev_copy, s->copy_stream) ||
-               xgh_stream_wait_event(s->compute_stream, ev_copy)))
-      ok = 0;
+    void *ev_copy = NULL;
+    if (ok && ncopies) {
+      ev_copy = get_event(s);
+      if (!ev_copy || xgh_event_record(ev_copy, s->copy_stream) ||
+          xgh_stream_wait_event(s->compute_stream, ev_copy))
+        ok = 0;
+    }
     if (ok && xgpuAmdCorrelateBatch(items, n, s->compute_stream)) ok = 0;
     if (ok) {
       rb->kind = 0;
@@ -257,20 +350,24 @@ static void *scheduler_main(void *arg) {
       rb->event = get_event(s);
       if (!rb->event || xgh_event_record(rb->event, s->compute_stream)) ok = 0;
     }
+    unsigned long long nblocks = 0;
     for (int i = 0; ok && i < n; ++i) {
-      Block *blk = batch[i];
-      Stream *st = &s->streams[blk->stream];
-      rb->stream[i] = blk->stream;
-      rb->off[i] = blk->off;
-      rb->bytes[i] = blk->bytes;
-      rb->cmac += st->cmac_per_sample * blk->ntime;
-      st->in_integ++;
-      st->last_arrival = blk->t_arrival;
+      Job *jb = &jobs[i];
+      const int sid = jb->blk[0]->stream;
+      Stream *st = &s->streams[sid];
+      rb->stream[i] = sid;
+      rb->nblk[i] = jb->nblk;
+      rb->off[i] = jb->off;
+      rb->bytes[i] = jb->staged;
+      rb->cmac += st->cmac_per_sample * jb->ntime;
+      nblocks += (unsigned long long)jb->nblk;
+      st->in_integ += (unsigned)jb->nblk;
+      st->last_arrival = jb->blk[jb->nblk - 1]->t_arrival;
       if (st->in_integ == st->cfg.blocks_per_dump) {
         Record *rd = (Record *)calloc(1, sizeof(Record));
         if (!rd) { ok = 0; break; }
         rd->kind = 1;
-        rd->sid = blk->stream;
+        rd->sid = sid;
         rd->t_arrival = st->last_arrival;
         char *slot = st->ring + (st->dumps_issued % st->slots) * st->dump_bytes;
         if (xg_dump_async(st->eng, slot, XGPU_DUMP_WIDE | XGPU_DUMP_TRIANGULAR)) { free(rd); ok = 0; break; }
@@ -286,7 +383,8 @@ static void *scheduler_main(void *arg) {
     if (!ok) {
       set_error(s, "batch launch");
       free(rb);
-      for (int i = 0; i < n; ++i) free(batch[i]);
+      for (int i = 0; i < n; ++i)
+        for (int k = 0; k < jobs[i].nblk; ++k) free(jobs[i].blk[k]);
       break;
     }
     push_record(s, rb);
@@ -302,8 +400,10 @@ static void *scheduler_main(void *arg) {
     }
     s->inflight++;
     s->launches++;
-    s->launched_blocks += (unsigned long long)n;
-    for (int i = 0; i < n; ++i) free(batch[i]);
+    s->launched_jobs += (unsigned long long)n;
+    s->launched_blocks += nblocks;
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < jobs[i].nblk; ++k) free(jobs[i].blk[k]);
     pthread_cond_broadcast(&s->cv);
   }
   pthread_cond_broadcast(&s->cv);
@@ -332,12 +432,14 @@ static void *completion_main(void *arg) {
     if (rc) {
       set_error(s, "completion");
     } else if (r->kind == 0) {
+      unsigned long long nb = 0;
       for (int i = 0; i < r->n; ++i) {
-        pool_free(s, r->off[i], r->bytes[i]);
-        s->streams[r->stream[i]].done++;
+        if (r->bytes[i]) pool_free(s, r->off[i], r->bytes[i]);
+        s->streams[r->stream[i]].done += (unsigned long long)r->nblk[i];
+        nb += (unsigned long long)r->nblk[i];
       }
-      s->blocks_done += (unsigned long long)r->n;
-      s->pending -= (unsigned long long)r->n;
+      s->blocks_done += nb;
+      s->pending -= nb;
       s->cmac_done += r->cmac;
       s->inflight--;
       s->t_last = t;
@@ -387,9 +489,13 @@ int xgpuServerCreate(XGPUServer **out, const XGPUServerConfig *cfg) {
   if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2;
   s->cfg.staging_bytes = (s->cfg.staging_bytes + XS_ALIGN - 1) / XS_ALIGN * XS_ALIGN;
   pthread_mutex_init(&s->mu, NULL);
+  pthread_mutex_init(&s->ev_mu, NULL);
   pthread_cond_init(&s->cv, NULL);
   int ndev = 0;
   if (xgh_device_count(&ndev) || ndev == 0) {
+    pthread_cond_destroy(&s->cv);
+    pthread_mutex_destroy(&s->ev_mu);
+    pthread_mutex_destroy(&s->mu);
     free(s);
     return xg_fail(XGPU_NO_DEVICE, "no HIP device found");
   }
@@ -458,6 +564,7 @@ int xgpuServerDestroy(XGPUServer *s) {
   xgh_free(s->pool);
   free(s->lat);
   pthread_cond_destroy(&s->cv);
+  pthread_mutex_destroy(&s->ev_mu);
   pthread_mutex_destroy(&s->mu);
   free(s);
   return XGPU_OK;
@@ -508,20 +615,24 @@ size_t xgpuServerDumpBytes(XGPUServer *s, int id) {
   return s->streams[id].dump_bytes;
 }
 
-int xgpuServerSubmit(XGPUServer *s, int id, const void *block_h, unsigned ntime) {
+static int submit(XGPUServer *s, int id, const void *src, int on_device, unsigned ntime) {
   if (!s) return xg_fail(XGPU_NOT_INITIALIZED, "NULL server");
   if (id < 0 || id >= s->nstreams) return xg_fail(XGPU_INVALID_CONFIG, "bad stream id %d", id);
-  if (!block_h || ntime == 0) return xg_fail(XGPU_INVALID_CONFIG, "empty block");
+  if (!src || ntime == 0) return xg_fail(XGPU_INVALID_CONFIG, "empty block");
   Stream *st = &s->streams[id];
   if (ntime > st->eng->bound)
     return xg_fail(XGPU_INVALID_CONFIG, "block of %u samples exceeds %u", ntime, st->eng->bound);
   const size_t bytes = st->bytes_per_sample * ntime;
-  if (bytes > s->cfg.staging_bytes)
+  if (on_device) {
+    if ((uintptr_t)src & 7) return xg_fail(XGPU_INVALID_CONFIG, "device block must be 8-byte aligned");
+  } else if (bytes > s->cfg.staging_bytes) {
     return xg_fail(XGPU_OUT_OF_MEMORY, "block of %zu bytes exceeds the staging pool", bytes);
+  }
   Block *b = (Block *)calloc(1, sizeof *b);
   if (!b) return xg_fail(XGPU_OUT_OF_MEMORY, "out of host memory");
   b->stream = id;
-  b->host = block_h;
+  b->src = src;
+  b->on_device = on_device;
   b->ntime = ntime;
   b->bytes = bytes;
   pthread_mutex_lock(&s->mu);
@@ -543,6 +654,14 @@ int xgpuServerSubmit(XGPUServer *s, int id, const void *block_h, unsigned ntime)
   pthread_cond_broadcast(&s->cv);
   pthread_mutex_unlock(&s->mu);
   return XGPU_OK;
+}
+
+int xgpuServerSubmit(XGPUServer *s, int id, const void *block_h, unsigned ntime) {
+  return submit(s, id, block_h, 0, ntime);
+}
+
+int xgpuServerSubmitDevice(XGPUServer *s, int id, const void *block_d, unsigned ntime) {
+  return submit(s, id, block_d, 1, ntime);
 }
 
 unsigned long long xgpuServerBlocksDone(XGPUServer *s, int id) {
@@ -585,7 +704,8 @@ int xgpuServerStats(XGPUServer *s, XGPUServerStats *st) {
   st->cmac_done = s->cmac_done;
   st->busy_s = (s->t_last > s->t_first) ? s->t_last - s->t_first : 0.0;
   st->tcmac_per_s = st->busy_s > 0 ? (double)s->cmac_done / st->busy_s / 1e12 : 0.0;
-  st->mean_batch = s->launches ? (double)s->launched_blocks / (double)s->launches : 0.0;
+  st->mean_batch = s->launches ? (double)s->launched_jobs / (double)s->launches : 0.0;
+  st->mean_fused = s->launched_jobs ? (double)s->launched_blocks / (double)s->launched_jobs : 0.0;
   st->staging_peak_bytes = s->peak;
   double *v = NULL;
   size_t n = s->nlat;
@@ -609,7 +729,7 @@ int xgpuServerResetStats(XGPUServer *s) {
   pthread_mutex_lock(&s->mu);
   s->nlat = 0;
   s->blocks_submitted = s->blocks_done = s->dumps_done = 0;
-  s->launches = s->launched_blocks = s->cmac_done = 0;
+  s->launches = s->launched_blocks = s->launched_jobs = s->cmac_done = 0;
   s->t_first = s->t_last = 0.0;
   s->peak = s->used;
   pthread_mutex_unlock(&s->mu);

@@ -855,6 +855,11 @@ __device__ __forceinline__ void rg_finish(const XgJob &J, const TileRef &T, Acc 
     rg_epilogue(J, T, acc, wr, wc, lane, RgBperm{4 * (lane ^ 1)}, lds);
 }
 
+// Column block of the i-th of a kstep's 16 MFMAs (row block i / 4): the
+// columns run backwards on odd rows, so consecutive MFMAs share an operand
+// (one operand's bits change per MFMA instead of two every fourth).
+__device__ __forceinline__ constexpr int rg_col(int i) { return ((i >> 2) & 1) ? 3 - (i & 3) : (i & 3); }
+
 // scheduling-group masks (llvm.amdgcn.sched.group.barrier)
 #define SG_MFMA 0x008
 #define SG_VMEM_READ 0x020
@@ -976,7 +981,7 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, const 
     constexpr int kFill = 2 * C::kLoads / 8;  // staging writes (and loads) per MFMA
 #pragma unroll
     for (int i = 0; i < 16; ++i)
-      acc[i >> 2][i & 3] = E::mfma(A[i >> 2], B[i & 3], acc[i >> 2][i & 3]);
+      acc[i >> 2][rg_col(i)] = E::mfma(A[i >> 2], B[rg_col(i)], acc[i >> 2][rg_col(i)]);
     lread(nbase, NA, NB);
     fill();
 #pragma unroll
@@ -1228,7 +1233,7 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
     constexpr bool FILL = decltype(fill_c)::value;
     constexpr int kTot = 2 * C::kDma;  // DMAs of one stage, spread over MFMAs 8..15
 #pragma unroll
-    for (int i = 0; i < 8; ++i) acc[i >> 2][i & 3] = E::mfma(A[i >> 2], B[i & 3], acc[i >> 2][i & 3]);
+    for (int i = 0; i < 8; ++i) acc[i >> 2][rg_col(i)] = E::mfma(A[i >> 2], B[rg_col(i)], acc[i >> 2][rg_col(i)]);
     lread(nbase, NA, NB);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -1237,7 +1242,7 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
     }
 #pragma unroll
     for (int i = 8; i < 16; ++i) {
-      acc[i >> 2][i & 3] = E::mfma(A[i >> 2], B[i & 3], acc[i >> 2][i & 3]);
+      acc[i >> 2][rg_col(i)] = E::mfma(A[i >> 2], B[rg_col(i)], acc[i >> 2][rg_col(i)]);
       if constexpr (FILL) {
         // DMAs [(i-8) kTot / 8, (i-7) kTot / 8): every DMA exactly once
 #pragma unroll
