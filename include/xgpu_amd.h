@@ -53,6 +53,11 @@ typedef struct XGPUAmdConfigStruct {
                                        COMPLEX_BLOCK_SIZE=32 layout (per 32-entry block, the
                                        real parts of 16 stations' (X, Y), then their
                                        imaginary parts); int8 input, nstation % 16 == 0 */
+  unsigned int time_swizzle;        /* 0 = samples in order; 4 = the depth-4 time corner turn
+                                       of xgpuSwizzleInput ([t/4][f][s][p][c][t%4], the input
+                                       of the reference's DP4A build, src/cpu_util.c:235-255),
+                                       applied to the bytes of the layout above; int8 input,
+                                       blocks of a multiple of 4 samples, 16-byte aligned */
 } XGPUAmdConfig;
 
 typedef struct XGPUAmdEngineStruct XGPUAmdEngine;
@@ -144,6 +149,31 @@ int xgpuAmdMfmaPeak(int device, int reps, double *cmac_per_s);
 #define XGPU_PEAK_OPERAND_BYTES 65536
 int xgpuAmdMfmaPeakData(int device, const void *operands_d, unsigned int compute_type, int reps,
                         double *cmac_per_s);
+
+/* ---- device converters ------------------------------------------------- */
+/* Device xgpuSwizzleInput (reference src/cpu_util.c:235-255): ntime samples
+ * of int8 input, plain [t][f][s][p][re,im] at src_d -> the depth-4 swizzled
+ * layout [t/4][f][s][p][c][t%4] at dst_d; with unswizzle != 0 the inverse.
+ * ntime % 4 == 0; the swizzled side 16-byte aligned, the plain side 4-byte
+ * aligned; dst_d and src_d do not overlap.  Asynchronous on `stream` (NULL:
+ * the null stream). */
+int xgpuAmdSwizzleInput(void *dst_d, const void *src_d, unsigned int nstation,
+                        unsigned int nfrequency, unsigned int ntime, int unswizzle, void *stream);
+
+/* Element kinds of the device converters */
+#define XGPU_ELEM_INT32   0
+#define XGPU_ELEM_INT64   1
+#define XGPU_ELEM_FLOAT32 2
+#define XGPU_ELEM_FLOAT64 3
+
+/* Device xgpuExtractMatrix (reference src/cpu_util.c:258-279): the packed
+ * triangular matrix (TRIANGULAR_ORDER, interleaved (re, im), nfrequency *
+ * nstation(nstation+1)/2 * 4 complex values, e.g. an XGPU_DUMP_TRIANGULAR
+ * dump) -> the full Hermitian matrix [f][i][j][p1][p2] of nfrequency *
+ * nstation^2 * 4 complex values, the upper half as conjugates.  Elements of
+ * kind XGPU_ELEM_*.  Asynchronous on `stream` (NULL: the null stream). */
+int xgpuAmdExtractMatrix(void *matrix_d, const void *packed_d, unsigned int nstation,
+                         unsigned int nfrequency, int elem_kind, void *stream);
 
 /* ---- measurement -------------------------------------------------------- */
 /* Run xgpuAmdCorrelate(engine, input_d, ntime) warmup + reps times on the

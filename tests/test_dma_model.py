@@ -1,0 +1,223 @@
+"""CPU model of the addresses the int8 DMA kernel xg_xengine_rgd issues.
+
+Restates, for every wave, lane, stage and DMA, the global buffer resource
+(base, num_records), per-lane offset and LDS window (M0 + 16 lane) of
+rgd_tile (xgpu_amd/csrc/xg_kernels.hip, the `dma` lambda), the transpose-read
+offsets of its operands (rg_lane_off), and the epilogue's output stores
+(rg_store_cells: address, bounds and triangle cut), and checks them against
+the buffers they must stay inside:
+  * every lane whose offset is in range (offset < num_records) reads 16 bytes
+    inside the input block, and only bytes of the tile's own inputs in valid
+    samples; every other lane is out of range (the hardware reads nothing);
+  * every DMA writes a 1 KiB window inside the kernel's LDS allocation, and
+    the DMAs of one stage tile that stage's buffer exactly once;
+  * every operand read stays inside the stage buffer it reads;
+  * every epilogue store lands inside the integration buffer, and each valid
+    visibility is stored exactly once (the register-tile order's unused
+    diagonal quadrant is zero-stored once, as the reference leaves it).
+The shapes include the parity case kept off the GPU (96 stations, 2
+channels, 257 samples, real/imag order) and the 512- and 2048-station shapes.
+"""
+import numpy as np
+import pytest
+
+TILE = 128          # inputs per tile side
+ROW = 256           # LDS bytes per sample per tile side (int8)
+NCHUNK = 16         # 16-byte chunks per sample row
+OOR = 0x7fffff00    # offset given to lanes whose chunk is past the station count
+CELL_GROUP, CELL_SCRATCH = 144, 16 * 144
+
+
+def dma_chunk(slot, srow):
+    x = slot ^ (srow & 7)
+    return x ^ ((x & 1) << 3)
+
+
+def rg_slot(chunk, s7):
+    return chunk ^ s7 ^ ((chunk & 1) << 3)
+
+
+def lane_off(blk, lane):
+    """rg_lane_off(2 blk, lane): per-lane LDS offset of a 32-real-row block."""
+    g, kg, r = (lane >> 4) & 1, lane >> 5, lane & 15
+    chunk = 2 * blk + g
+    return (kg * 16 + (r >> 1)) * ROW + (rg_slot(chunk, r >> 1) << 4) + ((r & 1) << 3)
+
+
+def model_dma(ns, nf, nt, f, rb, cb, stg, nbuf, stages=None):
+    """Checks the DMAs of tile (f, rb, cb) over `stages` (default: every stage
+    the kernel issues, including those past the block)."""
+    spd = 1024 // ROW                  # sample rows per wave DMA
+    kdma = stg // (4 * spd)            # DMAs per wave per side per stage
+    side_bytes, buf_bytes = stg * ROW, 2 * stg * ROW
+    lds_bytes = nbuf * buf_bytes
+    ts = nf * ns * 4                   # input bytes per sample
+    total = nt * ts
+    ninput = 2 * ns
+    diag = rb == cb
+    nstage = (nt + stg - 1) // stg
+    row_bytes = min(ts, ROW)
+    in_f = f * ns * 4
+    lanes = np.arange(64)
+    if stages is None:
+        stages = range(nstage + nbuf)
+    # LDS bytes written per stage by all waves: each stage buffer exactly once
+    cover = {st: np.zeros(buf_bytes, dtype=np.int32) for st in stages}
+    for wave in range(4):
+        srow, slot = lanes // NCHUNK, lanes % NCHUNK
+        chunk = dma_chunk(slot, wave * spd + srow)
+        dvoff = srow * ts + 16 * chunk
+        voff = {0: np.where(rb * TILE + 8 * chunk < ninput, dvoff, OOR),
+                1: np.where((not diag) & (cb * TILE + 8 * chunk < ninput), dvoff, OOR)}
+        for st in stages:
+            for side in (0, 1):
+                for j in range(kdma):
+                    g = 4 * j + wave
+                    s0 = st * stg + g * spd
+                    rows = min(nt - s0, spd)
+                    nrec = max(rows * ts - ts + row_bytes, 0)
+                    base = in_f + (cb if side else rb) * ROW + s0 * ts
+                    assert 0 <= base < (1 << 48)
+                    m0 = (st & (nbuf - 1)) * buf_bytes + side * side_bytes + g * 1024
+                    assert 0 <= m0 and m0 + 1024 <= lds_bytes
+                    rel = m0 - (st & (nbuf - 1)) * buf_bytes
+                    cover[st][rel:rel + 1024] += 1
+                    v = voff[side]
+                    live = v < nrec
+                    if not live.any():
+                        continue
+                    lo = base + v[live]
+                    assert lo.min() >= 0 and (lo + 16).max() <= total, (st, side, j)
+                    # the bytes belong to this tile side's inputs of valid samples
+                    t = lo // ts
+                    e = lo - t * ts
+                    blk = cb if side else rb
+                    first = in_f + blk * ROW
+                    assert (t < nt).all()
+                    assert ((e >= first) & (e + 16 <= first + ROW)).all()
+                    m = (e - in_f) // 2  # first of the 8 inputs (2 bytes each) read
+                    assert (m + 7 < ninput).all()
+    for st in stages:
+        assert (cover[st] == 1).all(), st
+
+
+def model_reads(stg, nbuf, diag):
+    side_bytes, buf_bytes = stg * ROW, 2 * stg * ROW
+    for wave in range(4):
+        wr, wc = wave >> 1, wave & 1
+        for lane in range(64):
+            for a in range(4):
+                for ks in range(stg // 32):
+                    o = ks * 32 * ROW + lane_off(4 * wr + a, lane)
+                    assert 0 <= o and o + 8 * ROW + 8 <= side_bytes
+                    ob = ks * 32 * ROW + lane_off(4 * wc + a, lane) + (0 if diag else side_bytes)
+                    assert 0 <= ob and ob + 8 * ROW + 8 <= buf_bytes
+
+
+def model_epilogue(ns, nf, f, rb, cb, order, accumulate, hits):
+    """rg_store_cells for every storing wave and lane of tile (f, rb, cb);
+    counts element stores in `hits` (flat output of 2 planes / interleaved)."""
+    diag = rb == cb
+    full = (not diag) and (rb * 64 + 64 <= ns)
+    nb = ns * (ns + 1) // 2
+    P = ns // 2
+    RT = P * (P + 1) // 2
+    if order == "regtile":
+        plane = nf * RT * 16
+    else:
+        plane = nf * nb * 4
+    for wave in range(4):
+        wr, wc = wave >> 1, wave & 1
+        if diag and wr < wc:
+            continue  # the mirror wave stores nothing
+        # epilogue LDS scratch of this wave
+        assert (wr * 2 + wc) * CELL_SCRATCH + 15 * CELL_GROUP + 64 + 3 * 16 + 16 <= 64 * 1024
+        for lane in range(64):
+            hi, mj, bq = lane >> 5, (lane & 31) >> 2, lane & 3
+            j = cb * 64 + 32 * wc + mj + 8 * bq
+            for a in range(4):
+                for q in range(4):
+                    i = rb * 64 + 32 * wr + hi + 8 * a + 2 * q
+                    if not full:
+                        if i >= ns or j >= ns:
+                            continue
+                        if i < j and (order != "regtile" or (i >> 1) != (j >> 1) or accumulate):
+                            continue
+                    if order == "regtile":
+                        base = 4 * ((f * 4 + 2 * (j & 1) + hi) * RT + (j >> 1))
+                        R = i >> 1
+                        p = base + 4 * (R * (R + 1) // 2)
+                        p1 = p + plane
+                        span = 4
+                    elif order == "realimag":
+                        p = 4 * (f * nb + j) + 4 * (i * (i + 1) // 2)
+                        p1 = p + plane
+                        span = 4
+                    else:  # triangular, interleaved: 8 values per cell
+                        p = 8 * (f * nb + j) + 8 * (i * (i + 1) // 2)
+                        p1 = p + 4
+                        span = 4
+                    size = 2 * plane
+                    assert 0 <= p and p + span <= size and 0 <= p1 and p1 + span <= size, (i, j)
+                    hits[p:p + span] += 1
+                    hits[p1:p1 + span] += 1
+
+
+SHAPES = [(96, 2, 257, "realimag"), (96, 2, 257, "regtile"), (96, 2, 257, "tri"),
+          (64, 1, 100, "tri"), (132, 2, 64, "regtile"), (16, 2, 64, "regtile"),
+          (256, 4, 1000, "regtile"), (6 * 4, 3, 65, "tri")]
+
+
+@pytest.mark.parametrize("ns,nf,nt,order", SHAPES)
+@pytest.mark.parametrize("stg,nbuf", [(32, 8), (64, 4)])
+def test_dma_addresses_in_bounds(ns, nf, nt, order, stg, nbuf):
+    nblk = (2 * ns + TILE - 1) // TILE
+    for f in range(nf):
+        for rb in range(nblk):
+            for cb in range(rb + 1):
+                model_dma(ns, nf, nt, f, rb, cb, stg, nbuf)
+
+
+@pytest.mark.parametrize("ns,nf,nt", [(512, 64, 8192), (2048, 16, 4096)])
+def test_dma_addresses_in_bounds_large(ns, nf, nt):
+    """First and last channel, every tile, the stages at both ends of the
+    block and past it (the address arithmetic is affine in the stage)."""
+    stg, nbuf = 32, 8
+    nstage = (nt + stg - 1) // stg
+    stages = list(range(0, nbuf + 1)) + list(range(nstage - nbuf, nstage + nbuf))
+    nblk = (2 * ns + TILE - 1) // TILE
+    for f in (0, nf - 1):
+        for rb in range(nblk):
+            for cb in range(rb + 1):
+                if nblk > 8 and rb not in (0, 1, nblk - 2, nblk - 1):
+                    continue
+                model_dma(ns, nf, nt, f, rb, cb, stg, nbuf, stages)
+
+
+@pytest.mark.parametrize("stg,nbuf", [(32, 8), (64, 4)])
+def test_operand_reads_inside_stage_buffer(stg, nbuf):
+    for diag in (False, True):
+        model_reads(stg, nbuf, diag)
+
+
+@pytest.mark.parametrize("ns,nf,order,acc", [(96, 2, "realimag", 0), (96, 2, "regtile", 0),
+                                             (96, 2, "tri", 1), (132, 2, "regtile", 0),
+                                             (132, 2, "regtile", 1), (64, 1, "tri", 0),
+                                             (6 * 4, 3, "tri", 0), (16, 2, "realimag", 1)])
+def test_epilogue_stores_in_bounds_and_exactly_once(ns, nf, order, acc):
+    nb = ns * (ns + 1) // 2
+    P = ns // 2
+    plane = nf * P * (P + 1) // 2 * 16 if order == "regtile" else nf * nb * 4
+    hits = np.zeros(2 * plane, dtype=np.int64)
+    nblk = (2 * ns + TILE - 1) // TILE
+    for f in range(nf):
+        for rb in range(nblk):
+            for cb in range(rb + 1):
+                model_epilogue(ns, nf, f, rb, cb, order, acc, hits)
+    if order == "regtile" and acc:
+        # accumulating calls skip quadrant "12" of diagonal register tiles,
+        # which holds the zeros the first call stored (reference kernel.cuh:65-71)
+        assert hits.max() == 1
+        assert (hits == 1).sum() == 2 * nf * nb * 4
+    else:
+        assert (hits == 1).all(), (int((hits == 0).sum()), int((hits > 1).sum()))

@@ -202,14 +202,16 @@ def test_i8_device_reorder_dump():
         eng.close()
 
 
-def test_i8_spilled_wide_triangular_dump():
-    """A spilled register-tile integration dumped in triangular order (spill +
-    current part reordered in one pass), int64 and int32, to host and to
-    device memory, aligned for paired stores and off by one element."""
+@pytest.mark.parametrize("order", ["regtile", "realimag"])
+def test_i8_spilled_wide_triangular_dump(order):
+    """A spilled split-plane integration (register-tile or real/imag order)
+    dumped in triangular order (spill + current part reordered in one pass),
+    int64 and int32, to host and to device memory, aligned for paired stores
+    and off by one element."""
     ns, nf, nt = 8, 2, 40000
     x = oracle.fill_uniform(nt * nf * ns * 4, 11, -128, 127)
-    flush("[i8] spilled wide triangular dump ns=8 nf=2 3x40000")
-    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    flush(f"[i8] spilled wide triangular dump ns=8 nf=2 3x40000 {order}")
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=ORDERS[order])
     buf = X.DeviceBuffer.from_array(x)
     for _ in range(3):
         eng.correlate(buf, nt)
@@ -429,6 +431,43 @@ def test_f16_spill_to_f64():
     buf.free()
     ref = oracle.xengine_f64_f16(x16, ns, nf, nt)
     assert_f16_close(wide, ref, ns, nf, nt, rms=False)
+
+
+@pytest.mark.parametrize("order", ["regtile", "realimag"])
+def test_f16_spilled_triangular_dump(order):
+    """A spilled fp16 integration (3 x 40000 samples: the fp32 part spills
+    into the FP64 one past 65536 samples) of a split-plane order dumped in
+    triangular order by the one-pass combine + reorder: double (wide) and
+    float, to host and to device memory, aligned and off by one element.
+    Against the FP64 oracle: normalized error |dV_ij| / sqrt(V_ii V_jj) <=
+    (3 NT / 16 + 4) 2^-24 (fp32 accumulation of each spill window, the
+    combine, and for the float dump its final rounding)."""
+    ns, nf, nt = 8, 2, 40000
+    x16 = f16_input(nt, nf, ns, 22, sigma=4.0)
+    flush(f"[f16] spilled triangular dump {order}")
+    eng = X.Engine(ns, nf, compute=X.XGPU_FLOAT16, order=ORDERS[order])
+    buf = X.DeviceBuffer.from_array(x16)
+    for _ in range(3):
+        eng.correlate(buf, nt)
+    ref = 3 * oracle.xengine_f64_f16(x16, ns, nf, nt)
+    bound = (3 * nt / 16 + 4) * 2.0 ** -24
+    outs = [eng.dump(wide=True, triangular=True), eng.dump(triangular=True)]
+    n = 2 * eng.tri_len
+    out = X.DeviceBuffer(n * 8 + 16)
+    for wide, dt, offs in ((True, np.float64, (0, 8)), (False, np.float32, (0, 4))):
+        flags = X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR | (X.XGPU_DUMP_WIDE if wide else 0)
+        for off in offs:
+            X.check(X.core().xgpuAmdDump(eng.handle, out.ptr + off, flags))
+            outs.append(out.download(dt, count=n, offset=off))
+    out.free()
+    eng.close()
+    buf.free()
+    for k, got in enumerate(outs):
+        st = W.vis_error_stats(got, ref, ns)
+        assert st["max_norm_err"] <= bound, (k, st)
+    # host and device dumps of one kind are the same values
+    assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[0], outs[3])
+    assert np.array_equal(outs[1], outs[4]) and np.array_equal(outs[1], outs[5])
 
 
 # ------------------------------------------------------------------ batching

@@ -243,10 +243,12 @@ def test_serving_device_blocks_fuse_only_when_adjacent():
         srv.drain()
         st = srv.stats()
         assert np.array_equal(ring.array[: ref.size], ref), stride
+        # scattered blocks are one job each (a job may still share a launch
+        # with the blocker's); adjacent ones fuse
         if fused:
             assert st["mean_fused"] > 1.0, st
         else:
-            assert st["mean_fused"] == 1.0 and st["launches"] == K + 1, st
+            assert st["mean_fused"] == 1.0, st
         srv.close()
     blocker.free()
     adj.free()

@@ -30,12 +30,14 @@ def cmac_per_integration(nstation: int, nfreq: int, ntime: int, npol: int = 2) -
 
 
 def _config(nstation, nfrequency, ntime=0, ntimepipe=0, compute=XGPU_INT8,
-            order=REGISTER_TILE_TRIANGULAR_ORDER, device=0, complex_block_size=1) -> XGPUAmdConfig:
+            order=REGISTER_TILE_TRIANGULAR_ORDER, device=0, complex_block_size=1,
+            time_swizzle=0) -> XGPUAmdConfig:
     c = XGPUAmdConfig()
     c.npol, c.nstation, c.nfrequency = 2, nstation, nfrequency
     c.ntime, c.ntimepipe = ntime, (ntimepipe or ntime)
     c.compute_type, c.matrix_order, c.device = compute, order, device
     c.complex_block_size = complex_block_size
+    c.time_swizzle = time_swizzle
     return c
 
 
@@ -96,11 +98,14 @@ class Engine:
     """One subarray's X-engine (runtime-sized), see include/xgpu_amd.h."""
 
     def __init__(self, nstation, nfrequency, ntime=0, ntimepipe=0, compute=XGPU_INT8,
-                 order=REGISTER_TILE_TRIANGULAR_ORDER, device=0, complex_block_size=1):
+                 order=REGISTER_TILE_TRIANGULAR_ORDER, device=0, complex_block_size=1,
+                 time_swizzle=0):
         """complex_block_size: input layout, 1 = plain [t][f][s][p], 32 = the
-        COMPLEX_BLOCK_SIZE=32 blocked layout (int8 input, nstation % 16 == 0)."""
+        COMPLEX_BLOCK_SIZE=32 blocked layout (int8 input, nstation % 16 == 0);
+        time_swizzle: 4 = input in the depth-4 time corner turn of
+        xgpuSwizzleInput (the reference's DP4A input layout)."""
         self.cfg = _config(nstation, nfrequency, ntime, ntimepipe, compute, order, device,
-                           complex_block_size)
+                           complex_block_size, time_swizzle)
         self.info = XGPUInfo()
         check(core().xgpuAmdInfo(C.byref(self.cfg), C.byref(self.info)))
         h = C.c_void_p()
@@ -240,6 +245,28 @@ def correlate_batch(items, stream=None) -> None:
         arr[k].input_d = buf.ptr if isinstance(buf, DeviceBuffer) else int(buf)
         arr[k].ntime = nt
     check(core().xgpuAmdCorrelateBatch(arr, len(items), stream))
+
+
+ELEM_INT32, ELEM_INT64, ELEM_FLOAT32, ELEM_FLOAT64 = 0, 1, 2, 3
+
+
+def _ptr(b):
+    return b.ptr if isinstance(b, (DeviceBuffer, HostBuffer)) else int(b)
+
+
+def swizzle_input(dst, src, nstation: int, nfrequency: int, ntime: int, unswizzle=False,
+                  stream=None) -> None:
+    """Device xgpuSwizzleInput (or its inverse): plain [t][f][s][p][re,im]
+    int8 input -> the depth-4 time corner turn [t/4][f][s][p][c][t%4]."""
+    check(core().xgpuAmdSwizzleInput(_ptr(dst), _ptr(src), nstation, nfrequency, ntime,
+                                     int(unswizzle), stream))
+    check(core().xgpuAmdDeviceSynchronize())
+
+
+def extract_matrix(dst, packed, nstation: int, nfrequency: int, elem: int, stream=None) -> None:
+    """Device xgpuExtractMatrix: packed triangular -> full Hermitian matrix."""
+    check(core().xgpuAmdExtractMatrix(_ptr(dst), _ptr(packed), nstation, nfrequency, elem, stream))
+    check(core().xgpuAmdDeviceSynchronize())
 
 
 FFT_NATURAL, FFT_PROTOTYPE = 0, 1

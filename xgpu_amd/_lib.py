@@ -67,6 +67,7 @@ class XGPUAmdConfig(C.Structure):
         ("npol", C.c_uint), ("nstation", C.c_uint), ("nfrequency", C.c_uint),
         ("ntime", C.c_uint), ("ntimepipe", C.c_uint), ("compute_type", C.c_uint),
         ("matrix_order", C.c_uint), ("device", C.c_int), ("complex_block_size", C.c_uint),
+        ("time_swizzle", C.c_uint),
     ]
 
 
@@ -144,6 +145,8 @@ def core() -> C.CDLL:
         _sig(lib, "xgpuAmdMemcpy", I, P, P, SZ, I)
         _sig(lib, "xgpuAmdDeviceSynchronize", I)
         _sig(lib, "xgpuAmdFillUniformInt8", I, P, SZ, ULL, I, I)
+        _sig(lib, "xgpuAmdSwizzleInput", I, P, P, U, U, U, I, P)
+        _sig(lib, "xgpuAmdExtractMatrix", I, P, P, U, U, I, P)
         _sig(lib, "xgpuAmdTimeCorrelate", I, P, P, U, I, I, C.POINTER(C.c_float))
         _sig(lib, "xgpuAmdMfmaPeak", I, I, I, C.POINTER(C.c_double))
         _sig(lib, "xgpuAmdMfmaPeakData", I, I, P, U, I, C.POINTER(C.c_double))

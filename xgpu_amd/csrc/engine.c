@@ -78,6 +78,12 @@ int xgpuAmdInfo(const XGPUAmdConfig *c, XGPUInfo *info) {
   if (c->complex_block_size == 32 && (c->compute_type == XGPU_FLOAT16 || c->nstation % 16))
     return xg_fail(XGPU_INVALID_CONFIG,
                    "complex_block_size 32 needs int8 input and nstation %% 16 == 0");
+  if (c->time_swizzle != 0 && c->time_swizzle != 4)
+    return xg_fail(XGPU_INVALID_CONFIG, "time_swizzle must be 0 or 4 (got %u)", c->time_swizzle);
+  if (c->time_swizzle == 4 && (c->compute_type == XGPU_FLOAT16 || (c->ntimepipe % 4) ||
+                               (c->ntime % 4)))
+    return xg_fail(XGPU_INVALID_CONFIG,
+                   "time_swizzle 4 needs int8 input and ntime, ntimepipe multiples of 4");
   if (c->ntimepipe && c->ntime % c->ntimepipe)
     return xg_fail(XGPU_INVALID_CONFIG, "ntime (%u) must be a multiple of ntimepipe (%u)",
                    c->ntime, c->ntimepipe);
@@ -169,6 +175,7 @@ int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
   e->bound = (e->family == XG_KERNEL_I8_MFMA) ? XG_I32_SAMPLE_BOUND : XG_F32_SPILL_SAMPLES;
   e->cplx_bytes = (e->family == XG_KERNEL_F16_MFMA) ? 4 : 2;
   e->block32 = (cfg->complex_block_size == 32);
+  e->swizzled = (cfg->time_swizzle == 4);
   e->tstride = (long long)cfg->nfrequency * cfg->nstation * 2 * (long long)e->cplx_bytes;
   e->integ_bytes = (size_t)info.matLength * 2 * 4;
   e->cleared = 1;
@@ -239,6 +246,8 @@ int xgpuAmdClear(XGPUAmdEngine *e) {
 unsigned long long xgpuAmdSamples(const XGPUAmdEngine *e) { return e ? e->samples : 0; }
 void *xgpuAmdStream(XGPUAmdEngine *e) { return e ? e->stream : NULL; }
 
+static void drop_graphs(XGPUAmdEngine *e);
+
 int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
   if (!e || !t) return xg_fail(XGPU_NOT_INITIALIZED, "NULL argument");
   if (t->wg_per_cu < 0 || t->wg_per_cu > 2)
@@ -267,6 +276,8 @@ int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
                      "stage must be 0, or 64 / 128 with register staging, 32 / 64 with DMA staging");
     stage = t->stage ? t->stage : (staging == XG_DEFAULT_RG_STAGING ? XG_DEFAULT_RG_STAGE : 64);
   }
+  /* graphs captured with the old kernel choice would keep replaying it */
+  drop_graphs(e);
   e->use_graph = (t->graph == 1);
   e->wg_per_cu = t->wg_per_cu ? t->wg_per_cu : 2;
   e->prefetch = t->prefetch == 0 ? XG_DEFAULT_PREFETCH : (t->prefetch == 1);
@@ -353,24 +364,56 @@ int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime) {
   return XGPU_OK;
 }
 
-/* Convert ntime samples of block-32 input into the engine's plain-layout
- * buffer on `stream`; *plain points at the result. */
+/* Forget the captured host-path graphs (they hold buffer pointers and the
+ * kernel tuning of the moment they were captured). */
+static void drop_graphs(XGPUAmdEngine *e) {
+  for (int g = 0; g < e->ngraphs; ++g) xgh_graph_destroy(e->graphs[g].exec);
+  e->ngraphs = 0;
+}
+
+/* The engine's conversion buffer, at least `bytes` long. */
+static int ensure_unblock(XGPUAmdEngine *e, size_t bytes) {
+  if (bytes <= e->unblock_bytes) return XGPU_OK;
+  if (e->unblock) { /* queued kernels, and captured graphs, may still read the old buffer */
+    HIPCALL(xgh_device_sync(), "sync");
+    drop_graphs(e);
+    xgh_free(e->unblock);
+    e->unblock = NULL;
+    e->unblock_bytes = 0;
+  }
+  HIPCALL(xgh_malloc(&e->unblock, bytes), "input conversion buffer");
+  e->unblock_bytes = bytes;
+  return XGPU_OK;
+}
+
+/* Queue on `stream` the conversion of ntime samples of this engine's input
+ * layout at src (device memory) into the plain layout at dst: unswizzle the
+ * depth-4 time corner turn, then undo the block-32 layout (in place). */
+static int convert_input(XGPUAmdEngine *e, void *dst, const void *src, unsigned ntime,
+                         void *stream) {
+  const size_t bytes = (size_t)ntime * (size_t)e->tstride;
+  if (e->swizzled) {
+    if (ntime % 4) return xg_fail(XGPU_INVALID_CONFIG, "swizzled input: ntime (%u) %% 4 != 0", ntime);
+    if (((uintptr_t)src) & 15) return xg_fail(XGPU_INVALID_CONFIG, "swizzled input must be 16-byte aligned");
+    HIPCALL(xgh_launch_swizzle(1, dst, src, (int)e->cfg.nstation, (int)e->cfg.nfrequency, (int)ntime,
+                               stream),
+            "input unswizzle");
+    src = dst;
+  }
+  if (e->block32) {
+    if (((uintptr_t)src) & 15) return xg_fail(XGPU_INVALID_CONFIG, "block-32 input must be 16-byte aligned");
+    HIPCALL(xgh_launch_unblock32(dst, src, bytes, stream), "block-32 conversion");
+  }
+  return XGPU_OK;
+}
+
+/* Convert ntime samples of device input in the engine's layout into the
+ * engine's plain-layout buffer on `stream`; *plain points at the result. */
 static int unblock_input(XGPUAmdEngine *e, const void *input_d, unsigned ntime, void *stream,
                          const void **plain) {
-  const size_t bytes = (size_t)ntime * (size_t)e->tstride;
-  if (((uintptr_t)input_d) & 15)
-    return xg_fail(XGPU_INVALID_CONFIG, "block-32 input must be 16-byte aligned");
-  if (bytes > e->unblock_bytes) {
-    if (e->unblock) { /* queued kernels may still read the old buffer */
-      HIPCALL(xgh_device_sync(), "sync");
-      xgh_free(e->unblock);
-      e->unblock = NULL;
-      e->unblock_bytes = 0;
-    }
-    HIPCALL(xgh_malloc(&e->unblock, bytes), "block-32 conversion buffer");
-    e->unblock_bytes = bytes;
-  }
-  HIPCALL(xgh_launch_unblock32(e->unblock, input_d, bytes, stream), "block-32 conversion");
+  int rc = ensure_unblock(e, (size_t)ntime * (size_t)e->tstride);
+  if (rc) return rc;
+  if ((rc = convert_input(e, e->unblock, input_d, ntime, stream))) return rc;
   *plain = e->unblock;
   return XGPU_OK;
 }
@@ -403,7 +446,7 @@ int xgpuAmdCorrelate(XGPUAmdEngine *e, const void *input_d, unsigned int ntime) 
   if (!input_d && ntime) return xg_fail(XGPU_INVALID_CONFIG, "NULL input");
   if (((uintptr_t)input_d) & 7) return xg_fail(XGPU_INVALID_CONFIG, "input must be 8-byte aligned");
   HIPCALL(xgh_set_device(e->cfg.device), "set device");
-  if (e->block32 && ntime) {
+  if ((e->block32 || e->swizzled) && ntime) {
     const void *plain;
     int rc = unblock_input(e, input_d, ntime, e->stream, &plain);
     if (rc) return rc;
@@ -538,11 +581,15 @@ static int host_issue(XGPUAmdEngine *e, const void *input_h, char *dbase, int np
     HIPCALL(xgh_memcpy_h2d_async(dbase + p * chunk, hbase + p * chunk, chunk, e->copy_stream), "H2D");
     HIPCALL(xgh_event_record(e->ev_chunk[p], e->copy_stream), "record");
     HIPCALL(xgh_stream_wait_event(e->stream, e->ev_chunk[p]), "wait");
-    if (e->block32) /* the library's own buffer: convert in place */
-      HIPCALL(xgh_launch_unblock32(dbase + p * chunk, dbase + p * chunk, chunk, e->stream),
-              "block-32 conversion");
-    int rc = correlate_plain(e, dbase + p * chunk, ntp);
-    if (rc) return rc;
+    const char *in = dbase + p * chunk;
+    int rc;
+    if (e->swizzled) { /* into the conversion buffer, sized for the whole call */
+      if ((rc = convert_input(e, (char *)e->unblock + p * chunk, in, ntp, e->stream))) return rc;
+      in = (const char *)e->unblock + p * chunk;
+    } else if (e->block32) { /* the library's own buffer: convert in place */
+      if ((rc = convert_input(e, (void *)in, in, ntp, e->stream))) return rc;
+    }
+    if ((rc = correlate_plain(e, in, ntp))) return rc;
   }
   return XGPU_OK;
 }
@@ -577,6 +624,7 @@ int xgpuAmdCorrelateHost(XGPUAmdEngine *e, const void *input_h, int syncOp, void
   }
   char *dbase = (char *)e->dev_in[buf];
   int rc;
+  if (e->swizzled && (rc = ensure_unblock(e, chunk * npipe))) return rc;
   const int graphable = e->use_graph && e->samples_in_integ + e->cfg.ntime <= e->bound &&
                         (e->cfg.ntime <= e->bound);
   if (graphable) {
@@ -628,8 +676,12 @@ int xgpuAmdCorrelateHost(XGPUAmdEngine *e, const void *input_h, int syncOp, void
       e->cleared = 0;
       e->samples_in_integ += e->cfg.ntime;
       e->samples += e->cfg.ntime;
-    } else if ((rc = host_issue(e, input_h, dbase, npipe, ntp, chunk))) {
-      return rc;
+    } else {
+      /* eager issue: this buffer is free once the kernels of the call that
+       * last used it have finished */
+      if (e->ev_kernel_used[buf])
+        HIPCALL(xgh_stream_wait_event(e->copy_stream, e->ev_kernel[buf]), "wait");
+      if ((rc = host_issue(e, input_h, dbase, npipe, ntp, chunk))) return rc;
     }
   } else {
     /* the buffer is free once the kernels of the call that used it finished */
@@ -688,7 +740,7 @@ int xgpuAmdCorrelateBatch(const XGPUAmdBatchItem *items, int nitems, void *strea
       HIPCALL(xgh_stream_wait_event(s, e->ev_aux), "wait");
     }
     const void *in = items[i].input_d;
-    if (e->block32 && items[i].ntime) {
+    if ((e->block32 || e->swizzled) && items[i].ntime) {
       int rc = unblock_input(e, in, items[i].ntime, s, &in);
       if (rc) return rc;
     }
@@ -840,6 +892,36 @@ int xgpuAmdDeviceSynchronize(void) {
   HIPCALL(xgh_device_sync(), "xgpuAmdDeviceSynchronize");
   return XGPU_OK;
 }
+int xgpuAmdSwizzleInput(void *dst_d, const void *src_d, unsigned nstation, unsigned nfrequency,
+                        unsigned ntime, int unswizzle, void *stream) {
+  if (!dst_d || !src_d) return xg_fail(XGPU_INVALID_CONFIG, "NULL pointer");
+  if (ntime % 4) return xg_fail(XGPU_INVALID_CONFIG, "ntime (%u) must be a multiple of 4", ntime);
+  const void *sw = unswizzle ? src_d : dst_d, *pl = unswizzle ? dst_d : src_d;
+  if (((uintptr_t)sw & 15) || ((uintptr_t)pl & 3))
+    return xg_fail(XGPU_INVALID_CONFIG, "swizzled buffer must be 16-byte and plain buffer 4-byte aligned");
+  const size_t bytes = (size_t)ntime * nfrequency * nstation * 4;
+  if ((const char *)dst_d < (const char *)src_d + bytes && (const char *)src_d < (const char *)dst_d + bytes)
+    return xg_fail(XGPU_INVALID_CONFIG, "source and destination overlap");
+  HIPCALL(xgh_launch_swizzle(unswizzle != 0, dst_d, src_d, (int)nstation, (int)nfrequency, (int)ntime,
+                             stream),
+          "xgpuAmdSwizzleInput");
+  return XGPU_OK;
+}
+
+int xgpuAmdExtractMatrix(void *matrix_d, const void *packed_d, unsigned nstation, unsigned nfrequency,
+                         int elem_kind, void *stream) {
+  if (!matrix_d || !packed_d) return xg_fail(XGPU_INVALID_CONFIG, "NULL pointer");
+  if (elem_kind < XGPU_ELEM_INT32 || elem_kind > XGPU_ELEM_FLOAT64)
+    return xg_fail(XGPU_INVALID_CONFIG, "unknown element kind %d", elem_kind);
+  const size_t esz = (elem_kind == XGPU_ELEM_INT64 || elem_kind == XGPU_ELEM_FLOAT64) ? 8 : 4;
+  if ((((uintptr_t)matrix_d) | ((uintptr_t)packed_d)) & (esz - 1))
+    return xg_fail(XGPU_INVALID_CONFIG, "buffers must be aligned to the element size");
+  /* XGPU_ELEM_* equal the XG_OUT_* kinds of the kernels */
+  HIPCALL(xgh_launch_extract(elem_kind, matrix_d, packed_d, (int)nstation, (int)nfrequency, stream),
+          "xgpuAmdExtractMatrix");
+  return XGPU_OK;
+}
+
 int xgpuAmdFillUniformInt8(void *ptr_d, size_t nbytes, unsigned long long seed, int lo, int hi) {
   if (hi < lo || lo < -128 || hi > 127) return xg_fail(XGPU_INVALID_CONFIG, "bad range");
   HIPCALL(xgh_launch_fill_uniform_i8(ptr_d, nbytes, seed, lo, hi, NULL), "fill");

@@ -46,9 +46,12 @@ struct XGPUAmdEngineStruct {
   void *dev_in[2];
   int parity;
 
-  /* COMPLEX_BLOCK_SIZE=32 input: converted to the plain layout before the
-   * kernels (in place in dev_in on the host path, else into `unblock`) */
+  /* COMPLEX_BLOCK_SIZE=32 and / or depth-4 time-swizzled input: converted to
+   * the plain layout before the kernels.  Block-32 only: in place in dev_in on
+   * the host path, else into `unblock`; swizzled: always into `unblock`
+   * (unswizzled there, then unblocked in place) */
   int block32;
+  int swizzled;
   void *unblock;
   size_t unblock_bytes;
 

@@ -93,10 +93,16 @@ int xgh_launch_reorder(int src_order, int elem_bytes, void *dst, const void *src
 int xgh_launch_mfma_peak(int grid, int iters, int *out, void *stream);
 int xgh_launch_mfma_peak_data(int f16, int grid, int iters, const void *ops, int *out,
                               void *stream);
-/* device corner-turn to the depth-4 swizzled layout of the dp4a input
- * format: [t/4][f][s][p][c][t%4] */
-int xgh_launch_swizzle(void *dst, const void *src, int nstation, int nfreq,
+/* device corner turn between the plain input layout and the depth-4
+ * swizzled layout of the dp4a input format, [t/4][f][s][p][c][t%4]:
+ * plain -> swizzled, or swizzled -> plain when unswizzle != 0 (ntime % 4 ==
+ * 0; swizzled side 16-byte aligned, plain side 4-byte aligned) */
+int xgh_launch_swizzle(int unswizzle, void *dst, const void *src, int nstation, int nfreq,
                        int ntime, void *stream);
+/* packed triangular (interleaved complex) -> full matrix [f][i][j][p1][p2],
+ * element kind XG_OUT_* */
+int xgh_launch_extract(int kind, void *dst, const void *packed, int nstation, int nfreq,
+                       void *stream);
 /* COMPLEX_BLOCK_SIZE=32 input -> plain [t][f][s][p] input, in 64-byte blocks
  * (bytes % 64 == 0, 16-byte aligned pointers; dst may equal src) */
 int xgh_launch_unblock32(void *dst, const void *src, size_t bytes, void *stream);

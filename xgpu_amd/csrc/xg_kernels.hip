@@ -1943,21 +1943,90 @@ __global__ void xg_reorder_cells_k(int src_order, xg_u4 *dst, const unsigned *sr
   }
 }
 
-// dp4a input corner turn (reference src/cpu_util.c:235-255) on the device:
-// out byte ((((t/4*NF+f)*NS+s)*2+p)*2+c)*4 + t%4 = in byte (((t*NF+f)*NS+s)*2+p)*2+c.
-// One thread per output 32-bit word (4 consecutive samples of one component).
-__global__ void xg_swizzle_k(uint32_t *dst, const unsigned char *src, int nstation, int nfreq,
-                             int ntime) {
-  const long long row = (long long)nfreq * nstation * 4;  // bytes per time sample
-  const long long total = (long long)(ntime / 4) * row;
+// dp4a input corner turn of depth 4 (reference src/cpu_util.c:235-255) on
+// the device, both directions.  With row = bytes per time sample (NF NS 4)
+// and e a byte within a sample row, swizzled byte ((t/4) row + e) 4 + t%4 is
+// plain byte t row + e.  Thread w handles 4 consecutive bytes e = 4 e4 .. +3
+// of the 4 samples of group t4 (w = t4 row/4 + e4): plain side four 32-bit
+// words (one per sample row), swizzled side one 16-byte word (16 w), related
+// by a 4 x 4 byte transpose.  UNSWIZZLE: swizzled -> plain.
+__device__ __forceinline__ void xg_transpose4x4(const uint32_t (&in)[4], uint32_t (&out)[4]) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    out[e] = ((in[0] >> (8 * e)) & 0xffu) | (((in[1] >> (8 * e)) & 0xffu) << 8) |
+             (((in[2] >> (8 * e)) & 0xffu) << 16) | (((in[3] >> (8 * e)) & 0xffu) << 24);
+}
+
+template <bool UNSWIZZLE>
+__global__ void xg_swizzle_k(unsigned char *dst, const unsigned char *src, long long row4,
+                             long long total) {
+  const long long row = 4 * row4;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += stride) {
-    const long long t4 = w / row;
-    const long long e = w - t4 * row;
-    uint32_t v = 0;
+    const long long t4 = w / row4, e4 = w - t4 * row4;
+    unsigned char *plain = UNSWIZZLE ? dst : (unsigned char *)src;
+    const long long base = 4 * t4 * row + 4 * e4;  // plain byte of (sample 4 t4, e = 4 e4)
+    uint32_t in[4], out[4];
+    if (UNSWIZZLE) {
+      const uint4 v = ((const uint4 *)src)[w];
+      in[0] = v.x;
+      in[1] = v.y;
+      in[2] = v.z;
+      in[3] = v.w;
+    } else {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) v |= (uint32_t)src[(t4 * 4 + b) * row + e] << (8 * b);
-    dst[w] = v;
+      for (int b = 0; b < 4; ++b) in[b] = *(const uint32_t *)(plain + base + b * row);
+    }
+    xg_transpose4x4(in, out);
+    if (UNSWIZZLE) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) *(uint32_t *)(plain + base + b * row) = out[b];
+    } else {
+      ((uint4 *)dst)[w] = make_uint4(out[0], out[1], out[2], out[3]);
+    }
+  }
+}
+
+// Packed Hermitian -> full matrix (reference xgpuExtractMatrix,
+// src/cpu_util.c:258-279): out[f][i][j][p1][p2] = packed[f][k(i,j)][p1][p2]
+// for i > j and conj(packed[f][k(j,i)][p2][p1]) for i < j, k(i,j) = i(i+1)/2
+// + j, interleaved (re, im).  A diagonal cell (i = j) is written twice per
+// polarization pair by the reference loop, and its last writes leave
+// [a][b] = packed[a][b] for a > b and conj(packed[b][a]) for a <= b; the
+// same values are produced here (for Hermitian input both readings agree).
+// One thread per output cell (i, j): 8 values written contiguously, read from
+// one packed cell.  Negation as the reference does it on its element type
+// (integer negation wraps for the most negative value; floating point flips
+// the sign).
+template <typename T>
+__global__ void xg_extract_k(T *out, const T *packed, int ns, long long ncell) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long nb = (long long)ns * (ns + 1) / 2;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += stride) {
+    const long long f = c / ((long long)ns * ns);
+    const long long r = c - f * ns * ns;
+    const int i = (int)(r / ns), j = (int)(r - (long long)i * ns);
+    const bool low = i >= j;
+    const int a = low ? i : j, b = low ? j : i;
+    const T *src = packed + (f * nb + (long long)a * (a + 1) / 2 + b) * 8;
+    T v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = src[q];
+    T o[8];
+#pragma unroll
+    for (int p1 = 0; p1 < 2; ++p1)
+#pragma unroll
+      for (int p2 = 0; p2 < 2; ++p2) {
+        // straight copy below the diagonal (and in a diagonal cell below its
+        // own diagonal), conjugate of the transposed entry elsewhere
+        const bool direct = (i > j) || (i == j && p1 > p2);
+        const int d = (p1 * 2 + p2) * 2, s2 = direct ? d : (p2 * 2 + p1) * 2;
+        o[d] = v[s2];
+        o[d + 1] = direct ? v[s2 + 1] : (T)(T(0) - v[s2 + 1]);
+      }
+    T *dst = out + c * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) dst[q] = o[q];
   }
 }
 
@@ -2194,15 +2263,52 @@ extern "C" int xgh_launch_reorder_combine(int src_order, int kind, int narrow, v
   return xgh_report(hipGetLastError(), "xg_reorder_combine launch");
 }
 
-extern "C" int xgh_launch_swizzle(void *dst, const void *src, int nstation, int nfreq, int ntime,
-                                  void *stream) {
-  if (ntime % 4) return xgh_report(hipErrorInvalidValue, "xgh_launch_swizzle(ntime%4)");
-  const long long total = (long long)(ntime / 4) * nfreq * nstation * 4;
-  if (total <= 0) return 0;
-  hipLaunchKernelGGL(xg_swizzle_k, dim3(grid_for((size_t)total, 256)), dim3(256), 0,
-                     (hipStream_t)stream, (uint32_t *)dst, (const unsigned char *)src, nstation,
-                     nfreq, ntime);
+extern "C" int xgh_launch_swizzle(int unswizzle, void *dst, const void *src, int nstation,
+                                  int nfreq, int ntime, void *stream) {
+  if (ntime % 4) return xgh_report(hipErrorInvalidValue, "xgh_launch_swizzle(ntime % 4)");
+  const void *sw = unswizzle ? src : dst, *pl = unswizzle ? dst : src;
+  if (((uintptr_t)sw & 15) || ((uintptr_t)pl & 3))
+    return xgh_report(hipErrorInvalidValue, "xgh_launch_swizzle(alignment)");
+  const long long row4 = (long long)nfreq * nstation;  // 4-byte words per sample row
+  const long long total = (long long)(ntime / 4) * row4;
+  if (total == 0) return 0;
+  if (unswizzle)
+    hipLaunchKernelGGL(xg_swizzle_k<true>, dim3(grid_for((size_t)total, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (unsigned char *)dst, (const unsigned char *)src, row4,
+                       total);
+  else
+    hipLaunchKernelGGL(xg_swizzle_k<false>, dim3(grid_for((size_t)total, 256)), dim3(256), 0,
+                       (hipStream_t)stream, (unsigned char *)dst, (const unsigned char *)src, row4,
+                       total);
   return xgh_report(hipGetLastError(), "xg_swizzle launch");
+}
+
+extern "C" int xgh_launch_extract(int kind, void *dst, const void *packed, int nstation, int nfreq,
+                                  void *stream) {
+  const long long ncell = (long long)nfreq * nstation * nstation;
+  if (ncell == 0) return 0;
+  const dim3 g(grid_for((size_t)ncell, 256)), b(256);
+  hipStream_t s = (hipStream_t)stream;
+  switch (kind) {
+    case XG_OUT_I32:
+      hipLaunchKernelGGL(xg_extract_k<int>, g, b, 0, s, (int *)dst, (const int *)packed, nstation, ncell);
+      break;
+    case XG_OUT_I64:
+      hipLaunchKernelGGL(xg_extract_k<long long>, g, b, 0, s, (long long *)dst,
+                         (const long long *)packed, nstation, ncell);
+      break;
+    case XG_OUT_F32:
+      hipLaunchKernelGGL(xg_extract_k<float>, g, b, 0, s, (float *)dst, (const float *)packed,
+                         nstation, ncell);
+      break;
+    case XG_OUT_F64:
+      hipLaunchKernelGGL(xg_extract_k<double>, g, b, 0, s, (double *)dst, (const double *)packed,
+                         nstation, ncell);
+      break;
+    default:
+      return xgh_report(hipErrorInvalidValue, "xgh_launch_extract(kind)");
+  }
+  return xgh_report(hipGetLastError(), "xg_extract launch");
 }
 
 extern "C" int xgh_launch_unblock32(void *dst, const void *src, size_t bytes, void *stream) {
@@ -2316,3 +2422,4 @@ extern "C" int xgh_launch_mfma_peak(int grid, int iters, int *out, void *stream)
   hipLaunchKernelGGL(xg_mfma_peak_k, dim3(grid), dim3(256), 0, (hipStream_t)stream, iters, out);
   return xgh_report(hipGetLastError(), "xg_mfma_peak_k launch");
 }
+

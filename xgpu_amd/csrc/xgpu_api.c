@@ -13,7 +13,9 @@
  *   - the device list is printed only when XGPU_VERBOSE is set;
  *   - a DP4A build (int Complex) accumulates exactly, with an int64 spill past
  *     65535 samples; a dump reports the integration modulo 2^32, which is what
- *     int32 accumulation holds.
+ *     int32 accumulation holds.  Like the reference's DP4A build it takes the
+ *     input in the layout of xgpuSwizzleInput; the corner turn is undone on
+ *     the device before the kernels.
  */
 #include <stdint.h>
 #include <stdio.h>
@@ -162,6 +164,12 @@ int xgpuInit(XGPUContext *context, int device_flags) {
   cfg.matrix_order = MATRIX_ORDER;
   cfg.device = in->device;
   cfg.complex_block_size = COMPLEX_BLOCK_SIZE;
+#ifdef DP4A
+  /* the reference's DP4A build reads input that the caller corner-turned
+   * with xgpuSwizzleInput (src/cuda_correlator.c:172-173); the engine undoes
+   * the corner turn on the device */
+  cfg.time_swizzle = 4;
+#endif
   int rc = xgpuAmdCreate(&in->engine, &cfg);
   if (rc) {
     report(rc, "xgpuInit");
