@@ -87,6 +87,20 @@ int xgpuAmdClear(XGPUAmdEngine *engine);
  * samples) are split and spilled to int64 transparently. */
 int xgpuAmdCorrelate(XGPUAmdEngine *engine, const void *input_d, unsigned int ntime);
 
+/* xgpuAmdCorrelate without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The conversion of a
+ * block-32 or time-swizzled input layout goes through a static scratch of 64
+ * MiB per device (XG_STATIC_SCRATCH_BYTES), in chunks of whole 4-sample groups
+ * that fit it; plain input needs none.  Refused with XGPU_OUT_OF_MEMORY when
+ * the call would cross the int32 bound before the engine's spill buffer
+ * exists (an xgpuAmdCorrelate past the bound allocates it), and with
+ * XGPU_INVALID_CONFIG when 4 samples of input exceed the scratch.  Every
+ * engine on a device shares that device's scratch, which keeps the last
+ * call's contents: calls that overlap on one device (different streams, or
+ * threads) race.  Same results as xgpuAmdCorrelate. */
+int xgpuAmdCorrelateNoAlloc(XGPUAmdEngine *engine, const void *input_d, unsigned int ntime);
+
 /* Host-buffer path with the semantics of xgpuCudaXengine (reference
  * src/cuda_xengine.cu:543-677): copies cfg.ntime samples from input_h in
  * ntimepipe chunks, overlapping transfer and compute, then applies syncOp;

@@ -133,6 +133,7 @@ def core() -> C.CDLL:
         _sig(lib, "xgpuAmdDestroy", None, P)
         _sig(lib, "xgpuAmdClear", I, P)
         _sig(lib, "xgpuAmdCorrelate", I, P, P, U)
+        _sig(lib, "xgpuAmdCorrelateNoAlloc", I, P, P, U)
         _sig(lib, "xgpuAmdCorrelateHost", I, P, P, I, P)
         _sig(lib, "xgpuAmdDump", I, P, P, I)
         _sig(lib, "xgpuAmdSynchronize", I, P)

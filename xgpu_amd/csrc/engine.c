@@ -191,6 +191,7 @@ int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
   }
   if ((rc = xgh_set_device(cfg->device)) != 0) goto hip_error;
   if ((rc = xgh_device_cu_count(cfg->device, &e->cus)) != 0) goto hip_error;
+  if ((rc = xgh_static_scratch(cfg->device, &e->static_scratch)) != 0) goto hip_error;
   if ((rc = xgh_malloc(&e->integ, e->integ_bytes)) != 0) goto hip_error;
   if ((rc = xgh_stream_create(&e->stream)) != 0) goto hip_error;
   if ((rc = xgh_stream_create(&e->copy_stream)) != 0) goto hip_error;
@@ -453,6 +454,39 @@ int xgpuAmdCorrelate(XGPUAmdEngine *e, const void *input_d, unsigned int ntime) 
     input_d = plain;
   }
   return correlate_plain(e, input_d, ntime);
+}
+
+/* xgpuAmdCorrelate without allocating: the input-layout conversion goes
+ * through the device's static scratch, in chunks of whole 4-sample groups
+ * that fit it, and the call is refused when it would need the spill buffer
+ * before it exists.  The scratch is shared by every engine on the device, so
+ * calls on different streams must not overlap. */
+int xgpuAmdCorrelateNoAlloc(XGPUAmdEngine *e, const void *input_d, unsigned int ntime) {
+  if (!e) return xg_fail(XGPU_NOT_INITIALIZED, "NULL engine");
+  if (!input_d && ntime) return xg_fail(XGPU_INVALID_CONFIG, "NULL input");
+  if (((uintptr_t)input_d) & 7) return xg_fail(XGPU_INVALID_CONFIG, "input must be 8-byte aligned");
+  if (!e->wide && (unsigned long long)e->samples_in_integ + ntime > e->bound)
+    return xg_fail(XGPU_OUT_OF_MEMORY,
+                   "this call passes the int32 bound (%u samples) and needs the spill buffer, "
+                   "which xgpuAmdCorrelateNoAlloc does not allocate", e->bound);
+  HIPCALL(xgh_set_device(e->cfg.device), "set device");
+  if (!(e->block32 || e->swizzled) || ntime == 0) return correlate_plain(e, input_d, ntime);
+  const unsigned unit = e->swizzled ? 4u : 1u;
+  const size_t unit_bytes = (size_t)unit * (size_t)e->tstride;
+  if (unit_bytes > XG_STATIC_SCRATCH_BYTES)
+    return xg_fail(XGPU_INVALID_CONFIG,
+                   "%u samples of %lld bytes exceed the static scratch (%u bytes)", unit,
+                   e->tstride, XG_STATIC_SCRATCH_BYTES);
+  const unsigned chunk = (unsigned)(XG_STATIC_SCRATCH_BYTES / unit_bytes) * unit;
+  const char *src = (const char *)input_d;
+  for (unsigned done = 0; done < ntime;) {
+    const unsigned n = ntime - done < chunk ? ntime - done : chunk;
+    int rc = convert_input(e, e->static_scratch, src + (size_t)done * (size_t)e->tstride, n, e->stream);
+    if (rc) return rc;
+    if ((rc = correlate_plain(e, e->static_scratch, n))) return rc;
+    done += n;
+  }
+  return XGPU_OK;
 }
 
 static int ensure_scratch(XGPUAmdEngine *e, void **buf, size_t bytes) {

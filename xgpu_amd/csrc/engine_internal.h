@@ -54,6 +54,9 @@ struct XGPUAmdEngineStruct {
   int swizzled;
   void *unblock;
   size_t unblock_bytes;
+  /* the device's static scratch (xgh_static_scratch): conversion buffer of
+   * xgpuAmdCorrelateNoAlloc, shared by every engine on the device */
+  void *static_scratch;
 
   /* host-buffer path graphs, keyed by (input pointer, cleared state, buffer) */
   int use_graph;

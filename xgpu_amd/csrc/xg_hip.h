@@ -107,6 +107,12 @@ int xgh_launch_extract(int kind, void *dst, const void *packed, int nstation, in
  * (bytes % 64 == 0, 16-byte aligned pointers; dst may equal src) */
 int xgh_launch_unblock32(void *dst, const void *src, size_t bytes, void *stream);
 
+/* Static device scratch of XG_STATIC_SCRATCH_BYTES (a file-scope __device__
+ * array, one per device, for callers that must not allocate): its device
+ * address on `device`, looked up once per device and cached. */
+#define XG_STATIC_SCRATCH_BYTES (64u << 20)
+int xgh_static_scratch(int device, void **ptr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,7 @@
 // FP32 plain path: SIMT FP32 FMA, exact int8 -> float conversion, 8x8
 // complex register tile per thread.
 #include <hip/hip_runtime.h>
+#include <pthread.h>
 #include <stdint.h>
 
 #include "xg_hip.h"
@@ -2423,3 +2424,38 @@ extern "C" int xgh_launch_mfma_peak(int grid, int iters, int *out, void *stream)
   return xgh_report(hipGetLastError(), "xg_mfma_peak_k launch");
 }
 
+// ---------------------------------------------------------------------------
+// Static device scratch for the entry point that allocates nothing
+// (xgpuAmdCorrelateNoAlloc): the input-layout conversion buffer lives in this
+// file-scope array, one copy per device, held until the process ends.  Host
+// code never uses the array's name as a device address: its device address is
+// looked up once per device with hipGetSymbolAddress (with that device
+// current) and kept here.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[XG_STATIC_SCRATCH_BYTES];
+
+#define XG_MAX_DEVICES 64
+static void *g_scratch_ptr[XG_MAX_DEVICES];
+static pthread_mutex_t g_scratch_mu = PTHREAD_MUTEX_INITIALIZER;
+
+extern "C" int xgh_static_scratch(int device, void **ptr) {
+  if (device < 0 || device >= XG_MAX_DEVICES)
+    return xgh_report(hipErrorInvalidDevice, "xgh_static_scratch(device)");
+  pthread_mutex_lock(&g_scratch_mu);
+  int rc = 0;
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    rc = xgh_report(hipGetDevice(&cur), "hipGetDevice");
+    if (!rc) rc = xgh_report(hipSetDevice(device), "hipSetDevice");
+    if (!rc) {
+      void *p = nullptr;
+      rc = xgh_report(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+      if (!rc) g_scratch_ptr[device] = p;
+      const int rc2 = xgh_report(hipSetDevice(cur), "hipSetDevice");
+      if (!rc) rc = rc2;
+    }
+  }
+  *ptr = g_scratch_ptr[device];
+  pthread_mutex_unlock(&g_scratch_mu);
+  return rc;
+}
