@@ -237,6 +237,8 @@ def kernel_name(tun):
         return "xg_xengine_kg"
     if tun["kernel"] != 2:
         return "xg_xengine_mfma<int8>"
+    if tun["staging"] == 2 and tun.get("mfma") == 2:
+        return "xg_xengine_rgd16<4>"
     if tun["staging"] == 2:  # 128 KiB of LDS stage buffers
         return f"xg_xengine_rgd<{tun['stage']}, {65536 // (tun['stage'] * 256)}>"
     return f"xg_xengine_rg<{tun['stage']}>"
@@ -414,11 +416,12 @@ def main():
     ms_p99 = per[min(len(per) - 1, int(round(0.99 * (len(per) - 1))))]
     # the other int8 kernels on the same input, kernel time only
     variants = {}
-    for k, stg, st in ((1, 0, 0), (2, 1, 64), (2, 1, 128), (2, 2, 32), (2, 2, 64), (3, 0, 0)):
-        name = kernel_name({"kernel": k, "staging": stg, "stage": st})
+    for k, stg, st, mf in ((1, 0, 0, 0), (2, 1, 64, 0), (2, 1, 128, 0), (2, 2, 32, 0),
+                           (2, 2, 64, 0), (2, 2, 64, 2), (3, 0, 0, 0)):
+        name = kernel_name({"kernel": k, "staging": stg, "stage": st, "mfma": mf})
         if name == kname:
             continue
-        eng.set_tuning(kernel=k, staging=stg, stage=st)
+        eng.set_tuning(kernel=k, staging=stg, stage=st, mfma=mf)
         ms_v = eng.time_correlate(buf, NT, warmup=3, reps=10)
         variants[name] = {"ms": round(ms_v, 4), "tcmac_s": round(cmac / (ms_v * 1e-3) / 1e12, 1)}
         log(f"[bench] int8 {name}: {ms_v:.3f} ms kernel")

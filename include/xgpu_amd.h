@@ -212,19 +212,39 @@ typedef struct XGPUAmdTuningStruct {
   int staging;     /* int8 real-Gram kernel: 1 = global -> VGPR -> LDS, 2 = global -> LDS
                       DMA (0 = default); fp16: 1 = registers (default), 2 = DMA
                       (stage 32) */
-  int reserved[2];
+  int ntimepipe;   /* host-buffer path: samples per host -> device transfer and kernel
+                      launch (the time-pipe length; divides the config's ntime, a multiple
+                      of 4 for time-swizzled input; 0 = the config's ntimepipe) */
+  int mfma;        /* int8 real-Gram DMA kernel: MFMA tile shape, 1 = 32x32x32 (default),
+                      2 = 16x16x64 (with staging 2, stage 64) (0 = default) */
 } XGPUAmdTuning;
 int xgpuAmdSetTuning(XGPUAmdEngine *engine, const XGPUAmdTuning *tuning);
 int xgpuAmdGetTuning(const XGPUAmdEngine *engine, XGPUAmdTuning *tuning);
 
-/* Autotuner: time every candidate tuning (workgroups per CU x L2 prefetch) on
- * ntime samples of input_d with HIP events (warmup + reps calls each), keep the
- * fastest on the engine and return it with its time.  When cache_path is not
- * NULL the result is appended to that text tuning cache, keyed by device name,
+/* Autotuner of the device path: time every candidate tuning on ntime samples
+ * of input_d with HIP events (warmup + reps calls each), keep the fastest on
+ * the engine and return it with its time.  Candidates: the complex-plane
+ * kernel x workgroups per CU x L2 prefetch; the real-Gram kernel x staging
+ * (registers or global -> LDS DMA) x LDS stage depth, and for the int8 DMA
+ * kernel x MFMA tile shape (32x32x32, 16x16x64); the K-interleaved int8
+ * kernel.  Candidates run on a scratch engine of the same configuration, so
+ * the caller's integration is left as it was.  When cache_path is not NULL
+ * the result is appended to that text tuning cache, keyed by device name,
  * station count, channel count, block length and compute type, and an engine
  * created later with the same key picks it up (xgpuAmdLoadTuning). */
 int xgpuAmdAutotune(XGPUAmdEngine *engine, const void *input_d, unsigned int ntime, int warmup,
                     int reps, const char *cache_path, XGPUAmdTuning *best, float *best_ms);
+
+/* Autotuner of the host-buffer path (xgpuAmdCorrelateHost): the time-pipe
+ * length (samples per transfer and launch, the divisors of the config's
+ * ntime from ntime down to ntime/64, at least 32 samples) x hipGraph replay
+ * on or off, each timed on the wall clock over warmup + reps calls of
+ * input_h with SYNCOP_SYNC_COMPUTE, on a scratch engine of the same
+ * configuration and kernel tuning.  Keeps the fastest (ntimepipe, graph) on
+ * the engine and returns it with its time per call; cache as above (key
+ * marked as the host path). */
+int xgpuAmdAutotuneHost(XGPUAmdEngine *engine, const void *input_h, int warmup, int reps,
+                        const char *cache_path, XGPUAmdTuning *best, float *best_ms);
 /* Apply the tuning cached for this engine's key and ntime; returns
  * XGPU_INVALID_CONFIG (and leaves the engine unchanged) when there is none. */
 int xgpuAmdLoadTuning(XGPUAmdEngine *engine, const char *cache_path, unsigned int ntime);

@@ -14,7 +14,10 @@ the buffers they must stay inside:
   * every operand read stays inside the stage buffer it reads;
   * every epilogue store lands inside the integration buffer, and each valid
     visibility is stored exactly once (the register-tile order's unused
-    diagonal quadrant is zero-stored once, as the reference leaves it).
+    diagonal quadrant is zero-stored once, as the reference leaves it), for
+    the 32x32x32 tiles' epilogue (rg_store_cells) and the 16x16x64 tiles'
+    (rg16_store_cells, whose kernel shares rgd_tile's DMAs at 64-sample
+    stages).
 The shapes include the parity case kept off the GPU (96 stations, 2
 channels, 257 samples, real/imag order) and the 512- and 2048-station shapes.
 """
@@ -163,6 +166,45 @@ def model_epilogue(ns, nf, f, rb, cb, order, accumulate, hits):
                     hits[p1:p1 + span] += 1
 
 
+def model_epilogue16(ns, nf, f, rb, cb, order, accumulate, hits):
+    """rg16_store_cells (the 16x16x64 kernel's epilogue) for every storing
+    wave and lane of tile (f, rb, cb), counting element stores in `hits`."""
+    diag = rb == cb
+    full = (not diag) and (rb * 64 + 64 <= ns)
+    nb = ns * (ns + 1) // 2
+    P = ns // 2
+    RT = P * (P + 1) // 2
+    plane = nf * RT * 16 if order == "regtile" else nf * nb * 4
+    for wave in range(4):
+        wr, wc = wave >> 1, wave & 1
+        if diag and wr < wc:
+            continue
+        for lane in range(64):
+            bq = lane & 3
+            assert (wr * 2 + wc) * CELL_SCRATCH + (lane >> 2) * CELL_GROUP + 128 <= 64 * 1024
+            for bu in range(8):
+                for bvg in range(2):
+                    i = rb * 64 + 32 * wr + 4 * bu + (lane >> 4)
+                    j = cb * 64 + 32 * wc + 16 * bvg + 4 * bq + ((lane & 15) >> 2)
+                    if not full:
+                        if i >= ns or j >= ns:
+                            continue
+                        if i < j and (order != "regtile" or (i >> 1) != (j >> 1) or accumulate):
+                            continue
+                    if order == "regtile":
+                        R = i >> 1
+                        p = 4 * ((f * 4 + 2 * (j & 1) + (i & 1)) * RT + R * (R + 1) // 2 + (j >> 1))
+                        p1 = p + plane
+                    else:
+                        k = f * nb + i * (i + 1) // 2 + j
+                        p = (8 if order == "tri" else 4) * k
+                        p1 = p + 4 if order == "tri" else p + plane
+                    size = 2 * plane
+                    assert 0 <= p and p + 4 <= size and 0 <= p1 and p1 + 4 <= size, (i, j)
+                    hits[p:p + 4] += 1
+                    hits[p1:p1 + 4] += 1
+
+
 SHAPES = [(96, 2, 257, "realimag"), (96, 2, 257, "regtile"), (96, 2, 257, "tri"),
           (64, 1, 100, "tri"), (132, 2, 64, "regtile"), (16, 2, 64, "regtile"),
           (256, 4, 1000, "regtile"), (6 * 4, 3, 65, "tri")]
@@ -204,7 +246,8 @@ def test_operand_reads_inside_stage_buffer(stg, nbuf):
                                              (96, 2, "tri", 1), (132, 2, "regtile", 0),
                                              (132, 2, "regtile", 1), (64, 1, "tri", 0),
                                              (6 * 4, 3, "tri", 0), (16, 2, "realimag", 1)])
-def test_epilogue_stores_in_bounds_and_exactly_once(ns, nf, order, acc):
+@pytest.mark.parametrize("epilogue", [model_epilogue, model_epilogue16])
+def test_epilogue_stores_in_bounds_and_exactly_once(ns, nf, order, acc, epilogue):
     nb = ns * (ns + 1) // 2
     P = ns // 2
     plane = nf * P * (P + 1) // 2 * 16 if order == "regtile" else nf * nb * 4
@@ -213,7 +256,7 @@ def test_epilogue_stores_in_bounds_and_exactly_once(ns, nf, order, acc):
     for f in range(nf):
         for rb in range(nblk):
             for cb in range(rb + 1):
-                model_epilogue(ns, nf, f, rb, cb, order, acc, hits)
+                epilogue(ns, nf, f, rb, cb, order, acc, hits)
     if order == "regtile" and acc:
         # accumulating calls skip quadrant "12" of diagonal register tiles,
         # which holds the zeros the first call stored (reference kernel.cuh:65-71)

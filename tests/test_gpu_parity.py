@@ -86,6 +86,20 @@ def test_i8_matches_exact_oracle(ns, nf, nt, order, kernel, staging, stage):
     assert np.array_equal(to_tri(None, wide, ns, nf, ORDERS[order]), ref)
 
 
+@pytest.mark.parametrize("ns,nf,nt,order", [
+    (16, 2, 64, "regtile"), (64, 3, 128, "regtile"), (64, 1, 100, "tri"),
+    (96, 2, 257, "tri"), (132, 2, 64, "regtile"), (256, 4, 1000, "realimag"),
+    (512, 2, 8192, "regtile")])
+def test_i8_mfma16_matches_exact_oracle(ns, nf, nt, order):
+    """The DMA kernel with 16x16x64 MFMA tiles (tuning mfma 2, 64-sample
+    stages), an autotuner candidate: bit-exact against the int64 oracle."""
+    x = oracle.fill_uniform(nt * nf * ns * 4, ns * 1000 + nt + 7, -128, 127)
+    raw, wide = run_i8(ns, nf, nt, x, ORDERS[order], kernel=2, staging=2, stage=64, mfma=2)
+    ref = oracle.xengine_i64(x, ns, nf, nt)
+    assert np.array_equal(to_tri(None, raw, ns, nf, ORDERS[order]).astype(np.int64), ref)
+    assert np.array_equal(to_tri(None, wide, ns, nf, ORDERS[order]), ref)
+
+
 @pytest.mark.parametrize("kernel,staging", [(1, 0), (2, 1), (2, 2), (3, 0)])
 def test_i8_minus_128_everywhere(kernel, staging):
     ns, nf, nt = 64, 2, 4096
@@ -470,6 +484,90 @@ def test_f16_spilled_triangular_dump(order):
     assert np.array_equal(outs[1], outs[4]) and np.array_equal(outs[1], outs[5])
 
 
+# ------------------------------------------------------------------ hipGraph steps
+def test_device_path_graph_replays():
+    """graph knob on: a device-path call (layout conversion + launch) is
+    captured once per launch and replayed.  Several replays, with eager work
+    and synchronizations between them, each checked against the oracle:
+    store after a clear, accumulate, new data in the same buffer, a
+    time-swizzled engine, and a call past the int32 bound (issued eagerly with
+    a spill)."""
+    ns, nf, nt = 64, 3, 4096
+    xs = [oracle.fill_uniform(nt * nf * ns * 4, 700 + k, -128, 127) for k in range(3)]
+    refs = [oracle.xengine_i64(x, ns, nf, nt) for x in xs]
+    buf = X.DeviceBuffer(xs[0].nbytes)
+    other = X.DeviceBuffer.from_array(xs[2])
+    plain = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)  # eager work
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    eng.set_tuning(graph=1)
+    flush("[graph] device path replays")
+    for rep in range(4):
+        k = rep % 2
+        buf.upload(xs[k])
+        eng.clear()
+        eng.correlate(buf, nt)                      # store step
+        plain.clear()
+        plain.correlate(other, nt)                  # eager work between replays
+        assert np.array_equal(eng.dump(wide=True, triangular=True), refs[k]), rep
+        eng.correlate(buf, nt)                      # accumulate step
+        X.core().xgpuAmdDeviceSynchronize()
+        assert np.array_equal(eng.dump(wide=True, triangular=True), 2 * refs[k]), rep
+        assert np.array_equal(plain.dump().astype(np.int64), refs[2]), rep
+    # past the int32 bound: 16 calls of 4096 = 65536 samples, the last spills
+    eng.clear()
+    buf.upload(xs[0])
+    for _ in range(16):
+        eng.correlate(buf, nt)
+    assert np.array_equal(eng.dump(wide=True, triangular=True), 16 * refs[0])
+    # a time-swizzled engine: the unswizzle is part of the captured step
+    sw = X.DeviceBuffer.from_array(oracle.swizzle(xs[1], ns, nf, nt))
+    es = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER, time_swizzle=4)
+    es.set_tuning(graph=1)
+    for rep in range(3):
+        es.clear()
+        es.correlate(sw, nt)
+        assert np.array_equal(es.dump(wide=True), refs[1]), rep
+    for b in (buf, other, sw):
+        b.free()
+    for e in (plain, eng, es):
+        e.close()
+
+
+def test_batch_graph_replays():
+    """graph knob on the batch's first engine: one ragged launch captured and
+    replayed; between replays the inputs are refilled, other engines work
+    eagerly, and each replay's integrations are checked against the oracle."""
+    shapes = [(48, 3, 512), (96, 2, 1000), (20, 4, 300)]
+    engines, bufs = [], []
+    for ns, nf, nt in shapes:
+        engines.append(X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER))
+        bufs.append(X.DeviceBuffer(nt * nf * ns * 4))
+    engines[0].set_tuning(graph=1)
+    items = [(e, b, s[2]) for e, b, s in zip(engines, bufs, shapes)]
+    extra = X.Engine(64, 2, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
+    xe = oracle.fill_uniform(256 * 2 * 64 * 4, 799, -128, 127)
+    be = X.DeviceBuffer.from_array(xe)
+    flush("[graph] batch replays")
+    for rep in range(4):
+        xs = [oracle.fill_uniform(nt * nf * ns * 4, 800 + 10 * rep + k, -128, 127)
+              for k, (ns, nf, nt) in enumerate(shapes)]
+        for b, x in zip(bufs, xs):
+            b.upload(x)
+        for e in engines:
+            e.clear()
+        X.correlate_batch(items)
+        extra.clear()
+        extra.correlate(be, 256)
+        X.correlate_batch(items)                 # accumulate replay
+        for k, (e, (ns, nf, nt)) in enumerate(zip(engines, shapes)):
+            assert np.array_equal(e.dump(wide=True), 2 * oracle.xengine_i64(xs[k], ns, nf, nt)), (rep, k)
+        assert np.array_equal(extra.dump().astype(np.int64), oracle.xengine_i64(xe, 64, 2, 256))
+    for b in bufs + [be]:
+        b.free()
+    for e in engines + [extra]:
+        e.close()
+
+
 # ------------------------------------------------------------------ batching
 @pytest.mark.parametrize("kernel", [0, 3])
 def test_batch_ragged_equals_alone(kernel):
@@ -536,6 +634,9 @@ def test_host_path_pipeline_and_graph_replays(graph):
 
 
 def test_autotune_keeps_results_exact(tmp_path):
+    """Every candidate the autotuner times is exact; the tuner leaves the
+    caller's integration as it was (candidates run on a scratch engine); the
+    winner goes to the tuning cache and an engine created later loads it."""
     ns, nf, nt = 128, 8, 1024
     x = oracle.fill_uniform(nt * nf * ns * 4, 31, -128, 127)
     ref = oracle.xengine_i64(x, ns, nf, nt)
@@ -543,11 +644,14 @@ def test_autotune_keeps_results_exact(tmp_path):
     eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
     cache = str(tmp_path / "tuning.txt")
     flush("[autotune] ns=128 nf=8 nt=1024")
+    eng.correlate(buf, nt)                       # an integration in progress
     best, ms = eng.autotune(buf, nt, warmup=1, reps=2, cache_path=cache)
     assert ms > 0 and best["wg_per_cu"] in (1, 2) and best["prefetch"] in (1, 2)
+    assert np.array_equal(eng.dump(wide=True), ref)          # untouched by the tuner
     eng.correlate(buf, nt)
-    assert np.array_equal(eng.dump().astype(np.int64), ref)
+    assert np.array_equal(eng.dump(wide=True), 2 * ref)      # and it continues
     assert best["kernel"] in (1, 2, 3) and best["stage"] in (0, 32, 64, 128)
+    assert best["mfma"] in (1, 2)
     for wg in (1, 2):           # every candidate is exact
         for pf in (1, 2):
             eng.set_tuning(wg, pf, kernel=1)
@@ -559,14 +663,51 @@ def test_autotune_keeps_results_exact(tmp_path):
         eng.clear()
         eng.correlate(buf, nt)
         assert np.array_equal(eng.dump().astype(np.int64), ref), (kernel, staging, stage)
+    eng.set_tuning(kernel=2, staging=2, stage=64, mfma=2)
+    eng.clear()
+    eng.correlate(buf, nt)
+    assert np.array_equal(eng.dump().astype(np.int64), ref), "mfma 16x16x64"
     eng2 = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
     assert eng2.load_tuning(cache, nt)
-    assert eng2.get_tuning()["wg_per_cu"] == best["wg_per_cu"]
-    assert eng2.get_tuning()["prefetch"] == best["prefetch"]
-    assert eng2.get_tuning()["kernel"] == best["kernel"]
-    assert eng2.get_tuning()["stage"] == best["stage"] or best["kernel"] == 1
-    assert eng2.get_tuning()["staging"] == best["staging"] or best["kernel"] == 1
+    got = eng2.get_tuning()
+    for k in ("wg_per_cu", "prefetch", "kernel", "mfma"):
+        assert got[k] == best[k], k
+    assert got["stage"] == best["stage"] or best["kernel"] == 1
+    assert got["staging"] == best["staging"] or best["kernel"] == 1
     assert not eng2.load_tuning(cache, nt + 1)
+    eng.close()
+    eng2.close()
+    buf.free()
+
+
+def test_autotune_host_path_time_pipe(tmp_path):
+    """Host-path tuner over the time-pipe length x graph replay: the winner
+    divides NTIME, every setting gives the same exact integration, and the
+    cache carries it to a new engine."""
+    ns, nf, nt, ntp = 64, 4, 1024, 128
+    x = oracle.fill_uniform(nt * nf * ns * 4, 41, -128, 127)
+    ref = oracle.xengine_i64(x, ns, nf, nt)
+    cache = str(tmp_path / "tuning.txt")
+    eng = X.Engine(ns, nf, nt, ntp, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
+    flush("[autotune] host path")
+    got = eng.correlate_host(x, X.SYNCOP_DUMP)   # an integration in progress
+    assert np.array_equal(got.astype(np.int64), ref)
+    best, ms = eng.autotune_host(x, warmup=1, reps=2, cache_path=cache)
+    assert ms > 0 and nt % best["ntimepipe"] == 0 and best["graph"] in (1, 2)
+    assert np.array_equal(eng.dump(wide=True), ref)          # untouched by the tuner
+    for p in (nt, nt // 2, nt // 8, 32):
+        for graph in (1, 2):
+            eng.set_tuning(ntimepipe=p, graph=graph)
+            eng.clear()
+            got = eng.correlate_host(x, X.SYNCOP_DUMP)
+            assert np.array_equal(got.astype(np.int64), ref), (p, graph)
+    # a device-path line is needed for the key the loader looks up first
+    buf = X.DeviceBuffer.from_array(x)
+    eng.autotune(buf, nt, warmup=1, reps=1, cache_path=cache)
+    eng2 = X.Engine(ns, nf, nt, ntp, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
+    assert eng2.load_tuning(cache, nt)
+    assert eng2.get_tuning()["ntimepipe"] == best["ntimepipe"]
+    assert eng2.get_tuning()["graph"] == best["graph"]
     eng.close()
     eng2.close()
     buf.free()

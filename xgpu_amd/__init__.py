@@ -166,36 +166,53 @@ class Engine:
         check(core().xgpuAmdTimeCorrelate(self.handle, ptr, ntime, warmup, reps, C.byref(ms)))
         return float(ms.value)
 
+    _TUNING = ("wg_per_cu", "prefetch", "graph", "kernel", "stage", "staging", "ntimepipe", "mfma")
+
     def set_tuning(self, wg_per_cu: int = 0, prefetch: int = 0, graph: int = 0,
-                   kernel: int = 0, stage: int = 0, staging: int = 0) -> None:
+                   kernel: int = 0, stage: int = 0, staging: int = 0, ntimepipe: int = 0,
+                   mfma: int = 0) -> None:
         """wg_per_cu: 1 or 2; prefetch / graph: 1 on, 2 off; kernel (int8 and
         fp16): 1 complex-plane, 2 real-Gram where it applies, 3 (int8)
         K-interleaved pair-tile kernel where the real-Gram one applies; staging
         (real-Gram): 1 registers, 2 global->LDS DMA; stage: samples per LDS
         stage (int8: 64 / 128 with register staging, 32 / 64 with DMA; fp16:
-        32 / 64 with register staging, 32 with DMA) (0 = default everywhere)."""
+        32 / 64 with register staging, 32 with DMA); ntimepipe: host-path
+        samples per transfer and launch; mfma (int8 DMA kernel): 1 32x32x32,
+        2 16x16x64 tiles (0 = default everywhere)."""
         t = _lib.XGPUAmdTuning()
         t.wg_per_cu, t.prefetch, t.graph = wg_per_cu, prefetch, graph
         t.kernel, t.stage, t.staging = kernel, stage, staging
+        t.ntimepipe, t.mfma = ntimepipe, mfma
         check(core().xgpuAmdSetTuning(self.handle, C.byref(t)))
 
     def get_tuning(self) -> dict:
         t = _lib.XGPUAmdTuning()
         check(core().xgpuAmdGetTuning(self.handle, C.byref(t)))
-        return {"wg_per_cu": t.wg_per_cu, "prefetch": t.prefetch, "graph": t.graph,
-                "kernel": t.kernel, "stage": t.stage, "staging": t.staging}
+        return {k: getattr(t, k) for k in self._TUNING}
 
     def autotune(self, input_d, ntime: int, warmup: int = 1, reps: int = 3,
                  cache_path: str | None = None):
-        """Time every kernel tuning on this input; keep and return the fastest."""
+        """Time every kernel tuning on this input (on a scratch engine: this
+        engine's integration is left as it is); keep and return the fastest."""
         t = _lib.XGPUAmdTuning()
         ms = C.c_float()
         ptr = input_d.ptr if isinstance(input_d, DeviceBuffer) else int(input_d)
         check(core().xgpuAmdAutotune(self.handle, ptr, ntime, warmup, reps,
                                      cache_path.encode() if cache_path else None,
                                      C.byref(t), C.byref(ms)))
-        return ({"wg_per_cu": t.wg_per_cu, "prefetch": t.prefetch, "kernel": t.kernel,
-                 "stage": t.stage, "staging": t.staging}, float(ms.value))
+        return ({k: getattr(t, k) for k in self._TUNING}, float(ms.value))
+
+    def autotune_host(self, input_h, warmup: int = 1, reps: int = 3,
+                      cache_path: str | None = None):
+        """Time the host-buffer path's time-pipe lengths x graph replay on this
+        host input; keep and return the fastest (ms per call)."""
+        t = _lib.XGPUAmdTuning()
+        ms = C.c_float()
+        ptr = input_h.ptr if isinstance(input_h, HostBuffer) else np.ascontiguousarray(input_h).ctypes.data
+        check(core().xgpuAmdAutotuneHost(self.handle, ptr, warmup, reps,
+                                         cache_path.encode() if cache_path else None,
+                                         C.byref(t), C.byref(ms)))
+        return ({k: getattr(t, k) for k in self._TUNING}, float(ms.value))
 
     def load_tuning(self, cache_path: str, ntime: int) -> bool:
         return core().xgpuAmdLoadTuning(self.handle, cache_path.encode(), ntime) == 0

@@ -78,7 +78,7 @@ class XGPUAmdBatchItem(C.Structure):
 class XGPUAmdTuning(C.Structure):
     _fields_ = [("wg_per_cu", C.c_int), ("prefetch", C.c_int), ("graph", C.c_int),
                 ("kernel", C.c_int), ("stage", C.c_int), ("staging", C.c_int),
-                ("reserved", C.c_int * 2)]
+                ("ntimepipe", C.c_int), ("mfma", C.c_int)]
 
 
 class XGPUServerConfig(C.Structure):
@@ -156,6 +156,8 @@ def core() -> C.CDLL:
         _sig(lib, "xgpuAmdAutotune", I, P, P, U, I, I, C.c_char_p, C.POINTER(XGPUAmdTuning),
              C.POINTER(C.c_float))
         _sig(lib, "xgpuAmdLoadTuning", I, P, C.c_char_p, U)
+        _sig(lib, "xgpuAmdAutotuneHost", I, P, P, I, I, C.c_char_p, C.POINTER(XGPUAmdTuning),
+             C.POINTER(C.c_float))
         _sig(lib, "xgpuAmdHostAlloc", I, C.POINTER(P), SZ)
         _sig(lib, "xgpuAmdHostFree", I, P)
         _sig(lib, "xgpuServerCreate", I, C.POINTER(P), C.POINTER(XGPUServerConfig))

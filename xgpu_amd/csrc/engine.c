@@ -179,6 +179,8 @@ int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
   e->tstride = (long long)cfg->nfrequency * cfg->nstation * 2 * (long long)e->cplx_bytes;
   e->integ_bytes = (size_t)info.matLength * 2 * 4;
   e->cleared = 1;
+  e->ntp = info.ntimepipe;
+  e->mfma = 1;
   e->wg_per_cu = 2;
   e->prefetch = XG_DEFAULT_PREFETCH;
   e->kernel_pref = 2;
@@ -209,6 +211,8 @@ hip_error:
   return rc;
 }
 
+static void drop_steps(XGPUAmdEngine *e);
+
 void xgpuAmdDestroy(XGPUAmdEngine *e) {
   if (!e) return;
   xgh_set_device(e->cfg.device);
@@ -225,6 +229,8 @@ void xgpuAmdDestroy(XGPUAmdEngine *e) {
   xgh_event_destroy(e->ev_aux);
   xgh_event_destroy(e->ev_join);
   for (int g = 0; g < e->ngraphs; ++g) xgh_graph_destroy(e->graphs[g].exec);
+  drop_steps(e);
+  free(e->steps);
   xgh_free(e->integ);
   xgh_free(e->wide);
   xgh_free(e->scratch);
@@ -248,6 +254,7 @@ unsigned long long xgpuAmdSamples(const XGPUAmdEngine *e) { return e ? e->sample
 void *xgpuAmdStream(XGPUAmdEngine *e) { return e ? e->stream : NULL; }
 
 static void drop_graphs(XGPUAmdEngine *e);
+static void drop_steps(XGPUAmdEngine *e);
 
 int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
   if (!e || !t) return xg_fail(XGPU_NOT_INITIALIZED, "NULL argument");
@@ -277,8 +284,18 @@ int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
                      "stage must be 0, or 64 / 128 with register staging, 32 / 64 with DMA staging");
     stage = t->stage ? t->stage : (staging == XG_DEFAULT_RG_STAGING ? XG_DEFAULT_RG_STAGE : 64);
   }
+  if (t->ntimepipe < 0 || (t->ntimepipe > 0 && (e->cfg.ntime % (unsigned)t->ntimepipe ||
+                                                 (e->swizzled && t->ntimepipe % 4))))
+    return xg_fail(XGPU_INVALID_CONFIG, "ntimepipe %d must divide ntime %u%s", t->ntimepipe,
+                   e->cfg.ntime, e->swizzled ? " and be a multiple of 4" : "");
+  if (t->mfma < 0 || t->mfma > 2) return xg_fail(XGPU_INVALID_CONFIG, "mfma must be 0..2");
+  if (t->mfma == 2 && !(e->family == XG_KERNEL_I8_MFMA && staging == 2 && stage == 64))
+    return xg_fail(XGPU_INVALID_CONFIG,
+                   "mfma 2 (16x16x64) is the int8 real-Gram DMA kernel's (staging 2, stage 64)");
   /* graphs captured with the old kernel choice would keep replaying it */
   drop_graphs(e);
+  e->ntp = t->ntimepipe ? (unsigned)t->ntimepipe : e->info.ntimepipe;
+  e->mfma = t->mfma ? t->mfma : 1;
   e->use_graph = (t->graph == 1);
   e->wg_per_cu = t->wg_per_cu ? t->wg_per_cu : 2;
   e->prefetch = t->prefetch == 0 ? XG_DEFAULT_PREFETCH : (t->prefetch == 1);
@@ -297,6 +314,8 @@ int xgpuAmdGetTuning(const XGPUAmdEngine *e, XGPUAmdTuning *t) {
   t->kernel = e->kernel_pref;
   t->stage = e->stage;
   t->staging = e->staging;
+  t->ntimepipe = (int)e->ntp;
+  t->mfma = e->mfma;
   return XGPU_OK;
 }
 
@@ -336,7 +355,7 @@ int xg_launch_family(const XGPUAmdEngine *e, const void *input_d) {
 /* XgLaunch.stage code of the real-Gram kernel variant */
 static char rg_stage_code(const XGPUAmdEngine *e) {
   if (e->family == XG_KERNEL_F16_MFMA) return (char)(e->staging == 2 ? 2 : e->stage == 64 ? 1 : 0);
-  if (e->staging == 2) return (char)(e->stage == 32 ? 4 : 3);
+  if (e->staging == 2) return (char)(e->stage == 32 ? 4 : e->mfma == 2 ? 5 : 3);
   return (char)(e->stage == 128 ? 1 : 0);
 }
 
@@ -348,28 +367,120 @@ static int wg_per_cu_for(const XGPUAmdEngine *e, int family) {
              : e->wg_per_cu;
 }
 
-int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime) {
-  XgLaunch L;
-  memset(&L, 0, sizeof L);
-  L.njobs = 1;
+/* The launch that correlates ntime samples of plain input at input_d into
+ * the engine's integration (store or accumulate by its cleared state). */
+static int single_launch(XGPUAmdEngine *e, const void *input_d, unsigned ntime, XgLaunch *L) {
+  memset(L, 0, sizeof *L);
+  L->njobs = 1;
   const int family = xg_launch_family(e, input_d);
-  job_fill(&L.jobs[0], e, input_d, ntime, !e->cleared, 0);
-  job_tiles(&L.jobs[0], family);
-  xg_plan_launch(&L, e->cus, wg_per_cu_for(e, family));
-  L.prefetch = (char)e->prefetch;
-  L.stage = rg_stage_code(e);
-  HIPCALL(xgh_launch_xengine(family, &L, e->stream), "xengine kernel");
+  job_fill(&L->jobs[0], e, input_d, ntime, !e->cleared, 0);
+  job_tiles(&L->jobs[0], family);
+  xg_plan_launch(L, e->cus, wg_per_cu_for(e, family));
+  L->prefetch = (char)e->prefetch;
+  L->stage = rg_stage_code(e);
+  return family;
+}
+
+static void advance(XGPUAmdEngine *e, unsigned ntime) {
   e->cleared = 0;
   e->samples_in_integ += ntime;
   e->samples += ntime;
+}
+
+int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime) {
+  XgLaunch L;
+  const int family = single_launch(e, input_d, ntime, &L);
+  HIPCALL(xgh_launch_xengine(family, &L, e->stream), "xengine kernel");
+  advance(e, ntime);
   return XGPU_OK;
 }
 
-/* Forget the captured host-path graphs (they hold buffer pointers and the
- * kernel tuning of the moment they were captured). */
+/* ---- launch steps captured in hipGraphs --------------------------------- */
+/* A step: the input-layout conversions of the jobs that need one (raw input
+ * -> the job's plain input), then one X-engine launch.  With the graph knob
+ * on, a step is captured once and replayed; the key is the whole launch
+ * descriptor (buffers, shapes, store or accumulate, kernel and tuning), the
+ * raw inputs and the stream, so a replay always repeats exactly the launch
+ * an eager call would issue.  Spills are never part of a step. */
+#define XG_STEP_GRAPHS 8
+struct XgStepGraph {
+  XgLaunch L;
+  const void *raw[XG_MAX_JOBS];
+  int family;
+  void *stream;
+  void *exec;
+};
+
+static void drop_steps(XGPUAmdEngine *e) {
+  for (int g = 0; g < e->nsteps; ++g) xgh_graph_destroy(e->steps[g].exec);
+  e->nsteps = 0;
+}
+
+/* Forget the captured graphs (they hold buffer pointers and the kernel
+ * tuning of the moment they were captured). */
 static void drop_graphs(XGPUAmdEngine *e) {
   for (int g = 0; g < e->ngraphs; ++g) xgh_graph_destroy(e->graphs[g].exec);
   e->ngraphs = 0;
+  drop_steps(e);
+}
+
+static int convert_input(XGPUAmdEngine *e, void *dst, const void *src, unsigned ntime,
+                         void *stream);
+
+/* Issue a step eagerly: conversions (engines[i] converts when raw[i] differs
+ * from the job's input), then the launch. */
+static int issue_step(XGPUAmdEngine *const *engines, const void *const *raw, const XgLaunch *L,
+                      int family, void *s) {
+  for (int i = 0; i < L->njobs; ++i)
+    if (raw[i] != L->jobs[i].input && L->jobs[i].ntime > 0) {
+      int rc = convert_input(engines[i], (void *)L->jobs[i].input, raw[i], (unsigned)L->jobs[i].ntime, s);
+      if (rc) return rc;
+    }
+  HIPCALL(xgh_launch_xengine(family, L, s), "xengine kernel");
+  return XGPU_OK;
+}
+
+/* Run a step on stream s: replay owner's captured graph of it, capturing it
+ * first when the owner's graph knob is on; otherwise issue it eagerly. */
+static int run_step(XGPUAmdEngine *owner, XGPUAmdEngine *const *engines, const void *const *raw,
+                    const XgLaunch *L, int family, void *s) {
+  if (!owner->use_graph) return issue_step(engines, raw, L, family, s);
+  if (!owner->steps) {
+    owner->steps = (struct XgStepGraph *)calloc(XG_STEP_GRAPHS, sizeof *owner->steps);
+    if (!owner->steps) return xg_fail(XGPU_OUT_OF_MEMORY, "out of host memory");
+  }
+  int g = -1;
+  for (int i = 0; i < owner->nsteps && g < 0; ++i) {
+    const struct XgStepGraph *t = &owner->steps[i];
+    if (t->family == family && t->stream == s && memcmp(&t->L, L, sizeof *L) == 0 &&
+        memcmp(t->raw, raw, sizeof(void *) * (size_t)L->njobs) == 0)
+      g = i;
+  }
+  if (g < 0) {
+    if (owner->nsteps == XG_STEP_GRAPHS) { /* drop the oldest */
+      xgh_graph_destroy(owner->steps[0].exec);
+      memmove(&owner->steps[0], &owner->steps[1], sizeof *owner->steps * (XG_STEP_GRAPHS - 1));
+      owner->nsteps--;
+    }
+    HIPCALL(xgh_graph_begin(s), "begin capture");
+    const int rc = issue_step(engines, raw, L, family, s);
+    void *exec = NULL;
+    const int rc_end = xgh_graph_end(s, &exec);
+    if (rc || rc_end) {
+      xgh_graph_destroy(exec);
+      return rc ? rc : xg_fail(XGPU_CUDA_ERROR, "graph capture: %s", xgh_last_error());
+    }
+    g = owner->nsteps++;
+    struct XgStepGraph *t = &owner->steps[g];
+    memcpy(&t->L, L, sizeof *L);
+    memset(t->raw, 0, sizeof t->raw);
+    memcpy(t->raw, raw, sizeof(void *) * (size_t)L->njobs);
+    t->family = family;
+    t->stream = s;
+    t->exec = exec;
+  }
+  HIPCALL(xgh_graph_launch(owner->steps[g].exec, s), "graph launch");
+  return XGPU_OK;
 }
 
 /* The engine's conversion buffer, at least `bytes` long. */
@@ -447,10 +558,23 @@ int xgpuAmdCorrelate(XGPUAmdEngine *e, const void *input_d, unsigned int ntime) 
   if (!input_d && ntime) return xg_fail(XGPU_INVALID_CONFIG, "NULL input");
   if (((uintptr_t)input_d) & 7) return xg_fail(XGPU_INVALID_CONFIG, "input must be 8-byte aligned");
   HIPCALL(xgh_set_device(e->cfg.device), "set device");
+  int rc;
+  if (e->use_graph && ntime > 0 && e->samples_in_integ + ntime <= e->bound) {
+    /* one step (conversion + launch), captured once and replayed */
+    const void *plain = input_d;
+    if (e->block32 || e->swizzled) {
+      if ((rc = ensure_unblock(e, (size_t)ntime * (size_t)e->tstride))) return rc;
+      plain = e->unblock;
+    }
+    XgLaunch L;
+    const int family = single_launch(e, plain, ntime, &L);
+    if ((rc = run_step(e, &e, &input_d, &L, family, e->stream))) return rc;
+    advance(e, ntime);
+    return XGPU_OK;
+  }
   if ((e->block32 || e->swizzled) && ntime) {
     const void *plain;
-    int rc = unblock_input(e, input_d, ntime, e->stream, &plain);
-    if (rc) return rc;
+    if ((rc = unblock_input(e, input_d, ntime, e->stream, &plain))) return rc;
     input_d = plain;
   }
   return correlate_plain(e, input_d, ntime);
@@ -641,7 +765,7 @@ int xgpuAmdCorrelateHost(XGPUAmdEngine *e, const void *input_h, int syncOp, void
   if (syncOp == SYNCOP_DUMP && !output_h) return xg_fail(XGPU_HOST_BUFFER_NOT_SET, "NULL host output");
   if (e->cfg.ntime == 0) return XGPU_OK;
   HIPCALL(xgh_set_device(e->cfg.device), "set device");
-  const unsigned ntp = e->info.ntimepipe;
+  const unsigned ntp = e->ntp;
   const int npipe = (int)(e->cfg.ntime / ntp);
   const size_t chunk = (size_t)ntp * (size_t)e->tstride;
   const int buf = e->parity;
@@ -659,7 +783,8 @@ int xgpuAmdCorrelateHost(XGPUAmdEngine *e, const void *input_h, int syncOp, void
   char *dbase = (char *)e->dev_in[buf];
   int rc;
   if (e->swizzled && (rc = ensure_unblock(e, chunk * npipe))) return rc;
-  const int graphable = e->use_graph && e->samples_in_integ + e->cfg.ntime <= e->bound &&
+  const int graphable = e->use_graph && !e->host_graph_off &&
+                        e->samples_in_integ + e->cfg.ntime <= e->bound &&
                         (e->cfg.ntime <= e->bound);
   if (graphable) {
     int g = -1;
@@ -700,9 +825,9 @@ int xgpuAmdCorrelateHost(XGPUAmdEngine *e, const void *input_h, int syncOp, void
         e->graphs[g].buf = buf;
         e->graphs[g].exec = exec;
       } else {
-        /* not capturable (e.g. a pageable source): issue this call eagerly */
+        /* not capturable (e.g. a pageable source): issue host-path calls eagerly */
         xgh_graph_destroy(exec);
-        e->use_graph = 0;
+        e->host_graph_off = 1;
       }
     }
     if (g >= 0) {
@@ -762,6 +887,8 @@ int xgpuAmdCorrelateBatch(const XGPUAmdBatchItem *items, int nitems, void *strea
   void *s = stream ? stream : e0->stream;
   XgLaunch L;
   memset(&L, 0, sizeof L);
+  XGPUAmdEngine *engines[XG_MAX_JOBS];
+  const void *raw[XG_MAX_JOBS];
   int chan = 0;
   for (int i = 0; i < nitems; ++i) {
     XGPUAmdEngine *e = items[i].engine;
@@ -774,9 +901,12 @@ int xgpuAmdCorrelateBatch(const XGPUAmdBatchItem *items, int nitems, void *strea
       HIPCALL(xgh_stream_wait_event(s, e->ev_aux), "wait");
     }
     const void *in = items[i].input_d;
-    if ((e->block32 || e->swizzled) && items[i].ntime) {
-      int rc = unblock_input(e, in, items[i].ntime, s, &in);
+    engines[i] = e;
+    raw[i] = in;
+    if ((e->block32 || e->swizzled) && items[i].ntime) { /* converted by the step */
+      int rc = ensure_unblock(e, (size_t)items[i].ntime * (size_t)e->tstride);
       if (rc) return rc;
+      in = e->unblock;
     }
     job_fill(&L.jobs[i], e, in, items[i].ntime, !e->cleared, chan);
     chan += (int)e->cfg.nfrequency;
@@ -790,7 +920,8 @@ int xgpuAmdCorrelateBatch(const XGPUAmdBatchItem *items, int nitems, void *strea
   xg_plan_launch(&L, e0->cus, wg_per_cu_for(e0, family));
   L.prefetch = (char)e0->prefetch;
   L.stage = rg_stage_code(e0);
-  HIPCALL(xgh_launch_xengine(family, &L, s), "batched xengine kernel");
+  int rc = run_step(e0, engines, raw, &L, family, s);
+  if (rc) return rc;
   HIPCALL(xgh_event_record(e0->ev_aux, s), "record");
   for (int i = 0; i < nitems; ++i) {
     XGPUAmdEngine *e = items[i].engine;

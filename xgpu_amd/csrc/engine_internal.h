@@ -25,6 +25,8 @@ struct XGPUAmdEngineStruct {
   int kernel_pref;   /* int8: 1 = complex-plane kernel, 2 = real-Gram when eligible */
   int stage;         /* real-Gram kernel: samples per LDS stage */
   int staging;       /* real-Gram kernel: 1 registers, 2 DMA */
+  int mfma;          /* int8 real-Gram DMA kernel: 1 = 32x32x32, 2 = 16x16x64 MFMA tiles */
+  unsigned ntp;      /* host-buffer path: samples per transfer and launch */
 
   void *integ;       /* int32 / f32 integration, 2*matLength scalars */
   void *wide;        /* int64 / f64 spill, allocated on first spill */
@@ -57,6 +59,13 @@ struct XGPUAmdEngineStruct {
   /* the device's static scratch (xgh_static_scratch): conversion buffer of
    * xgpuAmdCorrelateNoAlloc, shared by every engine on the device */
   void *static_scratch;
+
+  /* device-path and batch launch steps captured in hipGraphs (graph knob),
+   * keyed by everything the step launches; the batch cache lives in the
+   * batch's first engine */
+  struct XgStepGraph *steps;
+  int nsteps;
+  int host_graph_off;  /* a host-path capture failed (e.g. pageable source) */
 
   /* host-buffer path graphs, keyed by (input pointer, cleared state, buffer) */
   int use_graph;

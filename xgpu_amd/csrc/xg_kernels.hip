@@ -1310,6 +1310,239 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
   rg_finish(J, T, acc, lds);
 }
 
+// ---------------------------------------------------------------------------
+// int8 real-Gram DMA kernel with 16x16x64 MFMA tiles (tuning mfma = 2).  The
+// same LDS layout, DMA staging and stage barriers as rgd_tile at 64-sample
+// stages (one kstep per stage), but each wave's 128 x 128 reals are 8 x 8
+// blocks of v_mfma_i32_16x16x64_i8 (4 accumulators per lane and block, 256
+// in all).  Operand of a 16-real block (one 16-byte chunk of the sample row):
+// lane l takes real row l % 16 for samples 16 (l / 16) .. +15, two transpose
+// reads of 8 samples each (the 16-lane-group rule of rg_lane_off with the
+// whole wave on one chunk).  Accumulator element e of lane l is real row
+// 4 (l / 16) + e, real column l % 16 of the block.  Twice the operand bytes
+// per MAC of the 32x32x32 tiles: an autotuner alternative.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int rg16_lane_off(int chunk, int lane) {
+  const int q = lane >> 4, r = lane & 15;
+  return (q * 16 + (r >> 1)) * RG_ROW + (rg_slot(chunk, r >> 1) << 4) + ((r & 1) << 3);
+}
+
+// 32-bit stores of a 16x16-tile wave: for row block bu and a group of four
+// column blocks (bv = 4 bvg + 0..3), a lane quad (l & ~3) holds the four
+// polarization products (2 p1 + p2 = 2 (l & 1) + ((l >> 1) & 1)) of cell
+// (i, j) of each block, i = 64 rb + 32 wr + 4 bu + (l >> 4), j = 64 cb +
+// 32 wc + 4 bv + ((l & 15) >> 2).  A wave-private LDS transpose gives lane l
+// the whole cell of block bv = 4 bvg + (l & 3); its 16-lane group then
+// writes 16 cells of consecutive j.  Orders, bounds and the triangle cut as
+// rg_store_cells (reference src/kernel.cuh:49-117).
+__device__ __forceinline__ void rg16_store_cells(const XgJob &J, const TileRef &T, v4i (&acc)[8][8],
+                                                 int wr, int wc, int lane, const RgBperm &swap,
+                                                 unsigned char *lds) {
+  typedef unsigned S4 __attribute__((ext_vector_type(4)));
+  const int order = J.order;
+  const bool tri = order == XG_ORDER_TRI, regt = order == XG_ORDER_REGTILE;
+  const bool odd = lane & 1;
+  const int e = 2 * (lane & 1) + ((lane >> 1) & 1);
+  const int bq = lane & 3;
+  unsigned char *scr = lds + (wr * 2 + wc) * RG_CELL_SCRATCH + (lane >> 2) * RG_CELL_GROUP;
+  const int ns = J.nstation;
+  const int accum = J.accumulate;
+  const long long plane = J.plane;
+  const bool diag = T.rb == T.cb;
+  const bool full = !diag && (T.rb * 64 + 64 <= ns);
+  const long long RT = (long long)(ns >> 1) * ((ns >> 1) + 1) / 2;
+  const long long NB = (long long)ns * (ns + 1) / 2;
+#pragma unroll
+  for (int bu = 0; bu < 8; ++bu)
+#pragma unroll
+    for (int bvg = 0; bvg < 2; ++bvg) {
+      __builtin_amdgcn_sched_barrier(0);
+      int re[4], im[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        v16i blk;
+        blk[0] = acc[bu][4 * bvg + b][0];
+        blk[1] = acc[bu][4 * bvg + b][1];
+        blk[2] = acc[bu][4 * bvg + b][2];
+        blk[3] = acc[bu][4 * bvg + b][3];
+        rg_combine(blk, 0, odd, swap, re[b], im[b]);
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        *(unsigned *)(scr + 16 * b + 4 * e) = (unsigned)re[b];
+        *(unsigned *)(scr + 64 + 16 * b + 4 * e) = (unsigned)im[b];
+      }
+      const S4 cre = *(const S4 *)(scr + 16 * bq);
+      const S4 cim = *(const S4 *)(scr + 64 + 16 * bq);
+      S4 v0 = cre, v1 = cim;
+      if (tri) {
+        v0 = S4{cre[0], cim[0], cre[1], cim[1]};
+        v1 = S4{cre[2], cim[2], cre[3], cim[3]};
+      }
+      const int i = T.rb * 64 + 32 * wr + 4 * bu + (lane >> 4);
+      const int j = T.cb * 64 + 32 * wc + 16 * bvg + 4 * bq + ((lane & 15) >> 2);
+      if (!full) {
+        if (i >= ns || j >= ns) continue;
+        if (i < j) {
+          if (!regt || (i >> 1) != (j >> 1) || accum) continue;
+          v0 = (S4)0u;
+          v1 = (S4)0u;
+        }
+      }
+      unsigned *p, *p1;
+      if (regt) {
+        const long long R = i >> 1;
+        p = (unsigned *)J.out +
+            4 * (((long long)T.f * 4 + 2 * (j & 1) + (i & 1)) * RT + R * (R + 1) / 2 + (j >> 1));
+        p1 = p + plane;
+      } else {
+        const long long k = (long long)T.f * NB + (long long)i * (i + 1) / 2 + j;
+        p = (unsigned *)J.out + (tri ? 8 : 4) * k;
+        p1 = tri ? p + 4 : p + plane;
+      }
+      if (accum) {
+        v0 += *(const S4 *)p;
+        v1 += *(const S4 *)p1;
+      }
+      __builtin_nontemporal_store(v0, (S4 *)p);
+      __builtin_nontemporal_store(v1, (S4 *)p1);
+    }
+}
+
+template <int NBUF>
+__device__ __forceinline__ void rgd16_tile(const XgJob &J, const TileRef &T, unsigned char *lds) {
+  typedef RgInt8 E;
+  constexpr int STG = 64;
+  typedef RgdCfg<E, STG, NBUF> C;
+  static_assert(C::kDma == 4, "4 DMAs per wave per side per stage");
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ninput = 2 * J.nstation;
+  const bool diag = (T.rb == T.cb);
+  const int nstage = (J.ntime + STG - 1) / STG;
+  const int ntime = J.ntime;
+  const int ts = (int)J.tstride;  // < 2^24: the host keeps 128 * tstride < 2^31
+  const unsigned char *in_f =
+      (const unsigned char *)J.input + (long long)T.f * J.nstation * (2 * E::kRow / XG_TILE);
+  const unsigned long long row_base = (unsigned long long)(in_f + T.rb * E::kRow);
+  const unsigned long long col_base = (unsigned long long)(in_f + T.cb * E::kRow);
+  const int row_bytes = ts < E::kRow ? ts : E::kRow;
+  const unsigned lds_base = (unsigned)(unsigned long long)lds;
+
+  // DMA lane mapping and resources exactly as rgd_tile
+  const int srow = lane / E::kChunks, slot = lane % E::kChunks;
+  const int chunk = E::dma_chunk(slot, wave * C::kSpd + srow);
+  const int dvoff = srow * ts + 16 * chunk;
+  const int dvoff0 = (T.rb * XG_TILE + E::kInputs * chunk < ninput) ? dvoff : 0x7fffff00;
+  const int dvoff1 = (!diag && T.cb * XG_TILE + E::kInputs * chunk < ninput) ? dvoff : 0x7fffff00;
+  auto dma = [&](int st, int side, int j) {
+    const int g = 4 * j + wave;
+    const int s0 = st * STG + g * C::kSpd;
+    const int rows = min(ntime - s0, C::kSpd);
+    const int nrec = max(rows * ts - ts + row_bytes, 0);
+    const unsigned long long a = (side ? col_base : row_base) + (unsigned long long)(long long)s0 * ts;
+    v4i rs;
+    rs.x = (int)(unsigned)a;
+    rs.y = (int)((a >> 32) & 0xffff);
+    rs.z = nrec;
+    rs.w = 0x00020000;
+    const unsigned m0 = lds_base + (st & (NBUF - 1)) * C::kBuf + side * C::kSide + g * 1024;
+    rg_dma16(rs, side ? dvoff1 : dvoff0, m0);
+  };
+  auto dma_stage = [&](int st) {
+#pragma unroll
+    for (int side = 0; side < 2; ++side)
+#pragma unroll
+      for (int j = 0; j < C::kDma; ++j) dma(st, side, j);
+  };
+
+  const int wr = wave >> 1, wc = wave & 1;
+  // operand offsets: 16-real block b of the wave is chunk 8 w + b of its side
+  const int aoff = rg16_lane_off(8 * wr, lane) - (rg_slot(8 * wr, (lane & 15) >> 1) << 4);
+  const int boff = rg16_lane_off(8 * wc, lane) - (rg_slot(8 * wc, (lane & 15) >> 1) << 4) +
+                   (diag ? 0 : C::kSide);
+  const int s7 = (lane & 15) >> 1;  // the sample's swizzle phase (sample offsets are multiples of 8)
+  v4i acc[8][8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = (v4i)0;
+
+  v4i A0[8], B0[8], A1[8], B1[8];
+  auto operand = [&](const unsigned char *base, int off, int blk) {
+    return rg_operand(base + off + (rg_slot(blk, s7) << 4));
+  };
+  // One kstep (64 samples): MFMA i on (A, B) in snake order, the 16 operand
+  // reads of the next stage over the first 32, the stage's 8 DMAs over the
+  // last 32.
+  auto kstep = [&](const v4i *A, const v4i *B, v4i *NA, v4i *NB, const unsigned char *nbase,
+                   int st) {
+#pragma unroll
+    for (int i = 0; i < 64; ++i) {
+      const int a = i >> 3, b = ((i >> 3) & 1) ? 7 - (i & 7) : (i & 7);
+      acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[a], B[b], acc[a][b], 0, 0, 0);
+      if (i < 32 && (i & 1) == 0) {
+        const int k = i >> 1;  // operand k: A0..A7 then B0..B7
+        if (k < 8)
+          NA[k] = operand(nbase, aoff, 8 * wr + k);
+        else
+          NB[k - 8] = operand(nbase, boff, 8 * wc + k - 8);
+      }
+      if (i >= 32 && (i & 3) == 0) {
+        const int d = (i - 32) >> 2;  // DMA d of the stage: side d / 4, j = d % 4
+        dma(st, d >> 2, d & 3);
+      }
+    }
+  };
+
+  // prologue: DMAs of stages 0 .. NBUF-1, wait for stage 0
+#pragma unroll
+  for (int st = 0; st < NBUF; ++st) dma_stage(st);
+  rgd_barrier<(NBUF - 1) * 2 * C::kDma>();
+  const int nrun = (nstage + 1) & ~1;  // operand sets alternate: stages in pairs
+  if (diag && wr < wc) {
+    // the mirror wave of a diagonal tile: its DMAs and barriers, no MFMAs
+    for (int s = 0; s < nrun; ++s) {
+      rgd_barrier<(NBUF - 2) * 2 * C::kDma>();
+      dma_stage(s + NBUF);
+    }
+    rgd_barrier<0>();
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    A0[k] = operand(lds, aoff, 8 * wr + k);
+    B0[k] = operand(lds, boff, 8 * wc + k);
+  }
+  for (int s = 0; s < nrun; s += 2) {
+    // stage s + 1 has landed (this wave's DMAs; the barrier covers the
+    // others'), and every wave is done reading stage s's buffer, which is
+    // refilled with stage s + NBUF during the kstep
+    rgd_barrier<(NBUF - 2) * 2 * C::kDma>();
+    kstep(A0, B0, A1, B1, lds + ((s + 1) & (NBUF - 1)) * C::kBuf, s + NBUF);
+    rgd_barrier<(NBUF - 2) * 2 * C::kDma>();
+    kstep(A1, B1, A0, B0, lds + ((s + 2) & (NBUF - 1)) * C::kBuf, s + 1 + NBUF);
+  }
+  rgd_barrier<0>();
+  int tid2 = threadIdx.x;
+  asm volatile("" : "+v"(tid2));  // lane-derived values computed here, not hoisted
+  const int lane2 = tid2 & 63;
+  if (!(diag && wr < wc)) rg16_store_cells(J, T, acc, wr, wc, lane2, RgBperm{4 * (lane2 ^ 1)}, lds);
+}
+
+template <int NBUF>
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
+void xg_xengine_rgd16(XgLaunch L) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[RgdCfg<RgInt8, 64, NBUF>::kLds];
+  const int xcd = blockIdx.x % XG_NXCD;
+  const int wl = blockIdx.x / XG_NXCD;
+  TileRef T;
+  for (int k = wl; xg_tile(L, xcd, k, T); k += L.wg_per_xcd) {
+    rgd16_tile<NBUF>(L.jobs[T.job], T, lds);
+    rg_barrier();  // every wave is done with this tile's LDS buffers
+  }
+}
+
 template <int STG, int NBUF>
 __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
 void xg_xengine_rgd(XgLaunch L) {
@@ -2092,6 +2325,8 @@ extern "C" int xgh_launch_xengine(int family, const XgLaunch *L, void *stream) {
         hipLaunchKernelGGL((xg_xengine_rgd<64, 4>), dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       else if (L->stage == 4)
         hipLaunchKernelGGL((xg_xengine_rgd<32, 8>), dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      else if (L->stage == 5)
+        hipLaunchKernelGGL((xg_xengine_rgd16<4>), dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       else if (L->stage == 1)
         hipLaunchKernelGGL(xg_xengine_rg<128>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       else

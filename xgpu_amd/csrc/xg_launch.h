@@ -82,7 +82,9 @@ typedef struct XgLaunch {
   char stage;                                   /* real-Gram kernel: 0 / 1 = register
                                                    staging, 64 / 128 samples per stage;
                                                    3 / 4 = global->LDS DMA staging, 64 /
-                                                   32 samples per stage; fp16: 0 / 1 =
+                                                   32 samples per stage; 5 = DMA staging,
+                                                   64 samples per stage, 16x16x64 MFMA
+                                                   tiles; fp16: 0 / 1 =
                                                    32 / 64 samples per stage, register
                                                    staging; 2 = DMA staging, 32 */
   int xcd_start[XG_NXCD][XG_MAX_JOBS + 1];      /* per-XCD prefix of tile counts */
