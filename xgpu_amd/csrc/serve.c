@@ -382,6 +382,12 @@ static void *scheduler_main(void *arg) {
     pthread_mutex_lock(&s->mu);
     if (!ok) {
       set_error(s, "batch launch");
+      if (ev_copy) put_event(s, ev_copy);
+      if (rb && rb->event) put_event(s, rb->event);
+      for (int i = 0; i < ndump; ++i) {
+        put_event(s, dumps[i]->event);
+        free(dumps[i]);
+      }
       free(rb);
       for (int i = 0; i < n; ++i)
         for (int k = 0; k < jobs[i].nblk; ++k) free(jobs[i].blk[k]);
@@ -396,6 +402,8 @@ static void *scheduler_main(void *arg) {
         rc->kind = 2;
         rc->event = ev_copy;
         push_record(s, rc);
+      } else {
+        put_event(s, ev_copy); /* re-recording a pending event is allowed */
       }
     }
     s->inflight++;

@@ -177,9 +177,14 @@ int xgpuInit(XGPUContext *context, int device_flags) {
     context->internal = NULL;
     return rc == XGPU_OUT_OF_MEMORY ? XGPU_OUT_OF_MEMORY : XGPU_CUDA_ERROR;
   }
-  if (in->register_array && (rc = xgpuSetHostInputBuffer(context)) != XGPU_OK) return rc;
-  if (in->register_matrix && (rc = xgpuSetHostOutputBuffer(context)) != XGPU_OK) return rc;
-  return xgpuClearDeviceIntegrationBuffer(context);
+  if ((in->register_array && (rc = xgpuSetHostInputBuffer(context)) != XGPU_OK) ||
+      (in->register_matrix && (rc = xgpuSetHostOutputBuffer(context)) != XGPU_OK) ||
+      (rc = xgpuClearDeviceIntegrationBuffer(context)) != XGPU_OK) {
+    /* no half-initialised context: a failed init leaves nothing to free */
+    xgpuFree(context);
+    return rc;
+  }
+  return XGPU_OK;
 }
 
 int xgpuClearDeviceIntegrationBuffer(XGPUContext *context) {
