@@ -13,6 +13,13 @@
  *
  * All functions return XGPU_OK (0) or an xgpu.h error code; the message of
  * the last failure on the calling thread is xgpuAmdLastError().
+ *
+ * Tracing: with the environment variable XGPU_TRACE set ("1" or "stderr":
+ * stderr; any other value except "0": a file appended to), the library logs
+ * one line per X-engine launch (issued, captured into a hipGraph or replayed),
+ * spill and dump: microseconds since the first line, kernel, workgroups,
+ * stream and per job the channels, stations, samples, store / accumulate and
+ * tiles.  Read once per process.
  */
 #ifndef XGPU_AMD_H
 #define XGPU_AMD_H

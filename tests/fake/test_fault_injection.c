@@ -322,7 +322,61 @@ static void server_runs_and_reports_faults(void) {
   printf("ok server: exact dumps, and an injected launch failure is reported by drain\n");
 }
 
-int main(void) {
+/* ---- XGPU_TRACE (run as "test_fault_injection trace <file>": the switch is
+ * read once per process) ---------------------------------------------------- */
+static int count_lines(const char *path, const char *needle) {
+  FILE *f = fopen(path, "r");
+  if (!f) return -1;
+  char line[4096];
+  int n = 0;
+  while (fgets(line, sizeof line, f))
+    if (strstr(line, needle)) ++n;
+  fclose(f);
+  return n;
+}
+
+static void trace_log(const char *path) {
+  const int ns = 8, nf = 3, nt = 64;
+  const size_t nb = (size_t)nt * nf * ns * 4;
+  int8_t *x = malloc(nb);
+  fill(x, nb, 5);
+  const size_t n = (size_t)nf * ns * (ns + 1) / 2 * 8;
+  int64_t *got = malloc(n * sizeof(int64_t));
+  void *d = NULL;
+  CHECK(xgpuAmdMalloc(&d, nb) == XGPU_OK && xgpuAmdMemcpy(d, x, nb, XGPU_H2D) == XGPU_OK, "upload");
+  XGPUAmdConfig cfg = config(ns, nf, REGISTER_TILE_TRIANGULAR_ORDER);
+  XGPUAmdEngine *e = NULL;
+  CHECK(xgpuAmdCreate(&e, &cfg) == XGPU_OK, "create");
+  CHECK(xgpuAmdCorrelate(e, d, nt) == XGPU_OK, "eager");             /* 1 launch */
+  CHECK(xgpuAmdDump(e, got, XGPU_DUMP_WIDE | XGPU_DUMP_TRIANGULAR) == XGPU_OK, "dump");
+  XGPUAmdTuning t;
+  xgpuAmdGetTuning(e, &t);
+  t.graph = 1;
+  CHECK(xgpuAmdSetTuning(e, &t) == XGPU_OK, "graph knob");
+  CHECK(xgpuAmdClear(e) == XGPU_OK, "clear");
+  CHECK(xgpuAmdCorrelate(e, d, nt) == XGPU_OK, "capture + replay"); /* store step */
+  CHECK(xgpuAmdClear(e) == XGPU_OK, "clear");
+  CHECK(xgpuAmdCorrelate(e, d, nt) == XGPU_OK, "replay");           /* same step again */
+  xgpuAmdDestroy(e);
+  xgpuAmdFree(d);
+  free(x);
+  free(got);
+  CHECK(count_lines(path, "[xgpu-trace]") >= 5, "trace lines");
+  CHECK(count_lines(path, " launch kernel=") == 1, "eager launch lines: %d", count_lines(path, " launch kernel="));
+  CHECK(count_lines(path, " capture kernel=") == 1, "capture lines");
+  CHECK(count_lines(path, " replay kernel=") == 2, "replay lines");
+  CHECK(count_lines(path, "nf=3 ns=8 nt=64 store") == 4, "job fields");
+  CHECK(count_lines(path, " dump engine=") == 1, "dump lines");
+  printf("ok XGPU_TRACE logs launches, captures, replays and dumps\n");
+}
+
+int main(int argc, char **argv) {
+  if (argc == 3 && strcmp(argv[1], "trace") == 0) {
+    remove(argv[2]);
+    setenv("XGPU_TRACE", argv[2], 1);
+    trace_log(argv[2]);
+    return 0;
+  }
   xgpu_h_pipeline();
   xgpu_init_faults();
   xgpu_xengine_faults();

@@ -37,3 +37,21 @@ def test_host_runtime_under_fault_injection():
                      "server: exact dumps"):
         assert "ok " + scenario in r.stdout, r.stdout
     assert "runtime error" not in out  # UBSan
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None or shutil.which("make") is None,
+                    reason="needs gcc and make")
+def test_trace_switch_logs_device_work(tmp_path):
+    """XGPU_TRACE=<file>: one line per launch, capture, replay and dump."""
+    b = subprocess.run(["make", "-s", "-C", FAKE], capture_output=True, text=True, timeout=600)
+    assert b.returncode == 0, b.stdout + b.stderr
+    log = tmp_path / "trace.log"
+    r = subprocess.run([os.path.join(FAKE, "test_fault_injection"), "trace", str(log)],
+                       capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1"))
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = log.read_text().splitlines()
+    assert all(l.startswith("[xgpu-trace]") for l in lines)
+    kinds = [l.split()[3] for l in lines]
+    assert kinds == ["launch", "dump", "capture", "replay", "replay"], lines
+    assert "kernel=xg_xengine_rgd<32,8>" in lines[0] and "nf=3 ns=8 nt=64 store" in lines[0]

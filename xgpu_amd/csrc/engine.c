@@ -14,11 +14,13 @@
  *   - int32 (or f32) integrations are spilled into an int64 (or f64) buffer
  *     before they could overflow, so integrations of any length are exact.
  */
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "engine_internal.h"
 #include "xg_hip.h"
@@ -45,6 +47,89 @@ static int hipfail(const char *what) {
   } while (0)
 
 const char *xgpuAmdLastError(void) { return g_msg; }
+
+/* ---- XGPU_TRACE ----------------------------------------------------------
+ * The environment variable XGPU_TRACE, read once per process, switches on a
+ * log of the engines' device work: "1" (or "stderr") writes to stderr, any
+ * other value except "0" is a file that lines are appended to.  One line per
+ * X-engine launch (issued, captured into a hipGraph, or replayed from one),
+ * spill and dump, with microseconds since the first line, the kernel, its
+ * grid and stream and, per job, the channels, stations, samples, whether it
+ * stores or accumulates and its tiles.  It records what was issued and when;
+ * kernel durations come from the HIP events of xgpuAmdTimeCorrelate or a
+ * profiler.  (The reference's counterpart is its compile-time CUBE macros and
+ * TIME_CUDA_CALLS / RUNTIME_STATS switches, reference src/cube/cube.h:301-420,
+ * src/cuda_xengine.cu:103-109.) */
+static pthread_once_t g_trace_once = PTHREAD_ONCE_INIT;
+static FILE *g_trace;
+static double g_trace_t0;
+static _Thread_local const char *g_trace_how = "launch"; /* "capture" while capturing */
+
+static double mono_us(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return 1e6 * (double)ts.tv_sec + 1e-3 * (double)ts.tv_nsec;
+}
+
+static void trace_open(void) {
+  const char *v = getenv("XGPU_TRACE");
+  if (!v || !*v || strcmp(v, "0") == 0) return;
+  g_trace = (strcmp(v, "1") == 0 || strcmp(v, "stderr") == 0) ? stderr : fopen(v, "a");
+  g_trace_t0 = mono_us();
+}
+
+static FILE *trace_out(void) {
+  pthread_once(&g_trace_once, trace_open);
+  return g_trace;
+}
+
+/* The kernel an X-engine launch runs (the dispatch of xgh_launch_xengine). */
+static const char *kernel_label(int family, const XgLaunch *L) {
+  switch (family) {
+    case XG_KERNEL_I8_MFMA: return "xg_xengine_mfma<int8>";
+    case XG_KERNEL_F16_MFMA: return "xg_xengine_mfma<fp16>";
+    case XG_KERNEL_F32_PLAIN: return "xg_xengine_f32";
+    case XG_KERNEL_I8_KG: return "xg_xengine_kg";
+    case XG_KERNEL_I8_RG:
+      return L->stage == 3 ? "xg_xengine_rgd<64,4>" : L->stage == 4 ? "xg_xengine_rgd<32,8>"
+           : L->stage == 5 ? "xg_xengine_rgd16<4>" : L->stage == 1 ? "xg_xengine_rg<128>"
+           : "xg_xengine_rg<64>";
+    case XG_KERNEL_F16_RG:
+      return L->stage == 2 ? "xg_xengine_rgdh<32,4>" : L->stage == 1 ? "xg_xengine_rgh<64>"
+           : "xg_xengine_rgh<32>";
+    default: return "unknown";
+  }
+}
+
+static void trace_launch(const char *how, int family, const XgLaunch *L, const void *stream) {
+  FILE *f = trace_out();
+  if (!f) return;
+  char line[160 + 96 * XG_MAX_JOBS];
+  int n = snprintf(line, sizeof line, "[xgpu-trace] %12.1f us %s kernel=%s workgroups=%d threads=%d stream=%p jobs=%d",
+                   mono_us() - g_trace_t0, how, kernel_label(family, L), 8 * L->wg_per_xcd,
+                   XG_WG_THREADS, stream, L->njobs);
+  for (int i = 0; i < L->njobs && n < (int)sizeof line - 96; ++i) {
+    const XgJob *J = &L->jobs[i];
+    n += snprintf(line + n, sizeof line - n, " | nf=%d ns=%d nt=%d %s tiles=%d", J->nfreq,
+                  J->nstation, J->ntime, J->accumulate ? "accumulate" : "store",
+                  J->tiles_per_chan * J->nfreq);
+  }
+  fprintf(f, "%s\n", line);
+  fflush(f);
+}
+
+static void trace_event(const XGPUAmdEngine *e, const char *what, const char *fmt, ...) {
+  FILE *f = trace_out();
+  if (!f) return;
+  char detail[160];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(detail, sizeof detail, fmt, ap);
+  va_end(ap);
+  fprintf(f, "[xgpu-trace] %12.1f us %s engine=%p stream=%p %s\n", mono_us() - g_trace_t0, what,
+          (const void *)e, e->stream, detail);
+  fflush(f);
+}
 
 /* Largest number of samples an int32 integration can hold for any int8
  * input: |ar*br + ai*bi| <= 2*128*128 per sample, and both accumulators of
@@ -324,6 +409,8 @@ int xgpuAmdGetTuning(const XGPUAmdEngine *e, XGPUAmdTuning *t) {
  * them to a cleared buffer. */
 int xg_spill(XGPUAmdEngine *e) {
   const size_t n = e->info.matLength * 2;
+  trace_event(e, "spill", "samples=%u %s", e->samples_in_integ,
+              e->spilled ? "add" : "first");
   if (!e->wide) HIPCALL(xgh_malloc(&e->wide, n * 8), "spill buffer");
   if (e->cleared) { /* nothing integrated since the clear */
     if (!e->spilled) HIPCALL(xgh_launch_fill_zero(e->wide, n * 8, e->stream), "spill clear");
@@ -390,6 +477,7 @@ static void advance(XGPUAmdEngine *e, unsigned ntime) {
 int xg_launch_single(XGPUAmdEngine *e, const void *input_d, unsigned ntime) {
   XgLaunch L;
   const int family = single_launch(e, input_d, ntime, &L);
+  trace_launch(g_trace_how, family, &L, e->stream);
   HIPCALL(xgh_launch_xengine(family, &L, e->stream), "xengine kernel");
   advance(e, ntime);
   return XGPU_OK;
@@ -430,12 +518,13 @@ static int convert_input(XGPUAmdEngine *e, void *dst, const void *src, unsigned 
 /* Issue a step eagerly: conversions (engines[i] converts when raw[i] differs
  * from the job's input), then the launch. */
 static int issue_step(XGPUAmdEngine *const *engines, const void *const *raw, const XgLaunch *L,
-                      int family, void *s) {
+                      int family, void *s, const char *how) {
   for (int i = 0; i < L->njobs; ++i)
     if (raw[i] != L->jobs[i].input && L->jobs[i].ntime > 0) {
       int rc = convert_input(engines[i], (void *)L->jobs[i].input, raw[i], (unsigned)L->jobs[i].ntime, s);
       if (rc) return rc;
     }
+  trace_launch(how, family, L, s);
   HIPCALL(xgh_launch_xengine(family, L, s), "xengine kernel");
   return XGPU_OK;
 }
@@ -444,7 +533,7 @@ static int issue_step(XGPUAmdEngine *const *engines, const void *const *raw, con
  * first when the owner's graph knob is on; otherwise issue it eagerly. */
 static int run_step(XGPUAmdEngine *owner, XGPUAmdEngine *const *engines, const void *const *raw,
                     const XgLaunch *L, int family, void *s) {
-  if (!owner->use_graph) return issue_step(engines, raw, L, family, s);
+  if (!owner->use_graph) return issue_step(engines, raw, L, family, s, "launch");
   if (!owner->steps) {
     owner->steps = (struct XgStepGraph *)calloc(XG_STEP_GRAPHS, sizeof *owner->steps);
     if (!owner->steps) return xg_fail(XGPU_OUT_OF_MEMORY, "out of host memory");
@@ -463,7 +552,7 @@ static int run_step(XGPUAmdEngine *owner, XGPUAmdEngine *const *engines, const v
       owner->nsteps--;
     }
     HIPCALL(xgh_graph_begin(s), "begin capture");
-    const int rc = issue_step(engines, raw, L, family, s);
+    const int rc = issue_step(engines, raw, L, family, s, "capture");
     void *exec = NULL;
     const int rc_end = xgh_graph_end(s, &exec);
     if (rc || rc_end) {
@@ -479,6 +568,7 @@ static int run_step(XGPUAmdEngine *owner, XGPUAmdEngine *const *engines, const v
     t->stream = s;
     t->exec = exec;
   }
+  trace_launch("replay", family, L, s);
   HIPCALL(xgh_graph_launch(owner->steps[g].exec, s), "graph launch");
   return XGPU_OK;
 }
@@ -692,6 +782,9 @@ int xg_dump_async(XGPUAmdEngine *e, void *dst, int flags) {
   const void *val = NULL;
   size_t bytes = 0;
   const int tri = (flags & XGPU_DUMP_TRIANGULAR) && e->cfg.matrix_order != TRIANGULAR_ORDER;
+  trace_event(e, "dump", "dst=%p%s%s%s samples=%llu", dst, (flags & XGPU_DUMP_DEVICE) ? " device" : " host",
+              (flags & XGPU_DUMP_WIDE) ? " wide" : "", (flags & XGPU_DUMP_TRIANGULAR) ? " triangular" : "",
+              (unsigned long long)e->samples);
   /* A triangular dump to device memory is reordered straight into dst. */
   int rc = dump_stage(e, flags, (tri && (flags & XGPU_DUMP_DEVICE)) ? dst : NULL, &val, &bytes);
   if (rc) return rc;
@@ -806,9 +899,11 @@ int xgpuAmdCorrelateHost(XGPUAmdEngine *e, const void *input_h, int syncOp, void
       const unsigned s_in = e->samples_in_integ;
       const unsigned long long s_all = e->samples;
       HIPCALL(xgh_graph_begin(e->stream), "begin capture");
+      g_trace_how = "capture";
       rc = xgh_event_record(e->ev_join, e->stream) || xgh_stream_wait_event(e->copy_stream, e->ev_join)
                ? XGPU_CUDA_ERROR
                : host_issue(e, input_h, dbase, npipe, ntp, chunk);
+      g_trace_how = "launch";
       if (rc == XGPU_OK && (xgh_event_record(e->ev_join, e->copy_stream) ||
                             xgh_stream_wait_event(e->stream, e->ev_join)))
         rc = XGPU_CUDA_ERROR;
@@ -831,6 +926,7 @@ int xgpuAmdCorrelateHost(XGPUAmdEngine *e, const void *input_h, int syncOp, void
       }
     }
     if (g >= 0) {
+      trace_event(e, "replay", "host-path graph: input=%p, %d chunks of %u samples", input_h, npipe, ntp);
       HIPCALL(xgh_graph_launch(e->graphs[g].exec, e->stream), "graph launch");
       e->cleared = 0;
       e->samples_in_integ += e->cfg.ntime;
