@@ -5,6 +5,11 @@ NSTATION=512 x 2 pol, NFREQUENCY=64, NTIME=8192 on input already resident in
 HBM (seeded uniform int8 in [-128, 127]), output left in the device
 integration buffer in REGISTER_TILE order.  2.754e11 complex MACs per step.
 
+Timing: the W warm-up and K timed steps run in sustained operation, after
+back-to-back integrations have brought the power-managed shader clock to its
+steady state; the first W + K steps of the process are reported beside them
+as the cold start.
+
 Prints one JSON line: whole-job TCMAC/s, the fraction of the int8 MFMA peak,
 the FP32-plain kernel on the same workload, the FP16 MFMA path on its own
 workload, the roofline of the dominant kernel (timed with HIP events on the
@@ -29,18 +34,33 @@ PEAK_INT8_TOPS = 5000.0     # MI355X dense int8 MFMA, 10^12 int ops/s
 PEAK_FP16_TFLOPS = 2500.0   # MI355X dense fp16 MFMA
 PEAK_FP32_TFLOPS = 157.3    # MI355X fp32 vector
 NS, NF, NT = 512, 64, 8192
-CPU_SAMPLE = (512, 1, 8192)  # one channel of the workload
-CPU_THREADS = 16             # the box's CPU share for one GPU
+CPU_SAMPLE = (512, 16, 8192)  # 16 channels of the workload, one reference call per channel
+CPU_THREADS = 16               # the box's CPU share for one GPU
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(threads: int):
     """Reference xgpuOmpXengine (oracle/_ref, compiled from the reference's own
-    omp_xengine.c) on `threads` host cores, run in a child process pinned to
-    those cores (the reference sizes its OpenMP team by omp_get_num_procs)."""
+    omp_xengine.c for NSTATION=512, NFREQUENCY=1, NTIME=8192) on `threads` host
+    cores, run in a child process pinned to those cores (the reference sizes
+    its OpenMP team by omp_get_num_procs).  One sample is 16 channels of the
+    workload, one reference call per channel (about 3 s on 16 cores); median
+    of 3 samples after a one-channel warm-up."""
     code = r"""
 import os, sys, time, json
 import numpy as np
@@ -49,39 +69,43 @@ import oracle
 threads = %d
 cpus = sorted(os.sched_getaffinity(0))[:threads]
 os.sched_setaffinity(0, cpus)
-ns, nf, nt = %r
-x = oracle.fill_uniform(nt * nf * ns * 4, 2, -128, 127)
-L = oracle.ref_lib("ns%%d_nf%%d_nt%%d" %% (ns, nf, nt))
+ns, nch, nt = %r
+xs = [oracle.fill_uniform(nt * ns * 4, 2 + c, -128, 127) for c in range(nch)]
+L = oracle.ref_lib("ns%%d_nf1_nt%%d" %% (ns, nt))
 kind = "reference" if L is not None else "port"
-out = np.zeros(2 * oracle.tri_len(ns, nf), dtype=np.float32)
-def run():
-    t0 = time.perf_counter()
+out = np.zeros(2 * oracle.tri_len(ns, 1), dtype=np.float32)
+def one(x):
     if L is not None:
         L.xgpuOmpXengine(out.ctypes.data, x.ctypes.data)
     else:
-        oracle.xengine_f32(x, ns, nf, nt, len(cpus))
+        oracle.xengine_f32(x, ns, 1, nt, len(cpus))
+def run():
+    t0 = time.perf_counter()
+    for x in xs:
+        one(x)
     return time.perf_counter() - t0
-run()
+one(xs[0])
 ts = sorted(run() for _ in range(3))
 print(json.dumps({"sec": ts[1], "kind": kind, "cores": len(cpus)}))
 """ % (ROOT, threads, CPU_SAMPLE)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
-                       timeout=600)
+                       timeout=900)
     if r.returncode != 0:
         log(r.stderr[-2000:])
         return None
     res = json.loads(r.stdout.strip().splitlines()[-1])
-    ns, nf, nt = CPU_SAMPLE
-    cmac = nf * nt * ns * (ns + 1) // 2 * 4
+    ns, nch, nt = CPU_SAMPLE
+    cmac = nch * nt * ns * (ns + 1) // 2 * 4
     return {
         "value": cmac / res["sec"] / 1e12,
         "unit": "TCMAC/s",
         "cores": res["cores"],
         "kind": res["kind"],
-        "sample": f"xgpuOmpXengine (float build) on NSTATION=512, NFREQUENCY=1, NTIME=8192 "
-                  f"(one channel of the workload, {cmac:.3e} CMAC), median of 3 after a warm-up, "
-                  f"{res['sec']:.2f} s per run",
+        "cpu": host_cpu_model(),
+        "sample": f"xgpuOmpXengine (float build, NSTATION=512, NFREQUENCY=1, NTIME=8192) called "
+                  f"once per channel on {nch} channels of the workload ({cmac:.3e} CMAC), "
+                  f"median of 3 samples after a one-channel warm-up, {res['sec']:.2f} s per sample",
     }
 
 
@@ -89,10 +113,11 @@ def serving_bench(X, seed=5, dumps_per_stream=4, loads=(0.3, 0.6, 0.9)):
     """Serving mix (xgpu_amd.workloads.serving_mix: 8 subarray streams, NS from
     {64 .. 512}, NF in [8, 128], block NT from {256 .. 4096}, K in [4, 64]
     blocks per dump), one uniform int8 block per stream, submitted
-    dumps_per_stream * K times.  Two sources: pinned host blocks (staged to
-    HBM per job: bound by the host link) and blocks already in HBM (a ring of
-    K copies per stream, so an integration's blocks lie back to back and
-    fuse).  Per source: everything submitted at once (sustained rate), then
+    dumps_per_stream * K times.  Three modes: pinned host blocks (staged to
+    HBM per job: bound by the host link), blocks already in HBM (a ring of K
+    copies per stream, so an integration's blocks lie back to back and fuse)
+    with dumps copied to host rings, and HBM blocks with dumps left in device
+    rings.  Per mode: everything submitted at once (sustained rate), then
     Poisson arrivals at each `loads` fraction of that rate with block-to-dump
     latency percentiles.  Every stream's last dump is verified: its
     autocorrelations against K x the block's input power, and every
@@ -113,9 +138,10 @@ def serving_bench(X, seed=5, dumps_per_stream=4, loads=(0.3, 0.6, 0.9)):
             X.check(X.core().xgpuAmdMemcpy(d.ptr + r * hb.nbytes, d.ptr, hb.nbytes, 3))
         ring.append(d)
 
-    def run(device, arrival_rate=None):
+    def run(device, arrival_rate=None, dump_device=False):
         srv = X.Server(staging_bytes=8 << 30)
-        opened = [srv.open_stream(ns, nf, k, dump_slots=1) for ns, nf, nt, k in streams]
+        opened = [srv.open_stream(ns, nf, k, dump_slots=1, dump_device=dump_device)
+                  for ns, nf, nt, k in streams]
         order = []
         for i, (ns, nf, nt, k) in enumerate(streams):
             order += [i] * (k * dumps_per_stream)
@@ -137,7 +163,7 @@ def serving_bench(X, seed=5, dumps_per_stream=4, loads=(0.3, 0.6, 0.9)):
             count[i] += 1
         srv.drain()
         st = srv.stats()
-        dumps = [r.array.copy() for _, r in opened]
+        dumps = [r.download(np.int64) if dump_device else r.array.copy() for _, r in opened]
         srv.close()
         return st, len(order), dumps
 
@@ -157,10 +183,11 @@ def serving_bench(X, seed=5, dumps_per_stream=4, loads=(0.3, 0.6, 0.9)):
 
     out = {"streams": [{"nstation": ns, "nfrequency": nf, "ntime": nt, "blocks_per_dump": k}
                        for ns, nf, nt, k in streams],
-           "dumps": "exact int64, triangular order, copied to host"}
-    for device in (False, True):
-        run(device)  # warm-up: engines, staging, code paths
-        sat, nblocks, dumps = run(device)
+           "dumps": "exact int64, triangular order, copied to a host ring (device_blocks_"
+                    "device_dumps: left in a device ring)"}
+    for device, dump_device in ((False, False), (True, False), (True, True)):
+        run(device, dump_device=dump_device)  # warm-up: engines, staging, code paths
+        sat, nblocks, dumps = run(device, dump_device=dump_device)
         rate = nblocks / sat["busy_s"]
         res = {"saturated": {"blocks_per_s": round(rate, 1), "tcmac_s": round(sat["tcmac_per_s"], 3),
                              "p50_ms": round(sat["p50_ms"], 3), "p99_ms": round(sat["p99_ms"], 3),
@@ -169,7 +196,7 @@ def serving_bench(X, seed=5, dumps_per_stream=4, loads=(0.3, 0.6, 0.9)):
                "poisson": []}
         verified = verify(dumps)
         for load in loads:
-            lat, _, dumps = run(device, arrival_rate=load * rate)
+            lat, _, dumps = run(device, arrival_rate=load * rate, dump_device=dump_device)
             verified &= verify(dumps)
             res["poisson"].append({"load": load, "offered_blocks_per_s": round(load * rate, 1),
                                    "tcmac_s": round(lat["tcmac_per_s"], 3),
@@ -178,7 +205,8 @@ def serving_bench(X, seed=5, dumps_per_stream=4, loads=(0.3, 0.6, 0.9)):
                                    "mean_batch": round(lat["mean_batch"], 2),
                                    "mean_fused": round(lat["mean_fused"], 2)})
         res["verified"] = verified
-        key = "device_blocks" if device else "host_blocks"
+        key = ("device_blocks_device_dumps" if dump_device else "device_blocks") if device \
+            else "host_blocks"
         res["input"] = ("blocks resident in HBM, a ring of K copies per stream (an integration's "
                         "blocks back to back, fused into jobs)" if device else
                         "pinned host blocks, uniform int8 [-127,127], staged to HBM per job")
@@ -360,7 +388,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     # the first ~25 back-to-back integrations run while the GPU's power state settles
     # (durations 1.0 -> 1.5 -> 1.08 ms); the default warm-up covers that transient
-    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--only", choices=["int8", "fp32", "fp16", "copy", "fft", "dump"], default=None,
@@ -393,18 +421,37 @@ def main():
     tun = eng.get_tuning()
     kname = kernel_name(tun)
 
-    for _ in range(args.warmup):
-        eng.clear()
-        eng.correlate(buf, NT)
-    X.core().xgpuAmdDeviceSynchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.clear()
-        eng.correlate(buf, NT)
-    X.core().xgpuAmdDeviceSynchronize()
-    wall = time.perf_counter() - t0
+    def steps(k):
+        """k steps back to back between device synchronisations: wall seconds."""
+        X.core().xgpuAmdDeviceSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            eng.clear()
+            eng.correlate(buf, NT)
+        X.core().xgpuAmdDeviceSynchronize()
+        return time.perf_counter() - t0
+
+    # Cold start: the first W + K steps of the process.  The shader clock is
+    # power-managed and settles over the first ~35 back-to-back integrations
+    # (about 40 ms), so these steps run slower than sustained operation.
+    steps(args.warmup)
+    cold = steps(args.steps)
+    # Sustained operation (an X-engine runs continuously): back-to-back
+    # integrations until two consecutive 20-step windows agree within 1 %
+    # (at most 300 steps, ~0.3 s), then W warm-up and K timed steps.
+    settle, prev = 0, None
+    while settle < 300:
+        w = steps(20)
+        settle += 20
+        if prev is not None and abs(w - prev) <= 0.01 * prev:
+            break
+        prev = w
+    steps(args.warmup)
+    wall = steps(args.steps)
     ms_step = wall / args.steps * 1e3
     value = cmac * args.steps / wall / 1e12
+    log(f"[bench] cold start {cold / args.steps * 1e3:.3f} ms/step; sustained after {settle} "
+        f"settling steps {ms_step:.3f} ms/step")
     # kernel-only time with HIP events on the engine stream
     ms_kernel = eng.time_correlate(buf, NT, warmup=1, reps=args.steps)
     achieved_tops = cmac * 8 / (ms_kernel * 1e-3) / 1e12
@@ -490,8 +537,10 @@ def main():
     ns3, nf3, nt3 = 2048, 16, 4096
     cfg3 = None
     try:
-        b3 = X.DeviceBuffer(nt3 * nf3 * ns3 * 4)
-        b3.fill_uniform_int8(3, -128, 127)
+        from xgpu_amd import workloads as W
+        x3 = W.cfg3_input(ns3, nf3, nt3, seed=3)  # Gaussian sigma 20 plus a tone, int8
+        b3 = X.DeviceBuffer.from_array(x3)
+        del x3
         e3 = X.Engine(ns3, nf3, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
         # warm-up past the power-state transient after the host-path phase
         ms3 = e3.time_correlate(b3, nt3, warmup=25, reps=10)
@@ -512,7 +561,9 @@ def main():
         dump_bytes = (e3.mat_len + e3.tri_len) * 8
         e3.close()
         c3 = X.cmac_per_integration(ns3, nf3, nt3)
-        cfg3 = {"workload": "NSTATION=2048 x 2 pol, NFREQUENCY=16, NTIME=4096, int8",
+        cfg3 = {"workload": "NSTATION=2048 x 2 pol, NFREQUENCY=16, NTIME=4096, int8 input "
+                            "(round(N(0, 20^2)) per component plus a tone of amplitude 8 with a "
+                            "random phase per station, seed 3)",
                 "kernel_ms": round(ms3, 4), "tcmac_s": round(c3 / (ms3 * 1e-3) / 1e12, 1),
                 "pct_int8_mfma_peak": round(100 * c3 * 8 / (ms3 * 1e-3) / 1e12 / PEAK_INT8_TOPS, 1),
                 "dump": {"what": "device reorder to triangular int32 (read register-tile, write "
@@ -629,6 +680,14 @@ def main():
             "cmac_per_step": cmac, "input": "uniform int8 [-128,127], seed 2",
         },
         "pct_int8_mfma_peak": round(100.0 * cmac * 8 / (ms_step * 1e-3) / 1e12 / PEAK_INT8_TOPS, 2),
+        "timing": "sustained: the W warm-up and K timed steps follow back-to-back integrations "
+                  "until two consecutive 20-step windows agree within 1 % (power_settle_steps); "
+                  "cold_start is the first W + K steps of the process",
+        "power_settle_steps": settle,
+        "cold_start": {"ms_per_step": round(cold / args.steps * 1e3, 4),
+                       "tcmac_s": round(cmac * args.steps / cold / 1e12, 3),
+                       "pct_int8_mfma_peak": round(100.0 * cmac * 8 * args.steps / cold / 1e12
+                                                   / PEAK_INT8_TOPS, 2)},
         "kernel_ms_median": round(ms_med, 4),
         "kernel_ms_p99": round(ms_p99, 4),
         "mfma_only_probe_tcmac_s": round(peak_meas, 1),

@@ -47,6 +47,9 @@ typedef struct XGPUStreamConfigStruct {
   unsigned int nfrequency;
   unsigned int compute_type;    /* XGPU_INT8 or XGPU_FLOAT16 */
   unsigned int blocks_per_dump; /* K >= 1 */
+  unsigned int dump_device;     /* 0: the dump ring is host memory (dumps are copied back);
+                                   1: device memory (dumps stay in HBM for a device-side
+                                   consumer: no transfer over the host link) */
 } XGPUStreamConfig;
 
 typedef struct XGPUServerStatsStruct {
@@ -68,9 +71,11 @@ int xgpuServerCreate(XGPUServer **server, const XGPUServerConfig *cfg);
 int xgpuServerDestroy(XGPUServer *server);
 
 /* Open a stream with a device-resident integration.  dump_ring is host memory
- * of dump_slots * xgpuServerDumpBytes() bytes; dump k lands in slot k %
- * dump_slots as interleaved (re, im) int64 (int8) or double (FP16) pairs in
- * TRIANGULAR_ORDER. */
+ * (or, with cfg->dump_device, device memory) of dump_slots *
+ * xgpuServerDumpBytes() bytes; dump k lands in slot k % dump_slots as
+ * interleaved (re, im) int64 (int8) or double (FP16) pairs in
+ * TRIANGULAR_ORDER.  xgpuServerDumpsDone(stream) > k means dump k is complete
+ * in its slot. */
 int xgpuServerOpenStream(XGPUServer *server, const XGPUStreamConfig *cfg, void *dump_ring,
                          unsigned int dump_slots, int *stream_id);
 size_t xgpuServerDumpBytes(XGPUServer *server, int stream_id);

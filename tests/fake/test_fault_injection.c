@@ -292,34 +292,52 @@ static void server_runs_and_reports_faults(void) {
   const size_t n = (size_t)nf * ns * (ns + 1) / 2 * 8;
   int64_t *ref = malloc(n * sizeof(int64_t));
   exact(ref, x, ns, nf, nt);
-  for (int inject = 0; inject < 2; ++inject) {
+  /* mode 0: host blocks, host ring; 1: the same with a failing launch;
+   * 2: device blocks, device ring */
+  for (int mode = 0; mode < 3; ++mode) {
+    const int inject = mode == 1, dev = mode == 2;
     XGPUServerConfig sc = {0, (size_t)1 << 20, 0, 1};
     XGPUServer *s = NULL;
     CHECK(xgpuServerCreate(&s, &sc) == XGPU_OK, "server");
-    XGPUStreamConfig st = {ns, nf, XGPU_INT8, K};
+    XGPUStreamConfig st = {ns, nf, XGPU_INT8, K, (unsigned)dev};
     int64_t *ring = malloc(n * sizeof(int64_t) * 2);
+    void *ring_d = NULL, *x_d = NULL;
+    if (dev) {
+      CHECK(xgpuAmdMalloc(&ring_d, n * sizeof(int64_t) * 2) == XGPU_OK, "device ring");
+      CHECK(xgpuAmdMalloc(&x_d, nb) == XGPU_OK && xgpuAmdMemcpy(x_d, x, nb, XGPU_H2D) == XGPU_OK,
+            "device block");
+    }
     int id = -1;
-    CHECK(xgpuServerOpenStream(s, &st, ring, 2, &id) == XGPU_OK, "open");
+    CHECK(xgpuServerOpenStream(s, &st, dev ? ring_d : (void *)ring, 2, &id) == XGPU_OK, "open");
     if (inject) xgh_fake_fail("launch", 1);
     int rc = XGPU_OK;
-    for (int k = 0; k < 2 * K && rc == XGPU_OK; ++k) rc = xgpuServerSubmit(s, id, x, nt);
+    for (int k = 0; k < 2 * K && rc == XGPU_OK; ++k)
+      rc = dev ? xgpuServerSubmitDevice(s, id, x_d, nt) : xgpuServerSubmit(s, id, x, nt);
     const int drc = xgpuServerDrain(s);
     if (inject) {
       CHECK(drc == XGPU_CUDA_ERROR || rc != XGPU_OK, "the failure is reported");
     } else {
       CHECK(drc == XGPU_OK, "drain");
       CHECK(xgpuServerDumpsDone(s, id) == 2, "dumps");
+      if (dev)
+        CHECK(xgpuAmdMemcpy(ring, ring_d, n * sizeof(int64_t) * 2, XGPU_D2H) == XGPU_OK, "ring");
       for (int dmp = 0; dmp < 2; ++dmp)
-        for (size_t i = 0; i < n; ++i) CHECK(ring[dmp * n + i] == K * ref[i], "dump %d %zu", dmp, i);
+        for (size_t i = 0; i < n; ++i)
+          CHECK(ring[dmp * n + i] == K * ref[i], "mode %d dump %d %zu", mode, dmp, i);
     }
     xgh_fake_reset();
     xgpuServerDestroy(s);
     free(ring);
+    if (dev) {
+      xgpuAmdFree(ring_d);
+      xgpuAmdFree(x_d);
+    }
     CHECK(xgh_fake_live_allocations() == 0, "leaked %lld", xgh_fake_live_allocations());
   }
   free(ref);
   free(x);
-  printf("ok server: exact dumps, and an injected launch failure is reported by drain\n");
+  printf("ok server: exact dumps (host and device rings), and an injected launch failure is "
+         "reported by drain\n");
 }
 
 /* ---- XGPU_TRACE (run as "test_fault_injection trace <file>": the switch is

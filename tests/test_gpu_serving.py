@@ -157,10 +157,12 @@ class MixStream:
             self.ring.free()
 
 
-def serve_mix(streams, **kw):
-    """One dump per stream; submissions interleaved round-robin."""
+def serve_mix(streams, dump_device=False, **kw):
+    """One dump per stream; submissions interleaved round-robin.  With
+    dump_device the dump rings are device memory."""
     srv = X.Server(**kw)
-    ids = {st.k: srv.open_stream(st.ns, st.nf, st.K, dump_slots=1) for st in streams}
+    ids = {st.k: srv.open_stream(st.ns, st.nf, st.K, dump_slots=1, dump_device=dump_device)
+           for st in streams}
     for i in range(max(st.K for st in streams)):
         for st in streams:
             if i < st.K:
@@ -170,7 +172,8 @@ def serve_mix(streams, **kw):
     for st in streams:
         sid, ring = ids[st.k]
         assert srv.dumps_done(sid) == 1 and srv.blocks_done(sid) == st.K, st.k
-        out[st.k] = ring.array[: 2 * oracle.tri_len(st.ns, st.nf)].copy()
+        n = 2 * oracle.tri_len(st.ns, st.nf)
+        out[st.k] = ring.download(np.int64, n) if dump_device else ring.array[:n].copy()
     stats = srv.stats()
     srv.close()
     return out, stats
@@ -208,8 +211,8 @@ def test_serving_configured_mix_exact_and_equal_alone():
             check_stream_dump(st, mixed[st.k])
         spill = [st for st in streams if st.nt * st.K > W.I32_SAMPLE_BOUND][0]
         for st in {streams[0], streams[1], spill}:
-            flush(f"[serve] stream {st.k} alone")
-            alone, _ = serve_mix([st], staging_bytes=4 << 30)
+            flush(f"[serve] stream {st.k} alone, dump ring in device memory")
+            alone, _ = serve_mix([st], dump_device=True, staging_bytes=4 << 30)
             assert np.array_equal(alone[st.k], mixed[st.k]), st.k
     finally:
         for st in streams:
@@ -253,3 +256,27 @@ def test_serving_device_blocks_fuse_only_when_adjacent():
     blocker.free()
     adj.free()
     gap.free()
+
+
+def test_serving_device_dump_ring_slots():
+    """Dumps into a device ring of 2 slots: dump k lands in slot k % 2, and
+    dumps_done counts them; each dump is the integration of its own K blocks."""
+    ns, nf, nt, K = 64, 4, 512, 2
+    nbytes = nt * nf * ns * 4
+    x = oracle.fill_uniform(3 * K * nbytes, 78, -127, 127)
+    refs = [sum(oracle.xengine_i64(x[(d * K + i) * nbytes:(d * K + i + 1) * nbytes], ns, nf, nt)
+                for i in range(K)) for d in range(3)]
+    flush("[serve] device dump ring, 2 slots, 3 dumps")
+    blocks = X.DeviceBuffer.from_array(x)
+    srv = X.Server()
+    sid, ring = srv.open_stream(ns, nf, K, dump_slots=2, dump_device=True)
+    assert isinstance(ring, X.DeviceBuffer)
+    n = refs[0].size
+    for d in range(3):
+        for i in range(K):
+            srv.submit_device(sid, blocks.ptr + (d * K + i) * nbytes, nt)
+        srv.drain()
+        assert srv.dumps_done(sid) == d + 1
+        assert np.array_equal(ring.download(np.int64, n, offset=(d % 2) * n * 8), refs[d]), d
+    srv.close()
+    blocks.free()

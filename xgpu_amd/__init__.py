@@ -74,6 +74,7 @@ class DeviceBuffer:
     def download(self, dtype, count: int | None = None, offset: int = 0) -> np.ndarray:
         dt = np.dtype(dtype)
         n = (self.nbytes - offset) // dt.itemsize if count is None else count
+        assert 0 <= offset and offset + n * dt.itemsize <= self.nbytes
         out = np.empty(n, dtype=dt)
         check(core().xgpuAmdMemcpy(out.ctypes.data, self.ptr + offset, out.nbytes, _lib.XGPU_D2H))
         return out
@@ -334,12 +335,19 @@ class Server:
         self.handle = h.value
         self.rings = []
 
-    def open_stream(self, nstation, nfrequency, blocks_per_dump, compute=XGPU_INT8, dump_slots=4):
-        """Returns (stream id, dump ring: HostBuffer of dump_slots dumps)."""
+    def open_stream(self, nstation, nfrequency, blocks_per_dump, compute=XGPU_INT8, dump_slots=4,
+                    dump_device=False):
+        """Returns (stream id, dump ring of dump_slots dumps): a HostBuffer, or
+        with dump_device a DeviceBuffer (the dumps stay in HBM)."""
         ns2 = nstation * (nstation + 1) // 2
         dump_bytes = nfrequency * ns2 * 4 * 2 * 8
-        ring = HostBuffer(dump_bytes * dump_slots, np.int64 if compute == XGPU_INT8 else np.float64)
-        cfg = _lib.XGPUStreamConfig(nstation, nfrequency, compute, blocks_per_dump)
+        if dump_device:
+            ring = DeviceBuffer(dump_bytes * dump_slots)
+        else:
+            ring = HostBuffer(dump_bytes * dump_slots,
+                              np.int64 if compute == XGPU_INT8 else np.float64)
+        cfg = _lib.XGPUStreamConfig(nstation, nfrequency, compute, blocks_per_dump,
+                                    1 if dump_device else 0)
         sid = C.c_int()
         check(core().xgpuServerOpenStream(self.handle, C.byref(cfg), ring.ptr, dump_slots,
                                           C.byref(sid)))

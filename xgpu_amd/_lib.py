@@ -88,7 +88,7 @@ class XGPUServerConfig(C.Structure):
 
 class XGPUStreamConfig(C.Structure):
     _fields_ = [("nstation", C.c_uint), ("nfrequency", C.c_uint), ("compute_type", C.c_uint),
-                ("blocks_per_dump", C.c_uint)]
+                ("blocks_per_dump", C.c_uint), ("dump_device", C.c_uint)]
 
 
 class XGPUServerStats(C.Structure):

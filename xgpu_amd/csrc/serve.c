@@ -370,7 +370,12 @@ static void *scheduler_main(void *arg) {
         rd->sid = sid;
         rd->t_arrival = st->last_arrival;
         char *slot = st->ring + (st->dumps_issued % st->slots) * st->dump_bytes;
-        if (xg_dump_async(st->eng, slot, XGPU_DUMP_WIDE | XGPU_DUMP_TRIANGULAR)) { free(rd); ok = 0; break; }
+        if (xg_dump_async(st->eng, slot, XGPU_DUMP_WIDE | XGPU_DUMP_TRIANGULAR |
+                                             (st->cfg.dump_device ? XGPU_DUMP_DEVICE : 0))) {
+          free(rd);
+          ok = 0;
+          break;
+        }
         rd->event = get_event(s);
         if (!rd->event || xgh_event_record(rd->event, xgpuAmdStream(st->eng))) { free(rd); ok = 0; break; }
         xgpuAmdClear(st->eng);
@@ -583,6 +588,7 @@ int xgpuServerOpenStream(XGPUServer *s, const XGPUStreamConfig *cfg, void *dump_
   if (!s || !cfg || !stream_id) return xg_fail(XGPU_INVALID_CONFIG, "NULL argument");
   if (!dump_ring || dump_slots == 0) return xg_fail(XGPU_INVALID_CONFIG, "no dump ring");
   if (cfg->blocks_per_dump == 0) return xg_fail(XGPU_INVALID_CONFIG, "blocks_per_dump must be >= 1");
+  if (cfg->dump_device > 1) return xg_fail(XGPU_INVALID_CONFIG, "dump_device must be 0 or 1");
   if (cfg->compute_type != XGPU_INT8 && cfg->compute_type != XGPU_FLOAT16)
     return xg_fail(XGPU_INVALID_CONFIG, "server streams are XGPU_INT8 or XGPU_FLOAT16");
   XGPUAmdConfig ec;
