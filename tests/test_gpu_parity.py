@@ -248,11 +248,12 @@ def test_i8_spilled_wide_triangular_dump(order):
     eng.close()
 
 
-def test_i8_full_size_cfg2_sampled():
+def test_i8_full_size_cfg2():
     """NS=512 x 2 pol, NF=64, NT=8192: exact on sampled baselines of every
     channel and on every visibility of 4 whole channels (int64 oracle), every
     autocorrelation against the input power, and every int8 kernel variant
-    bit-identical to the default on every visibility."""
+    (16x16x64 MFMA tiles included) bit-identical to the default on every
+    visibility."""
     ns, nf, nt = 512, 64, 8192
     flush("[i8] cfg2 full size")
     buf = X.DeviceBuffer(nt * nf * ns * 4)
@@ -286,11 +287,11 @@ def test_i8_full_size_cfg2_sampled():
     assert np.array_equal(autos.transpose(1, 0, 2), power)
     flush("[i8] cfg2 full size, every int8 kernel variant against the default")
     first = eng.dump()
-    for kernel, staging, stage in I8_VARIANTS:
-        eng.set_tuning(kernel=kernel, staging=staging, stage=stage)
+    for kernel, staging, stage, mfma in [v + (0,) for v in I8_VARIANTS] + [(2, 2, 64, 2)]:
+        eng.set_tuning(kernel=kernel, staging=staging, stage=stage, mfma=mfma)
         eng.clear()
         eng.correlate(buf, nt)
-        assert np.array_equal(eng.dump(), first), (kernel, staging, stage)
+        assert np.array_equal(eng.dump(), first), (kernel, staging, stage, mfma)
     eng.close()
     buf.free()
 
