@@ -35,7 +35,7 @@ PEAK_FP16_TFLOPS = 2500.0   # MI355X dense fp16 MFMA
 PEAK_FP32_TFLOPS = 157.3    # MI355X fp32 vector
 NS, NF, NT = 512, 64, 8192
 CPU_SAMPLE = (512, 16, 8192)  # 16 channels of the workload, one reference call per channel
-CPU_THREADS = 16               # the box's CPU share for one GPU
+CPU_THREADS = 16               # host cores allotted per GPU on the benchmark host
 
 
 def log(msg):
