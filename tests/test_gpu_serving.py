@@ -1,8 +1,9 @@
 """GPU tests of the continuous-batching server (include/xgpu_serve.h).
 
-Each stream's dumps must be bit-exact (int64) against the oracle summed over
-the blocks of its integration, and identical whether the stream is served
-alone or mixed with other streams of different shapes.
+Each int8 stream's dumps must be bit-exact (int64) against the oracle summed
+over the blocks of its integration, and identical whether the stream is
+served alone or mixed with other streams of different shapes; FP16 streams'
+dumps (double) must match the FP64 oracle within the fp32 accumulation bound.
 """
 import numpy as np
 import pytest
@@ -280,3 +281,54 @@ def test_serving_device_dump_ring_slots():
         assert np.array_equal(ring.download(np.int64, n, offset=(d % 2) * n * 8), refs[d]), d
     srv.close()
     blocks.free()
+
+
+def test_serving_fp16_streams_beside_int8():
+    """FP16 streams (host blocks; device blocks back to back, so they fuse)
+    served beside an int8 stream: the FP16 dumps (double, triangular) match
+    the FP64 oracle of the same blocks within the fp32 accumulation bound, the
+    int8 dump is exact.  FP16 and int8 jobs never share a launch."""
+    from test_gpu_parity import assert_f16_close, f16_input
+    specs = [("f16", 64, 4, 256, 5, False), ("f16", 96, 3, 512, 4, True), ("i8", 64, 2, 256, 3, False)]
+    flush(f"[serve] fp16 beside int8 {specs}")
+    srv = X.Server()
+    keep, ids = [], []
+    for s, (kind, ns, nf, nt, K, dev) in enumerate(specs):
+        comp = X.XGPU_FLOAT16 if kind == "f16" else X.XGPU_INT8
+        sid, ring = srv.open_stream(ns, nf, K, compute=comp, dump_slots=1)
+        if kind == "f16":
+            blocks = [f16_input(nt, nf, ns, 900 + 10 * s + b) for b in range(K)]
+        else:
+            nb = nt * nf * ns * 4
+            blocks = [oracle.fill_uniform(nb, 950 + b, -127, 127).reshape(nt, nf, ns, 2, 2)
+                      for b in range(K)]
+        if dev:
+            d = X.DeviceBuffer.from_array(np.concatenate(blocks, axis=0))
+            keep.append(d)
+            for b in range(K):
+                srv.submit_device(sid, d.ptr + b * blocks[0].nbytes, nt)
+        else:
+            hbs = []
+            for blk in blocks:
+                hb = X.HostBuffer(blk.nbytes)
+                hb.array[:] = blk.reshape(-1).view(np.int8)
+                hbs.append(hb)
+            keep += hbs
+            for hb in hbs:
+                srv.submit(sid, hb, nt)
+        ids.append((sid, ring, blocks))
+    srv.drain()
+    for (kind, ns, nf, nt, K, dev), (sid, ring, blocks) in zip(specs, ids):
+        assert srv.dumps_done(sid) == 1
+        x = np.concatenate(blocks, axis=0)
+        n = 2 * oracle.tri_len(ns, nf)
+        got = ring.array[:n]
+        if kind == "f16":
+            ref = oracle.xengine_f64_f16(x, ns, nf, K * nt)
+            # one more fp32 rounding per job that accumulates: K at most
+            assert_f16_close(got, ref, ns, nf, K * nt + 8 * K)
+        else:
+            assert np.array_equal(got, oracle.xengine_i64(x.reshape(-1), ns, nf, K * nt))
+    srv.close()
+    for b in keep:
+        b.free()
