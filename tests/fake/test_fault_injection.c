@@ -17,6 +17,7 @@
  * Built with the xgpu.h sizing NSTATION=16, NFREQUENCY=2, NTIME=64,
  * NTIME_PIPE=16 (float Complex).
  */
+#include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -340,6 +341,135 @@ static void server_runs_and_reports_faults(void) {
          "reported by drain\n");
 }
 
+/* ---- randomized serving ----------------------------------------------------
+ * Streams of random shapes, host or device blocks (device blocks back to back
+ * in memory, so they fuse), host or device dump rings of 2 slots, random
+ * server limits, submitted by two threads in a random interleaving (each
+ * thread owns half the streams, so each stream's blocks stay in order).  Every
+ * stream's last two dumps are checked against the exact sums of their
+ * integrations' blocks. */
+#define RS_STREAMS 6
+#define RS_DUMPS 3
+typedef struct {
+  int ns, nf, nt, K, dev, dump_dev, id;
+  size_t nb, nout;
+  int8_t *x;          /* RS_DUMPS * K blocks, back to back */
+  void *x_d, *ring_d;
+  int64_t *ring;
+} RsStream;
+
+typedef struct {
+  XGPUServer *s;
+  RsStream *st;
+  int first, count;
+  uint64_t seed;
+  int rc;
+} RsSubmitter;
+
+static uint64_t rs_next(uint64_t *s) {
+  *s ^= *s << 13, *s ^= *s >> 7, *s ^= *s << 17;
+  return *s;
+}
+
+static void *rs_submit(void *arg) {
+  RsSubmitter *w = (RsSubmitter *)arg;
+  int next[RS_STREAMS] = {0};
+  int left = 0;
+  for (int i = 0; i < w->count; ++i) left += RS_DUMPS * w->st[w->first + i].K;
+  uint64_t r = w->seed;
+  while (left > 0 && w->rc == XGPU_OK) {
+    const int i = w->first + (int)(rs_next(&r) % (uint64_t)w->count);
+    RsStream *q = &w->st[i];
+    if (next[i] == RS_DUMPS * q->K) continue;
+    const int b = next[i]++;
+    w->rc = q->dev ? xgpuServerSubmitDevice(w->s, q->id, (char *)q->x_d + (size_t)b * q->nb, q->nt)
+                   : xgpuServerSubmit(w->s, q->id, q->x + (size_t)b * q->nb, q->nt);
+    --left;
+  }
+  return NULL;
+}
+
+static void server_randomized(uint64_t seed) {
+  uint64_t r = 0x9E3779B97F4A7C15ull ^ (seed * 0xD1B54A32D192ED03ull);
+  static const int NS[] = {4, 8, 12, 16}, NT[] = {16, 32, 48};
+  RsStream st[RS_STREAMS];
+  XGPUServerConfig sc = {0, (rs_next(&r) & 1) ? ((size_t)64 << 10) : ((size_t)1 << 20),
+                         1 + (int)(rs_next(&r) % 16), 1 + (int)(rs_next(&r) % 3)};
+  XGPUServer *s = NULL;
+  CHECK(xgpuServerCreate(&s, &sc) == XGPU_OK, "server");
+  for (int i = 0; i < RS_STREAMS; ++i) {
+    RsStream *q = &st[i];
+    q->ns = NS[rs_next(&r) % 4];
+    q->nf = 1 + (int)(rs_next(&r) % 3);
+    q->nt = NT[rs_next(&r) % 3];
+    q->K = 1 + (int)(rs_next(&r) % 6);
+    q->dev = (int)(rs_next(&r) & 1);
+    q->dump_dev = (int)(rs_next(&r) & 1);
+    q->nb = (size_t)q->nt * q->nf * q->ns * 4;
+    q->nout = (size_t)q->nf * q->ns * (q->ns + 1) / 2 * 8;
+    q->x = malloc(RS_DUMPS * q->K * q->nb);
+    fill(q->x, RS_DUMPS * q->K * q->nb, (unsigned)(1000 * seed + i));
+    q->x_d = q->ring_d = NULL;
+    q->ring = calloc(2 * q->nout, sizeof(int64_t));
+    if (q->dev) {
+      CHECK(xgpuAmdMalloc(&q->x_d, RS_DUMPS * q->K * q->nb) == XGPU_OK, "blocks");
+      CHECK(xgpuAmdMemcpy(q->x_d, q->x, RS_DUMPS * q->K * q->nb, XGPU_H2D) == XGPU_OK, "upload");
+    }
+    if (q->dump_dev) CHECK(xgpuAmdMalloc(&q->ring_d, 2 * q->nout * sizeof(int64_t)) == XGPU_OK, "ring");
+    XGPUStreamConfig cfg = {(unsigned)q->ns, (unsigned)q->nf, XGPU_INT8, (unsigned)q->K,
+                            (unsigned)q->dump_dev};
+    CHECK(xgpuServerOpenStream(s, &cfg, q->dump_dev ? q->ring_d : (void *)q->ring, 2, &q->id) == XGPU_OK,
+          "open: %s", xgpuAmdLastError());
+  }
+  RsSubmitter w[2] = {{s, st, 0, RS_STREAMS / 2, seed * 2 + 1, XGPU_OK},
+                      {s, st, RS_STREAMS / 2, RS_STREAMS - RS_STREAMS / 2, seed * 2 + 2, XGPU_OK}};
+  pthread_t th[2];
+  for (int t = 0; t < 2; ++t) CHECK(pthread_create(&th[t], NULL, rs_submit, &w[t]) == 0, "thread");
+  for (int t = 0; t < 2; ++t) pthread_join(th[t], NULL);
+  CHECK(w[0].rc == XGPU_OK && w[1].rc == XGPU_OK, "submit: %s", xgpuAmdLastError());
+  CHECK(xgpuServerDrain(s) == XGPU_OK, "drain: %s", xgpuAmdLastError());
+  XGPUServerStats stats;
+  CHECK(xgpuServerStats(s, &stats) == XGPU_OK, "stats");
+  unsigned long long total = 0;
+  for (int i = 0; i < RS_STREAMS; ++i) total += RS_DUMPS * st[i].K;
+  CHECK(stats.blocks_done == total, "blocks done %llu of %llu", stats.blocks_done, total);
+  for (int i = 0; i < RS_STREAMS; ++i) {
+    RsStream *q = &st[i];
+    CHECK(xgpuServerDumpsDone(s, q->id) == RS_DUMPS, "stream %d dumps", i);
+    if (q->dump_dev)
+      CHECK(xgpuAmdMemcpy(q->ring, q->ring_d, 2 * q->nout * sizeof(int64_t), XGPU_D2H) == XGPU_OK, "ring");
+    int64_t *ref = calloc(q->nout, sizeof(int64_t)), *one = malloc(q->nout * sizeof(int64_t));
+    for (int d = RS_DUMPS - 2; d < RS_DUMPS; ++d) {  /* dump d sits in slot d % 2 */
+      memset(ref, 0, q->nout * sizeof(int64_t));
+      for (int b = 0; b < q->K; ++b) {
+        exact(one, q->x + (size_t)(d * q->K + b) * q->nb, q->ns, q->nf, q->nt);
+        for (size_t k = 0; k < q->nout; ++k) ref[k] += one[k];
+      }
+      const int64_t *got = q->ring + (size_t)(d % 2) * q->nout;
+      for (size_t k = 0; k < q->nout; ++k)
+        CHECK(got[k] == ref[k], "seed %llu stream %d (ns %d nf %d nt %d K %d dev %d ring %d) dump %d "
+              "value %zu", (unsigned long long)seed, i, q->ns, q->nf, q->nt, q->K, q->dev, q->dump_dev,
+              d, k);
+    }
+    free(ref);
+    free(one);
+  }
+  xgpuServerDestroy(s);
+  for (int i = 0; i < RS_STREAMS; ++i) {
+    free(st[i].x);
+    free(st[i].ring);
+    if (st[i].x_d) xgpuAmdFree(st[i].x_d);
+    if (st[i].ring_d) xgpuAmdFree(st[i].ring_d);
+  }
+  CHECK(xgh_fake_live_allocations() == 0, "leaked %lld", xgh_fake_live_allocations());
+}
+
+static void server_randomized_runs(int nseeds) {
+  for (int seed = 1; seed <= nseeds; ++seed) server_randomized((uint64_t)seed);
+  printf("ok randomized serving: %d seeds of streams, limits and submission orders, every dump exact\n",
+         nseeds);
+}
+
 /* ---- XGPU_TRACE (run as "test_fault_injection trace <file>": the switch is
  * read once per process) ---------------------------------------------------- */
 static int count_lines(const char *path, const char *needle) {
@@ -395,6 +525,10 @@ int main(int argc, char **argv) {
     trace_log(argv[2]);
     return 0;
   }
+  if (argc == 3 && strcmp(argv[1], "serve") == 0) {  /* the randomized serving runs only */
+    server_randomized_runs(atoi(argv[2]));
+    return 0;
+  }
   xgpu_h_pipeline();
   xgpu_init_faults();
   xgpu_xengine_faults();
@@ -402,6 +536,7 @@ int main(int argc, char **argv) {
   engine_spill_and_dump();
   engine_graphs_batch_noalloc();
   server_runs_and_reports_faults();
+  server_randomized_runs(8);
   printf("all fault-injection scenarios passed\n");
   return 0;
 }

@@ -8,7 +8,10 @@ n-th call of any category (malloc, copy, launch, event, stream, sync, graph)
 can be made to fail.  The scenarios check that every failure comes back as the
 reference's XGPU_CUDA_ERROR (reference src/cuda_xengine.cu:94-101, there with
 exit()) or an allocation error, that nothing leaks (LeakSanitizer), and that a
-retry afterwards is exact.  CPU only.
+retry afterwards is exact; a randomized serving scenario (random stream
+shapes, host or device blocks and dump rings, server limits, two submitting
+threads) checks every dump.  The same scenarios also run under
+ThreadSanitizer.  CPU only.
 """
 import os
 import shutil
@@ -34,9 +37,25 @@ def test_host_runtime_under_fault_injection():
     assert "all fault-injection scenarios passed" in r.stdout
     for scenario in ("xgpu.h pipeline", "xgpuInit fails cleanly", "xgpuCudaXengine returns",
                      "xgpuAmdCreate fails cleanly", "spill to int64", "graph steps, batch",
-                     "server: exact dumps"):
+                     "server: exact dumps", "randomized serving"):
         assert "ok " + scenario in r.stdout, r.stdout
     assert "runtime error" not in out  # UBSan
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None or shutil.which("make") is None,
+                    reason="needs gcc and make")
+def test_host_runtime_under_thread_sanitizer():
+    """The same scenarios, the server's threads included, under TSan."""
+    b = subprocess.run(["make", "-s", "-C", FAKE, "test_fault_injection_tsan"], capture_output=True,
+                       text=True, timeout=600)
+    assert b.returncode == 0, b.stdout + b.stderr
+    r = subprocess.run([os.path.join(FAKE, "test_fault_injection_tsan")], capture_output=True,
+                       text=True, timeout=900,
+                       env=dict(os.environ, TSAN_OPTIONS="halt_on_error=1"))
+    out = r.stdout + r.stderr
+    assert r.returncode == 0, out[-4000:]
+    assert "all fault-injection scenarios passed" in r.stdout
+    assert "ThreadSanitizer" not in out
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None or shutil.which("make") is None,
