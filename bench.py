@@ -289,7 +289,8 @@ def kernel_name(tun):
 def profile_only(X, which, steps, warmup, kernel=0, stage=0, staging=0):
     """Run one path's workload warmup+steps times (one kernel per step).
     "copy": a plain 1 GiB device-to-device copy per step (counter calibration);
-    "dump": one NS=2048, NF=16 integration, then one device triangular dump per step."""
+    "dump": one NS=2048, NF=16 integration, then one device triangular dump per step;
+    "cfg3": int8 integrations of NS=2048, NF=16, NT=4096."""
     if which == "fft":
         import numpy as np
         buf = X.DeviceBuffer(FFT_ENTRIES * 8)
@@ -325,6 +326,9 @@ def profile_only(X, which, steps, warmup, kernel=0, stage=0, staging=0):
         return
     if which == "fp16":
         ns, nf, nt, comp, nbytes = 256, 128, 2048, X.XGPU_FLOAT16, 256 * 128 * 2048 * 8
+    elif which == "cfg3":
+        ns, nf, nt, comp = 2048, 16, 4096, X.XGPU_INT8
+        nbytes = ns * nf * nt * 4
     else:
         ns, nf, nt = NS, NF, NT
         comp = X.XGPU_INT8 if which == "int8" else X.XGPU_FLOAT32
@@ -405,7 +409,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--only", choices=["int8", "fp32", "fp16", "copy", "fft", "dump"], default=None,
+    ap.add_argument("--only", choices=["int8", "fp32", "fp16", "copy", "fft", "dump", "cfg3"],
+                    default=None,
                     help="run just this path's workload (for profiling); prints no JSON line")
     ap.add_argument("--kernel", type=int, default=0, choices=[0, 1, 2, 3],
                     help="int8 kernel: 1 complex-plane, 2 real-Gram, 3 K-interleaved "
