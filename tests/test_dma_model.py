@@ -31,6 +31,21 @@ OOR = 0x7fffff00    # offset given to lanes whose chunk is past the station coun
 CELL_GROUP, CELL_SCRATCH = 144, 16 * 144
 
 
+def i32(v):
+    """v as the kernel's 32-bit int arithmetic leaves it (two's complement wrap)."""
+    v &= 0xFFFFFFFF
+    return v - (1 << 32) if v >= (1 << 31) else v
+
+
+def kernel_nrec(nt, s0, spd, ts, row_bytes):
+    """num_records of a DMA / buffer load as the kernels compute it, in 32-bit
+    ints: rows clamped at 0 before the product (past the block nt - s0 is far
+    below 0, and (nt - s0) * ts wrapped to a large positive range at
+    tstride > ~7 MB before the clamp)."""
+    rows = max(min(nt - s0, spd), 0)
+    return i32(i32(i32(rows * ts) - ts) + row_bytes) if rows > 0 else 0
+
+
 def dma_chunk(slot, srow):
     x = slot ^ (srow & 7)
     return x ^ ((x & 1) << 3)
@@ -63,13 +78,14 @@ def model_dma(ns, nf, nt, f, rb, cb, stg, nbuf, stages=None):
     in_f = f * ns * 4
     lanes = np.arange(64)
     if stages is None:
-        stages = range(nstage + nbuf)
+        stages = range(nstage + nbuf + 1)  # with an odd stage count, one more
     # LDS bytes written per stage by all waves: each stage buffer exactly once
     cover = {st: np.zeros(buf_bytes, dtype=np.int32) for st in stages}
     for wave in range(4):
         srow, slot = lanes // NCHUNK, lanes % NCHUNK
         chunk = dma_chunk(slot, wave * spd + srow)
         dvoff = srow * ts + 16 * chunk
+        assert dvoff.max() < 2 ** 31  # an int in the kernel
         voff = {0: np.where(rb * TILE + 8 * chunk < ninput, dvoff, OOR),
                 1: np.where((not diag) & (cb * TILE + 8 * chunk < ninput), dvoff, OOR)}
         for st in stages:
@@ -77,8 +93,9 @@ def model_dma(ns, nf, nt, f, rb, cb, stg, nbuf, stages=None):
                 for j in range(kdma):
                     g = 4 * j + wave
                     s0 = st * stg + g * spd
-                    rows = min(nt - s0, spd)
-                    nrec = max(rows * ts - ts + row_bytes, 0)
+                    rows = max(min(nt - s0, spd), 0)
+                    nrec = kernel_nrec(nt, s0, spd, ts, row_bytes)
+                    assert nrec == (rows * ts - ts + row_bytes if rows else 0) and nrec < OOR
                     base = in_f + (cb if side else rb) * ROW + s0 * ts
                     assert 0 <= base < (1 << 48)
                     m0 = (st & (nbuf - 1)) * buf_bytes + side * side_bytes + g * 1024
@@ -236,6 +253,47 @@ def test_dma_addresses_in_bounds_large(ns, nf, nt):
                 model_dma(ns, nf, nt, f, rb, cb, stg, nbuf, stages)
 
 
+def test_dma_addresses_at_the_offset_limit():
+    """The largest block the engine gives the real-Gram kernels: 128 * tstride
+    just below 2^31 - 4096 (NSTATION=512, NFREQUENCY=8191; tstride 16.8 MB).
+    Every tile of the first and last channel, every stage."""
+    ns, nf, nt = 512, 8191, 64
+    assert 128 * nf * ns * 4 < 2 ** 31 - 4096 <= 128 * (nf + 1) * ns * 4
+    nblk = (2 * ns + TILE - 1) // TILE
+    for f in (0, nf - 1):
+        for rb in range(nblk):
+            for cb in range(rb + 1):
+                model_dma(ns, nf, nt, f, rb, cb, 32, 8)
+
+
+@pytest.mark.parametrize("ns,nf", [(512, 64), (512, 4096), (512, 8191), (2048, 2047), (96, 2)])
+def test_buffer_ranges_never_wrap(ns, nf):
+    """Every buffer load's num_records in every int8 kernel (DMA stages of
+    rgd<32,8>, rgd<64,4> and rgd16, register-staged loads of rg<64> and
+    rg<128>, the K-interleaved kernel's row and column DMAs) over every stage
+    it issues, past the block included, for block lengths 1..300: the 32-bit
+    computation equals the exact range and is 0 exactly when no sample of the
+    load is valid.  At NF=8191 (tstride 16.8 MB) the unclamped form wrapped to
+    ranges of ~2e9 bytes past the block."""
+    ts = nf * ns * 4
+    if 128 * ts >= 2 ** 31 - 4096:
+        return  # the engine does not give such blocks to these kernels
+    row_bytes = min(ts, ROW)
+    forms = []  # (samples per stage, stages in flight past the block, samples per load)
+    forms += [(32, 8, 4), (64, 4, 4)]         # rgd<32,8>, rgd<64,4> and rgd16: 4 rows per DMA
+    forms += [(64, 2, 16), (128, 2, 16)]      # rg<64>, rg<128>: 16 rows per load pass
+    forms += [(32, 4, 4), (32, 4, 2)]         # kg: row side 4, column side 2 rows per DMA
+    for nt in list(range(1, 70)) + [255, 256, 257, 300]:
+        for stg, ahead, spd in forms:
+            nstage = (nt + stg - 1) // stg
+            for st in range(nstage + ahead + 2):
+                for s0 in range(st * stg, (st + 1) * stg, spd):
+                    rows = max(min(nt - s0, spd), 0)
+                    exact = rows * ts - ts + row_bytes if rows else 0
+                    assert kernel_nrec(nt, s0, spd, ts, row_bytes) == exact
+                    assert (exact == 0) == (rows == 0)
+
+
 @pytest.mark.parametrize("stg,nbuf", [(32, 8), (64, 4)])
 def test_operand_reads_inside_stage_buffer(stg, nbuf):
     for diag in (False, True):
@@ -264,3 +322,21 @@ def test_epilogue_stores_in_bounds_and_exactly_once(ns, nf, order, acc, epilogue
         assert (hits == 1).sum() == 2 * nf * nb * 4
     else:
         assert (hits == 1).all(), (int((hits == 0).sum()), int((hits > 1).sum()))
+
+
+def test_kernel_source_clamps_rows_before_the_range_product():
+    """Every `rows * ts` range product in the kernel source multiplies a rows
+    value clamped at 0 (the form kernel_nrec models), so no buffer range can
+    wrap past the block."""
+    import os
+    import re
+    src = open(os.path.join(os.path.dirname(__file__), "..", "xgpu_amd", "csrc",
+                            "xg_kernels.hip")).read()
+    uses = [m.start() for m in re.finditer(r"rows \* ts", src)]
+    assert len(uses) >= 5
+    for u in uses:
+        before = src[:u]
+        d = before.rfind("const int rows = ")
+        assert d >= 0
+        line = before[d:before.find("\n", d)]
+        assert re.match(r"const int rows = max\(min\([^;]*\), 0\);", line), line

@@ -757,6 +757,46 @@ def test_i8_cfg3_shape_exact_by_properties():
     assert not tri[:, d][:, :, [0, 3], 1].any()  # autocorrelations are real
 
 
+@pytest.mark.parametrize("nf", [8191, 8192])
+def test_i8_largest_blocks_at_the_offset_limit(nf):
+    """NSTATION=512 x 2 pol, NTIME=64: NFREQUENCY=8191 is the largest block the
+    real-Gram kernels take (128 * tstride just below 2^31 - 4096: their in-stage
+    byte offsets are 32-bit), 8192 the smallest that falls back to the
+    complex-plane kernel (64-bit offsets).  About 34 GB of int32 output each.
+    Exact on every visibility of the first, middle and last channel, and on
+    sampled cells of every 64th channel, through a device triangular dump of
+    which only those parts are copied back."""
+    ns, nt = 512, 64
+    assert (128 * nf * ns * 4 < (1 << 31) - 4096) == (nf == 8191)
+    flush(f"[i8] offset limit NS={ns} NF={nf} NT={nt}")
+    nbytes = nt * nf * ns * 4
+    buf = X.DeviceBuffer(nbytes)
+    buf.fill_uniform_int8(nf, -128, 127)
+    x = buf.download(np.int8)
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    eng.correlate(buf, nt)
+    buf.free()
+    tri_d = X.DeviceBuffer(2 * eng.tri_len * 4)
+    X.check(X.core().xgpuAmdDump(eng.handle, tri_d.ptr, X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR))
+    eng.close()
+    nb = ns * (ns + 1) // 2
+    chan_vals = nb * 8  # int32 values per channel
+    chans = [0, nf // 2, nf - 1]
+    whole = oracle.xengine_i64_channels(x, ns, nf, nt, chans)
+    for c, f in enumerate(chans):
+        got = tri_d.download(np.int32, chan_vals, offset=f * chan_vals * 4).reshape(nb, 4, 2)
+        assert np.array_equal(got.astype(np.int64), whole[c]), f
+    rng = np.random.default_rng(nf)
+    fij = [(f, int(i), int(rng.integers(i + 1))) for f in range(0, nf, 64)
+           for i in rng.integers(0, ns, 2)]
+    ref = oracle.xengine_i64_pairs(x, ns, nf, nt, fij)
+    for n, (f, i, j) in enumerate(fij):
+        k = f * nb + i * (i + 1) // 2 + j
+        got = tri_d.download(np.int32, 8, offset=k * 32).reshape(4, 2)
+        assert np.array_equal(got.astype(np.int64), ref[n]), (f, i, j)
+    tri_d.free()
+
+
 def test_f16_cfg4_shape_errors_vs_f64():
     """NSTATION=256 x 2 pol, NFREQUENCY=128, NTIME=2048, fp16 complex
     N(0, 16^2) plus a delayed tone 4 e^{i(2 pi f tau_s / NF + w t)} (the cfg4

@@ -916,9 +916,11 @@ struct RgLoader {
 #pragma unroll
     for (int i = 0; i < C::kLoads; ++i) {
       const int t0 = st * STG + C::kPass * i;       // first sample of this load
-      const int rows = min(ntime - t0, C::kPass);   // valid samples of the kPass
-      // (rows - 1) full sample rows + the last one's reachable bytes; <= 0: none
-      const int nrec = max(rows * ts - ts + row_bytes, 0);
+      // valid samples of the kPass, clamped at 0 before the product: past the
+      // block ntime - t0 is far below 0 and (ntime - t0) * ts would wrap
+      const int rows = max(min(ntime - t0, C::kPass), 0);
+      // (rows - 1) full sample rows + the last one's reachable bytes; none: 0
+      const int nrec = rows > 0 ? rows * ts - ts + row_bytes : 0;
       const long long boff = sbase + i * load_step;
       const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
           (void *)(row_base + boff), (short)0, nrec, 0x00020000);
@@ -1188,8 +1190,11 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
   auto dma = [&](int st, int side, int j) {
     const int g = 4 * j + wave;
     const int s0 = st * STG + g * C::kSpd;  // first sample of this DMA
-    const int rows = min(ntime - s0, C::kSpd);
-    const int nrec = max(rows * ts - ts + row_bytes, 0);
+    // valid sample rows, clamped at 0 before the product: past the block
+    // ntime - s0 is far below 0 and (ntime - s0) * ts would wrap to a large
+    // positive range
+    const int rows = max(min(ntime - s0, C::kSpd), 0);
+    const int nrec = rows > 0 ? rows * ts - ts + row_bytes : 0;
     const unsigned long long a = (side ? col_base : row_base) + (unsigned long long)(long long)s0 * ts;
     v4i rs;
     rs.x = (int)(unsigned)a;
@@ -1438,8 +1443,8 @@ __device__ __forceinline__ void rgd16_tile(const XgJob &J, const TileRef &T, uns
   auto dma = [&](int st, int side, int j) {
     const int g = 4 * j + wave;
     const int s0 = st * STG + g * C::kSpd;
-    const int rows = min(ntime - s0, C::kSpd);
-    const int nrec = max(rows * ts - ts + row_bytes, 0);
+    const int rows = max(min(ntime - s0, C::kSpd), 0);  // clamped before the product
+    const int nrec = rows > 0 ? rows * ts - ts + row_bytes : 0;
     const unsigned long long a = (side ? col_base : row_base) + (unsigned long long)(long long)s0 * ts;
     v4i rs;
     rs.x = (int)(unsigned)a;
@@ -1783,16 +1788,16 @@ __device__ __forceinline__ void kg_tile(const XgJob &J, const TileRef &T, unsign
     if (d < 2) {
       const int g = 2 * wave + d;
       const int s0 = st * KG_STG + 4 * g;
-      const int rows = min(ntime - s0, 4);
-      rs.z = max(rows * ts - ts + arow_bytes, 0);
+      const int rows = max(min(ntime - s0, 4), 0);  // clamped before the product
+      rs.z = rows > 0 ? rows * ts - ts + arow_bytes : 0;
       a = abase + (unsigned long long)(long long)s0 * ts;
       voff = voa;
       m0 += 4 * g * KG_AROW;
     } else {
       const int j = 4 * wave + d - 2;
       const int s0 = st * KG_STG + 2 * j;
-      const int rows = min(ntime - s0, 2);
-      rs.z = max(rows * ts - ts + brow_bytes, 0);
+      const int rows = max(min(ntime - s0, 2), 0);  // clamped before the product
+      rs.z = rows > 0 ? rows * ts - ts + brow_bytes : 0;
       a = bbase + (unsigned long long)(long long)s0 * ts;
       voff = vob[(d - 2) & 1];
       m0 += KG_SIDEA + 2 * j * KG_BROW;
