@@ -62,20 +62,29 @@ def lane_off(blk, lane):
     return (kg * 16 + (r >> 1)) * ROW + (rg_slot(chunk, r >> 1) << 4) + ((r & 1) << 3)
 
 
-def model_dma(ns, nf, nt, f, rb, cb, stg, nbuf, stages=None):
+# element formats of the real-Gram kernels: LDS row bytes per sample and tile
+# side, 16-byte chunks per row, inputs per chunk, bytes per input, and the
+# chunk a DMA lane of sample row `srow` loads into its slot
+FMT = {"i8": dict(row=256, nchunk=16, kin=8, bpi=2, chunk=lambda slot, srow: dma_chunk(slot, srow)),
+       "f16": dict(row=512, nchunk=32, kin=4, bpi=4, chunk=lambda slot, srow: slot ^ ((srow & 3) << 2))}
+
+
+def model_dma(ns, nf, nt, f, rb, cb, stg, nbuf, stages=None, fmt="i8"):
     """Checks the DMAs of tile (f, rb, cb) over `stages` (default: every stage
     the kernel issues, including those past the block)."""
+    F = FMT[fmt]
+    ROW, NCHUNK, kin, bpi = F["row"], F["nchunk"], F["kin"], F["bpi"]
     spd = 1024 // ROW                  # sample rows per wave DMA
     kdma = stg // (4 * spd)            # DMAs per wave per side per stage
     side_bytes, buf_bytes = stg * ROW, 2 * stg * ROW
     lds_bytes = nbuf * buf_bytes
-    ts = nf * ns * 4                   # input bytes per sample
+    ts = nf * ns * 2 * bpi             # input bytes per sample
     total = nt * ts
     ninput = 2 * ns
     diag = rb == cb
     nstage = (nt + stg - 1) // stg
     row_bytes = min(ts, ROW)
-    in_f = f * ns * 4
+    in_f = f * ns * 2 * bpi
     lanes = np.arange(64)
     if stages is None:
         stages = range(nstage + nbuf + 1)  # with an odd stage count, one more
@@ -83,11 +92,11 @@ def model_dma(ns, nf, nt, f, rb, cb, stg, nbuf, stages=None):
     cover = {st: np.zeros(buf_bytes, dtype=np.int32) for st in stages}
     for wave in range(4):
         srow, slot = lanes // NCHUNK, lanes % NCHUNK
-        chunk = dma_chunk(slot, wave * spd + srow)
+        chunk = F["chunk"](slot, wave * spd + srow)
         dvoff = srow * ts + 16 * chunk
         assert dvoff.max() < 2 ** 31  # an int in the kernel
-        voff = {0: np.where(rb * TILE + 8 * chunk < ninput, dvoff, OOR),
-                1: np.where((not diag) & (cb * TILE + 8 * chunk < ninput), dvoff, OOR)}
+        voff = {0: np.where(rb * TILE + kin * chunk < ninput, dvoff, OOR),
+                1: np.where((not diag) & (cb * TILE + kin * chunk < ninput), dvoff, OOR)}
         for st in stages:
             for side in (0, 1):
                 for j in range(kdma):
@@ -115,10 +124,130 @@ def model_dma(ns, nf, nt, f, rb, cb, stg, nbuf, stages=None):
                     first = in_f + blk * ROW
                     assert (t < nt).all()
                     assert ((e >= first) & (e + 16 <= first + ROW)).all()
-                    m = (e - in_f) // 2  # first of the 8 inputs (2 bytes each) read
-                    assert (m + 7 < ninput).all()
+                    m = (e - in_f) // bpi  # first of the kin inputs read
+                    assert (m + kin - 1 < ninput).all()
     for st in stages:
         assert (cover[st] == 1).all(), st
+
+
+def check_lanes(lo, total, nt, ts, in_f, first_byte, span, ninput, bpi=2):
+    """Lanes reading 16 bytes at byte offsets `lo` of the input block: inside
+    the block, in valid samples, inside this tile side's `span` bytes of a
+    sample row, and on valid inputs only (bpi bytes per input)."""
+    assert lo.min() >= 0 and (lo + 16).max() <= total
+    t = lo // ts
+    e = lo - t * ts
+    assert (t < nt).all()
+    assert ((e >= first_byte) & (e + 16 <= first_byte + span)).all()
+    m = (e - in_f) // bpi  # first of the 16 / bpi inputs read
+    assert (m + 16 // bpi - 1 < ninput).all()
+
+
+def model_rg_loads(ns, nf, nt, f, rb, cb, stg, fmt="i8"):
+    """The register-staged kernels' loads (RgLoader: 256 / chunks-per-row
+    sample rows per pass, STG / pass loads per thread and side) of tile
+    (f, rb, cb) for every stage the tile and the next tile's prologue issue
+    (up to nstage + 5)."""
+    F = FMT[fmt]
+    ROW, NCHUNK, kin, bpi = F["row"], F["nchunk"], F["kin"], F["bpi"]
+    kpass = 256 // NCHUNK
+    kloads = stg // kpass
+    ts = nf * ns * 2 * bpi
+    total, ninput = nt * ts, 2 * ns
+    row_bytes = min(ts, ROW)
+    in_f = f * ns * 2 * bpi
+    nstage = (nt + stg - 1) // stg
+    tid = np.arange(256)
+    chunk, srow = tid % NCHUNK, tid // NCHUNK
+    voff = srow * ts + 16 * chunk
+    assert voff.max() < 2 ** 31
+    v = {0: np.where(rb * TILE + kin * chunk < ninput, voff, OOR),
+         1: np.where((rb != cb) & (cb * TILE + kin * chunk < ninput), voff, OOR)}
+    for st in range(nstage + 7):
+        for i in range(kloads):
+            t0 = st * stg + kpass * i
+            nrec = kernel_nrec(nt, t0, kpass, ts, row_bytes)
+            for side in (0, 1):
+                blk = cb if side else rb
+                base = in_f + blk * ROW + t0 * ts
+                live = v[side] < nrec
+                if live.any():
+                    check_lanes(base + v[side][live], total, nt, ts, in_f, in_f + blk * ROW, ROW,
+                                ninput, bpi)
+
+
+def kg_slot(chunk, s3):
+    return chunk ^ (s3 << 2)
+
+
+def model_kg_dma(ns, nf, nt, f, rb, cp):
+    """The K-interleaved kernel's DMAs of pair tile (f, rb, cp): row side 4
+    samples of 256 bytes per DMA, column side 2 samples of 512 bytes, 32-sample
+    stages, 4 buffers; every stage issued (up to nstage + 4)."""
+    ts = nf * ns * 4
+    total, ninput = nt * ts, 2 * ns
+    in_f = f * ns * 4
+    arow, brow = min(ts, 256), min(ts, 512)
+    nstage = (nt + 31) // 32
+    lanes = np.arange(64)
+    r, c = lanes >> 4, kg_slot(lanes & 15, lanes >> 4)
+    voa = r * ts + 16 * c
+    voa = np.where(rb * TILE + 8 * c < ninput, voa, OOR)
+    vob = []
+    for par in range(2):
+        rr = lanes >> 5
+        cc = kg_slot(lanes & 31, (2 * par + rr) & 3)
+        vob.append(np.where(cp * 2 * TILE + 8 * cc < ninput, rr * ts + 16 * cc, OOR))
+    for st in range(nstage + 5):
+        for wave in range(4):
+            for d in range(6):
+                if d < 2:
+                    g = 2 * wave + d
+                    s0 = st * 32 + 4 * g
+                    nrec = kernel_nrec(nt, s0, 4, ts, arow)
+                    base, vv, first, span = in_f + rb * 256 + s0 * ts, voa, in_f + rb * 256, 256
+                else:
+                    j = 4 * wave + d - 2
+                    s0 = st * 32 + 2 * j
+                    nrec = kernel_nrec(nt, s0, 2, ts, brow)
+                    base, vv, first, span = in_f + cp * 512 + s0 * ts, vob[(d - 2) & 1], in_f + cp * 512, 512
+                live = vv < nrec
+                if live.any():
+                    check_lanes(base + vv[live], total, nt, ts, in_f, first, span, ninput)
+
+
+@pytest.mark.parametrize("ns,nf,nt", [(96, 2, 257), (512, 8191, 64), (132, 2, 64), (256, 4, 1000)])
+@pytest.mark.parametrize("stg", [64, 128])
+def test_rg_loads_in_bounds(ns, nf, nt, stg):
+    nblk = (2 * ns + TILE - 1) // TILE
+    for f in sorted({0, nf - 1}):
+        for rb in range(nblk):
+            for cb in range(rb + 1):
+                model_rg_loads(ns, nf, nt, f, rb, cb, stg)
+
+
+@pytest.mark.parametrize("ns,nf,nt", [(96, 2, 257), (512, 4095, 64), (130, 1, 64), (256, 4, 1000)])
+def test_fp16_loads_in_bounds(ns, nf, nt):
+    """The FP16 real-Gram kernels: register-staged rgh<32>, rgh<64> and the DMA
+    kernel rgdh<32,4>, up to the largest FP16 block they take (NF=4095 at
+    NSTATION=512: tstride 16.8 MB)."""
+    assert 128 * nf * ns * 8 < 2 ** 31 - 4096
+    nblk = (2 * ns + TILE - 1) // TILE
+    for f in sorted({0, nf - 1}):
+        for rb in range(nblk):
+            for cb in range(rb + 1):
+                for stg in (32, 64):
+                    model_rg_loads(ns, nf, nt, f, rb, cb, stg, fmt="f16")
+                model_dma(ns, nf, nt, f, rb, cb, 32, 4, fmt="f16")
+
+
+@pytest.mark.parametrize("ns,nf,nt", [(96, 2, 257), (512, 8191, 64), (132, 2, 64), (256, 4, 1000)])
+def test_kg_dma_in_bounds(ns, nf, nt):
+    nblk = (2 * ns + TILE - 1) // TILE
+    for f in sorted({0, nf - 1}):
+        for cp in range((nblk + 1) // 2):
+            for rb in range(2 * cp, nblk):
+                model_kg_dma(ns, nf, nt, f, rb, cp)
 
 
 def model_reads(stg, nbuf, diag):
@@ -263,7 +392,8 @@ def test_dma_addresses_at_the_offset_limit():
     for f in (0, nf - 1):
         for rb in range(nblk):
             for cb in range(rb + 1):
-                model_dma(ns, nf, nt, f, rb, cb, 32, 8)
+                for stg, nbuf in ((32, 8), (64, 4)):  # rgd<32,8>; rgd<64,4> and rgd16
+                    model_dma(ns, nf, nt, f, rb, cb, stg, nbuf)
 
 
 @pytest.mark.parametrize("ns,nf", [(512, 64), (512, 4096), (512, 8191), (2048, 2047), (96, 2)])

@@ -757,15 +757,29 @@ def test_i8_cfg3_shape_exact_by_properties():
     assert not tri[:, d][:, :, [0, 3], 1].any()  # autocorrelations are real
 
 
+def _limit_dump_check(eng, tri_d, nf, ns, ref_chans, chans, compare):
+    """Device triangular dump of eng into tri_d, then each of `chans` copied
+    back and compared with ref_chans[c] by `compare`."""
+    X.check(X.core().xgpuAmdDump(eng.handle, tri_d.ptr, X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR))
+    nb = ns * (ns + 1) // 2
+    for c, f in enumerate(chans):
+        dt = np.int32 if eng.out_dtype == np.int32 else np.float32
+        got = tri_d.download(dt, nb * 8, offset=f * nb * 8 * 4)
+        compare(got, ref_chans[c], f)
+
+
 @pytest.mark.parametrize("nf", [8191, 8192])
 def test_i8_largest_blocks_at_the_offset_limit(nf):
     """NSTATION=512 x 2 pol, NTIME=64: NFREQUENCY=8191 is the largest block the
     real-Gram kernels take (128 * tstride just below 2^31 - 4096: their in-stage
     byte offsets are 32-bit), 8192 the smallest that falls back to the
-    complex-plane kernel (64-bit offsets).  About 34 GB of int32 output each.
-    Exact on every visibility of the first, middle and last channel, and on
-    sampled cells of every 64th channel, through a device triangular dump of
-    which only those parts are copied back."""
+    complex-plane kernel (64-bit offsets).  About 34 GB of int32 output.  At
+    8191 every int8 kernel variant runs (the default DMA kernel, the 64-sample
+    DMA and 16x16x64 variants, the register-staged kernels, the K-interleaved
+    and the complex-plane kernel).  Exact on every visibility of the first,
+    middle and last channel, and (default kernel) on sampled cells of every
+    64th channel, through a device triangular dump of which only those parts
+    are copied back."""
     ns, nt = 512, 64
     assert (128 * nf * ns * 4 < (1 << 31) - 4096) == (nf == 8191)
     flush(f"[i8] offset limit NS={ns} NF={nf} NT={nt}")
@@ -774,18 +788,30 @@ def test_i8_largest_blocks_at_the_offset_limit(nf):
     buf.fill_uniform_int8(nf, -128, 127)
     x = buf.download(np.int8)
     eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
-    eng.correlate(buf, nt)
-    buf.free()
     tri_d = X.DeviceBuffer(2 * eng.tri_len * 4)
-    X.check(X.core().xgpuAmdDump(eng.handle, tri_d.ptr, X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR))
-    eng.close()
-    nb = ns * (ns + 1) // 2
-    chan_vals = nb * 8  # int32 values per channel
     chans = [0, nf // 2, nf - 1]
     whole = oracle.xengine_i64_channels(x, ns, nf, nt, chans)
-    for c, f in enumerate(chans):
-        got = tri_d.download(np.int32, chan_vals, offset=f * chan_vals * 4).reshape(nb, 4, 2)
-        assert np.array_equal(got.astype(np.int64), whole[c]), f
+
+    def exact(got, ref, f):
+        assert np.array_equal(got.reshape(-1, 4, 2).astype(np.int64), ref), f
+
+    variants = [(0, 0, 0, 0)]
+    if nf == 8191:
+        variants += [(2, 2, 64, 0), (2, 2, 64, 2), (2, 1, 64, 0), (2, 1, 128, 0), (3, 0, 0, 0),
+                     (1, 0, 0, 0)]
+    for kernel, staging, stage, mfma in variants:
+        flush(f"[i8] offset limit NF={nf} kernel={kernel} staging={staging} stage={stage} mfma={mfma}")
+        eng.set_tuning(kernel=kernel, staging=staging, stage=stage, mfma=mfma)
+        eng.clear()
+        eng.correlate(buf, nt)
+        _limit_dump_check(eng, tri_d, nf, ns, whole, chans, exact)
+    eng.set_tuning()
+    eng.clear()
+    eng.correlate(buf, nt)
+    X.check(X.core().xgpuAmdDump(eng.handle, tri_d.ptr, X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR))
+    eng.close()
+    buf.free()
+    nb = ns * (ns + 1) // 2
     rng = np.random.default_rng(nf)
     fij = [(f, int(i), int(rng.integers(i + 1))) for f in range(0, nf, 64)
            for i in rng.integers(0, ns, 2)]
@@ -794,6 +820,42 @@ def test_i8_largest_blocks_at_the_offset_limit(nf):
         k = f * nb + i * (i + 1) // 2 + j
         got = tri_d.download(np.int32, 8, offset=k * 32).reshape(4, 2)
         assert np.array_equal(got.astype(np.int64), ref[n]), (f, i, j)
+    tri_d.free()
+
+
+def test_f16_largest_blocks_at_the_offset_limit():
+    """FP16 at NSTATION=512 x 2 pol, NFREQUENCY=4095, NTIME=64: the largest
+    block the FP16 real-Gram kernels take (tstride 16.8 MB; 17 GB of float
+    output), for rgh<32> (default), rgh<64> and the DMA kernel rgdh<32,4>:
+    the first, middle and last channel against the FP64 oracle within the
+    fp32 accumulation bound."""
+    ns, nf, nt = 512, 4095, 64
+    assert 128 * nf * ns * 8 < (1 << 31) - 4096
+    flush(f"[f16] offset limit NS={ns} NF={nf} NT={nt}")
+    # 64 sample rows of complex Gaussian fp16 values (256 KiB) tiled over the
+    # 1.07 GB block: sample row (t, f) gets row (t nf + f) mod 64
+    base = f16_input(64, 1, ns, 91).reshape(-1)
+    assert (nt * nf * ns * 4) % base.size == 0
+    x16 = np.tile(base, nt * nf * ns * 4 // base.size).reshape(nt, nf, ns, 2, 2)
+    buf = X.DeviceBuffer.from_array(x16)
+    chans = [0, nf // 2, nf - 1]
+    ref = oracle.xengine_f64_f16(np.ascontiguousarray(x16[:, chans]), ns, len(chans), nt)
+    ref = ref.reshape(len(chans), -1)
+    del x16
+    eng = X.Engine(ns, nf, compute=X.XGPU_FLOAT16, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    tri_d = X.DeviceBuffer(2 * eng.tri_len * 4)
+
+    def close(got, r, f):
+        assert_f16_close(got, r, ns, 1, nt)
+
+    for kernel, staging, stage in [(0, 0, 0), (2, 1, 64), (2, 2, 32)]:
+        flush(f"[f16] offset limit kernel={kernel} staging={staging} stage={stage}")
+        eng.set_tuning(kernel=kernel, staging=staging, stage=stage)
+        eng.clear()
+        eng.correlate(buf, nt)
+        _limit_dump_check(eng, tri_d, nf, ns, ref, chans, close)
+    eng.close()
+    buf.free()
     tri_d.free()
 
 
