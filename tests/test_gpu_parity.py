@@ -823,6 +823,45 @@ def test_i8_largest_blocks_at_the_offset_limit(nf):
     tri_d.free()
 
 
+def test_i8_8192_stations():
+    """NSTATION=8192 x 2 pol (16384 inputs, 8256 tiles per channel), NFREQUENCY=8,
+    NTIME=256, register-tile order (8.6 GB of int32 output): every visibility
+    of the last channel and every autocorrelation of the first against the
+    int64 oracle and the input power, and sampled cells of every channel,
+    through a device triangular dump."""
+    ns, nf, nt = 8192, 8, 256
+    flush(f"[i8] NS={ns} NF={nf} NT={nt}")
+    buf = X.DeviceBuffer(nt * nf * ns * 4)
+    buf.fill_uniform_int8(8192, -128, 127)
+    x = buf.download(np.int8)
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    eng.correlate(buf, nt)
+    buf.free()
+    tri_d = X.DeviceBuffer(2 * eng.tri_len * 4)
+    X.check(X.core().xgpuAmdDump(eng.handle, tri_d.ptr, X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR))
+    eng.close()
+    nb = ns * (ns + 1) // 2
+    last = tri_d.download(np.int32, nb * 8, offset=(nf - 1) * nb * 32).reshape(nb, 4, 2)
+    flush("[i8] NS=8192: whole last channel against the oracle")
+    ref = oracle.xengine_i64_channels(x, ns, nf, nt, [nf - 1])[0]
+    assert np.array_equal(last.astype(np.int64), ref)
+    del last, ref
+    first = tri_d.download(np.int32, nb * 8, offset=0).reshape(nb, 4, 2)
+    xt = x.reshape(nt, nf, ns, 2, 2)[:, 0].astype(np.int32)
+    power = (xt * xt).sum(axis=(0, 3), dtype=np.int64)  # [s, p]
+    d = np.arange(ns) * (np.arange(ns) + 1) // 2 + np.arange(ns)
+    assert np.array_equal(first[d][:, [0, 3], 0].astype(np.int64), power)
+    del first
+    rng = np.random.default_rng(8192)
+    fij = [(f, int(i), int(rng.integers(i + 1))) for f in range(nf) for i in rng.integers(0, ns, 32)]
+    ref = oracle.xengine_i64_pairs(x, ns, nf, nt, fij)
+    for n, (f, i, j) in enumerate(fij):
+        k = f * nb + i * (i + 1) // 2 + j
+        got = tri_d.download(np.int32, 8, offset=k * 32).reshape(4, 2)
+        assert np.array_equal(got.astype(np.int64), ref[n]), (f, i, j)
+    tri_d.free()
+
+
 def test_f16_largest_blocks_at_the_offset_limit():
     """FP16 at NSTATION=512 x 2 pol, NFREQUENCY=4095, NTIME=64: the largest
     block the FP16 real-Gram kernels take (tstride 16.8 MB; 17 GB of float
