@@ -354,7 +354,8 @@ def model_epilogue16(ns, nf, f, rb, cb, order, accumulate, hits):
                     hits[p1:p1 + 4] += 1
 
 
-SHAPES = [(96, 2, 257, "realimag"), (96, 2, 257, "regtile"), (96, 2, 257, "tri"),
+SHAPES = [(96, 2, 0, "regtile"), (64, 2, 0, "tri"),  # empty blocks (a batch job of 0 samples)
+          (96, 2, 257, "realimag"), (96, 2, 257, "regtile"), (96, 2, 257, "tri"),
           (64, 1, 100, "tri"), (132, 2, 64, "regtile"), (16, 2, 64, "regtile"),
           (256, 4, 1000, "regtile"), (6 * 4, 3, 65, "tri")]
 

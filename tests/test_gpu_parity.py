@@ -823,6 +823,35 @@ def test_i8_largest_blocks_at_the_offset_limit(nf):
     tri_d.free()
 
 
+def test_i8_empty_blocks():
+    """ntime = 0: a plain call leaves the integration as it was; in a batch,
+    an empty job leaves its engine's integration as it was (zeros after a
+    clear) and the other jobs are exact."""
+    flush("[i8] empty blocks")
+    xa = oracle.fill_uniform(128 * 2 * 64 * 4, 61, -128, 127)
+    xb = oracle.fill_uniform(200 * 3 * 96 * 4, 62, -128, 127)
+    ra, rb = oracle.xengine_i64(xa, 64, 2, 128), oracle.xengine_i64(xb, 96, 3, 200)
+    ba, bb = X.DeviceBuffer.from_array(xa), X.DeviceBuffer.from_array(xb)
+    a = X.Engine(64, 2, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
+    b = X.Engine(96, 3, compute=X.XGPU_INT8, order=X.TRIANGULAR_ORDER)
+    a.correlate(ba, 128)
+    a.correlate(ba, 0)
+    assert np.array_equal(a.dump(wide=True), ra)
+    X.correlate_batch([(a, ba, 0), (b, bb, 200)])
+    assert np.array_equal(a.dump(wide=True), ra)
+    assert np.array_equal(b.dump(wide=True), rb)
+    a.clear()
+    X.correlate_batch([(a, ba, 0), (b, bb, 0)])
+    assert not a.dump(wide=True).any()
+    assert np.array_equal(b.dump(wide=True), rb)
+    a.correlate(ba, 128)
+    assert np.array_equal(a.dump(wide=True), ra)
+    a.close()
+    b.close()
+    ba.free()
+    bb.free()
+
+
 def test_i8_8192_stations():
     """NSTATION=8192 x 2 pol (16384 inputs, 8256 tiles per channel), NFREQUENCY=8,
     NTIME=256, register-tile order (8.6 GB of int32 output): every visibility
