@@ -595,6 +595,32 @@ def test_batch_ragged_equals_alone(kernel):
         e.close()
 
 
+def test_batch_of_the_maximum_16_jobs():
+    """16 engines of different station counts, channel counts, block lengths
+    and output orders in one launch (the most a batch takes): each equals its
+    oracle; a second batch of every other engine accumulates."""
+    rng = np.random.default_rng(16)
+    orders = [X.TRIANGULAR_ORDER, X.REAL_IMAG_TRIANGULAR_ORDER, X.REGISTER_TILE_TRIANGULAR_ORDER]
+    cases = []
+    for k in range(16):
+        ns = int(rng.choice([4, 8, 16, 28, 64, 96, 132, 200]))
+        nf, nt = int(rng.integers(1, 6)), int(rng.integers(1, 400))
+        order = orders[k % 3]
+        x = oracle.fill_uniform(nt * nf * ns * 4, 700 + k, -128, 127)
+        eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=order)
+        cases.append((eng, X.DeviceBuffer.from_array(x), nt, oracle.xengine_i64(x, ns, nf, nt),
+                      ns, nf, order))
+    flush(f"[i8] batch of 16: {[(c[4], c[5], c[2]) for c in cases]}")
+    X.correlate_batch([(e, b, nt) for e, b, nt, *_ in cases])
+    for e, b, nt, ref, ns, nf, order in cases:
+        assert np.array_equal(to_tri(None, e.dump(wide=True), ns, nf, order), ref), (ns, nf, nt)
+    X.correlate_batch([(e, b, nt) for e, b, nt, *_ in cases[::2]])
+    for k, (e, b, nt, ref, ns, nf, order) in enumerate(cases):
+        assert np.array_equal(to_tri(None, e.dump(wide=True), ns, nf, order), (2 - k % 2) * ref)
+        e.close()
+        b.free()
+
+
 # ------------------------------------------------------------------ host path + graphs
 def _host_case(ns=64, nf=4, nt=512, ntp=64, seed=9):
     xs = [oracle.fill_uniform(nt * nf * ns * 4, seed + k, -128, 127) for k in range(3)]

@@ -82,6 +82,8 @@ def schedule(L):
     ([(132, 3)], 256, 1),             # partial last block
     ([(64, 3), (96, 2), (200, 1), (16, 5)], 256, 1),   # ragged batch
     ([(512, 5), (1000, 3)], 40, 2),   # few CUs: many rounds per XCD
+    ([(16 + 12 * j, 1 + j % 5) for j in range(16)], 256, 1),   # a batch of the maximum 16 jobs
+    ([(512, 8191)], 256, 1),          # the largest real-Gram block: 294,876 tiles
 ])
 def test_every_tile_exactly_once(jobs, cus, wg):
     L = plan(jobs, cus, wg)
