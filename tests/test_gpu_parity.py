@@ -660,6 +660,55 @@ def test_host_path_pipeline_and_graph_replays(graph):
         h.free()
 
 
+@pytest.mark.parametrize("graph", [2, 1])
+def test_f16_host_path_batch_and_graph_steps(graph):
+    """The FP16 path through the host-buffer pipeline (chunked transfers and
+    kernels, captured in hipGraphs with the graph knob), the device path with
+    the same knob, and a ragged batch of three FP16 engines: each against the
+    FP64 oracle within the fp32 accumulation bound."""
+    ns, nf, nt, ntp = 64, 3, 512, 128
+    x = f16_input(nt, nf, ns, 81 + graph)
+    ref = oracle.xengine_f64_f16(x, ns, nf, nt)
+    flush(f"[f16] host path, graph={graph}")
+    eng = X.Engine(ns, nf, nt, ntp, compute=X.XGPU_FLOAT16, order=X.TRIANGULAR_ORDER)
+    eng.set_tuning(graph=graph)
+    hb = X.HostBuffer(x.nbytes)
+    hb.array[:] = x.reshape(-1).view(np.int8)
+    out = np.zeros(2 * eng.mat_len, dtype=np.float32)
+    lib = X.core()
+    for rep in range(3):
+        eng.clear()
+        X.check(lib.xgpuAmdCorrelateHost(eng.handle, hb.ptr, X.SYNCOP_DUMP, out.ctypes.data))
+        assert_f16_close(out, ref, ns, nf, nt)
+    buf = X.DeviceBuffer.from_array(x)
+    for rep in range(3):  # device path: captured once with the knob on, then replayed
+        eng.clear()
+        eng.correlate(buf, nt)
+        assert_f16_close(eng.dump(), ref, ns, nf, nt)
+    eng.close()
+    hb.free()
+    # ragged FP16 batch
+    shapes = [(64, 3, 512), (96, 2, 100), (130, 1, 64)]
+    cases = []
+    for k, (bns, bnf, bnt) in enumerate(shapes):
+        xk = f16_input(bnt, bnf, bns, 90 + k)
+        e = X.Engine(bns, bnf, compute=X.XGPU_FLOAT16, order=X.TRIANGULAR_ORDER)
+        e.set_tuning(graph=graph)
+        cases.append((e, X.DeviceBuffer.from_array(xk), bnt, oracle.xengine_f64_f16(xk, bns, bnf, bnt),
+                      bns, bnf))
+    flush(f"[f16] ragged batch, graph={graph}")
+    for rep in range(2):
+        for e, *_ in cases:
+            e.clear()
+        X.correlate_batch([(e, b, t) for e, b, t, *_ in cases])
+        for e, b, t, r, bns, bnf in cases:
+            assert_f16_close(e.dump(), r, bns, bnf, t)
+    for e, b, *_ in cases:
+        e.close()
+        b.free()
+    buf.free()
+
+
 def test_autotune_keeps_results_exact(tmp_path):
     """Every candidate the autotuner times is exact; the tuner leaves the
     caller's integration as it was (candidates run on a scratch engine); the
