@@ -370,7 +370,8 @@ def test_dma_addresses_in_bounds(ns, nf, nt, order, stg, nbuf):
                 model_dma(ns, nf, nt, f, rb, cb, stg, nbuf)
 
 
-@pytest.mark.parametrize("ns,nf,nt", [(512, 64, 8192), (2048, 16, 4096), (8192, 8, 256)])
+@pytest.mark.parametrize("ns,nf,nt", [(512, 64, 8192), (2048, 16, 4096), (8192, 8, 256),
+                                      (32640, 1, 64)])
 def test_dma_addresses_in_bounds_large(ns, nf, nt):
     """First and last channel, every tile, the stages at both ends of the
     block and past it (the address arithmetic is affine in the stage)."""

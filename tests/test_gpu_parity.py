@@ -966,6 +966,37 @@ def test_i8_8192_stations():
     tri_d.free()
 
 
+def test_i8_largest_station_count():
+    """NSTATION=32640 x 2 pol, the largest station count an engine takes (510
+    row blocks, 130,305 tiles in one channel), NFREQUENCY=1, NTIME=64 (17 GB of
+    int32 output): the whole last row of baselines (i = 32639, every j) and
+    sampled cells against the int64 oracle, through a device triangular dump."""
+    ns, nf, nt = 32640, 1, 64
+    flush(f"[i8] NS={ns} NF={nf} NT={nt}")
+    buf = X.DeviceBuffer(nt * nf * ns * 4)
+    buf.fill_uniform_int8(ns, -128, 127)
+    x = buf.download(np.int8)
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
+    eng.correlate(buf, nt)
+    buf.free()
+    tri_d = X.DeviceBuffer(2 * eng.tri_len * 4)
+    X.check(X.core().xgpuAmdDump(eng.handle, tri_d.ptr, X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR))
+    eng.close()
+    i = ns - 1
+    row = tri_d.download(np.int32, (i + 1) * 8, offset=i * (i + 1) // 2 * 32).reshape(i + 1, 4, 2)
+    ref = oracle.xengine_i64_pairs(x, ns, nf, nt, [(0, i, j) for j in range(i + 1)])
+    assert np.array_equal(row.astype(np.int64), ref)
+    rng = np.random.default_rng(ns)
+    fij = [(0, int(a), int(rng.integers(a + 1))) for a in rng.integers(0, ns, 300)]
+    fij += [(0, int(a), int(a)) for a in rng.integers(0, ns, 100)]
+    ref = oracle.xengine_i64_pairs(x, ns, nf, nt, fij)
+    for n, (f, a, b) in enumerate(fij):
+        k = a * (a + 1) // 2 + b
+        got = tri_d.download(np.int32, 8, offset=k * 32).reshape(4, 2)
+        assert np.array_equal(got.astype(np.int64), ref[n]), (a, b)
+    tri_d.free()
+
+
 def test_f16_largest_blocks_at_the_offset_limit():
     """FP16 at NSTATION=512 x 2 pol, NFREQUENCY=4095, NTIME=64: the largest
     block the FP16 real-Gram kernels take (tstride 16.8 MB; 17 GB of float
