@@ -332,3 +332,70 @@ def test_serving_fp16_streams_beside_int8():
     srv.close()
     for b in keep:
         b.free()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_serving_randomized(seed):
+    """Random stream shapes, host or device blocks (device blocks back to back,
+    so they fuse), host or device dump rings of 2 slots, random staging pool,
+    batch and in-flight limits; two threads submit in a random interleaving
+    (each owns half the streams, so a stream's blocks stay in order).  Every
+    stream's last two dumps are exact."""
+    import threading
+    rng = np.random.default_rng(seed)
+    D = 3
+    srv = X.Server(staging_bytes=int(rng.choice([4 << 20, 64 << 20])),
+                   max_batch=int(rng.integers(1, 17)), max_inflight=int(rng.integers(1, 4)))
+    streams = []
+    for k in range(6):
+        ns = int(rng.choice([8, 16, 64, 96, 200]))
+        nf, nt, K = int(rng.integers(1, 6)), int(rng.choice([32, 100, 256, 512])), int(rng.integers(1, 7))
+        dev, ddev = bool(rng.integers(2)), bool(rng.integers(2))
+        nb = nt * nf * ns * 4
+        x = oracle.fill_uniform(D * K * nb, 100 * seed + k, -127, 127)
+        sid, ring = srv.open_stream(ns, nf, K, dump_slots=2, dump_device=ddev)
+        src = X.DeviceBuffer.from_array(x) if dev else X.HostBuffer(x.nbytes)
+        if not dev:
+            src.array[:] = x
+        streams.append(dict(ns=ns, nf=nf, nt=nt, K=K, dev=dev, ddev=ddev, nb=nb, x=x, sid=sid,
+                            ring=ring, src=src))
+    flush(f"[serve] randomized seed {seed}: "
+          f"{[(s['ns'], s['nf'], s['nt'], s['K'], s['dev'], s['ddev']) for s in streams]}")
+    errors = []
+
+    def submitter(mine, r):
+        try:
+            nxt = {i: 0 for i in mine}
+            while any(nxt[i] < D * streams[i]["K"] for i in mine):
+                i = mine[int(r.integers(len(mine)))]
+                s = streams[i]
+                if nxt[i] == D * s["K"]:
+                    continue
+                b = nxt[i]
+                nxt[i] += 1
+                if s["dev"]:
+                    srv.submit_device(s["sid"], s["src"].ptr + b * s["nb"], s["nt"])
+                else:
+                    srv.submit(s["sid"], s["src"].ptr + b * s["nb"], s["nt"])
+        except Exception as exc:  # reported by the main thread
+            errors.append(exc)
+
+    th = [threading.Thread(target=submitter, args=(mine, np.random.default_rng(seed * 10 + t)))
+          for t, mine in enumerate(([0, 1, 2], [3, 4, 5]))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    srv.drain()
+    for s in streams:
+        assert srv.dumps_done(s["sid"]) == D
+        n = 2 * oracle.tri_len(s["ns"], s["nf"])
+        ring = s["ring"].download(np.int64) if s["ddev"] else s["ring"].array.copy()
+        for d in (D - 2, D - 1):  # dump d sits in slot d % 2
+            ref = sum(oracle.xengine_i64(s["x"][(d * s["K"] + b) * s["nb"]:(d * s["K"] + b + 1) * s["nb"]],
+                                         s["ns"], s["nf"], s["nt"]) for b in range(s["K"]))
+            assert np.array_equal(ring[(d % 2) * n:(d % 2 + 1) * n], ref), (seed, s["ns"], d)
+    srv.close()
+    for s in streams:
+        s["src"].free()
