@@ -286,6 +286,15 @@ def kernel_name(tun):
     return f"xg_xengine_rg<{tun['stage']}>"
 
 
+def f16_kernel_name(tun):
+    """Kernel symbol of an FP16 tuning (as rocprofv3 lists it)."""
+    if tun["kernel"] == 1:
+        return "xg_xengine_mfma<true, false>"
+    if tun["staging"] == 2:
+        return "xg_xengine_rgdh<32, 4>"
+    return f"xg_xengine_rgh<{tun['stage']}>"
+
+
 def profile_only(X, which, steps, warmup, kernel=0, stage=0, staging=0):
     """Run one path's workload warmup+steps times (one kernel per step).
     "copy": a plain 1 GiB device-to-device copy per step (counter calibration);
@@ -606,7 +615,8 @@ def main():
     b16 = X.DeviceBuffer.from_array(x16)
     e16 = X.Engine(ns16, nf16, compute=X.XGPU_FLOAT16, order=X.REGISTER_TILE_TRIANGULAR_ORDER)
     # warm-up past the power-state transient (~35 ms of MFMA work)
-    ms16 = e16.time_correlate(b16, nt16, warmup=100, reps=20)  # default: real-Gram rgh<32>
+    ms16 = e16.time_correlate(b16, nt16, warmup=100, reps=20)  # the default FP16 kernel
+    kname16 = f16_kernel_name(e16.get_tuning())
     e16.clear()
     e16.correlate(b16, nt16)
     got16 = e16.dump(triangular=True).reshape(nf16, -1)
@@ -725,7 +735,7 @@ def main():
         "fp16_mfma": {"workload": "NSTATION=256 x 2 pol, NFREQUENCY=128, NTIME=2048, fp16 input "
                                   "(complex Gaussian sigma 16 plus a tone of amplitude 4 with a "
                                   "random delay per station, seed 4)",
-                      "kernel": "xg_xengine_rgh<32>", "tcmac_s": round(t16, 2),
+                      "kernel": kname16, "tcmac_s": round(t16, 2),
                       "ms": round(ms16, 4), "complex_plane_kernel_ms": round(ms16_cp, 4),
                       "pct_fp16_peak": round(100 * t16 * 8 / PEAK_FP16_TFLOPS, 1),
                       "mfma_only_ceiling_same_input_tcmac_s": round(ceil16, 1),

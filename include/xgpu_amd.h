@@ -214,11 +214,10 @@ typedef struct XGPUAmdTuningStruct {
                       nstation % 4 == 0; default); int8: 3 = K-interleaved kernel
                       (pair tiles of 128 x 256 inputs) where the real-Gram one applies */
   int stage;       /* real-Gram kernel: samples per LDS stage: int8 64 or 128 with
-                      register staging, 32 or 64 with DMA staging; fp16 32 or 64
-                      (0 = default) */
-  int staging;     /* int8 real-Gram kernel: 1 = global -> VGPR -> LDS, 2 = global -> LDS
-                      DMA (0 = default); fp16: 1 = registers (default), 2 = DMA
-                      (stage 32) */
+                      register staging, 32 or 64 with DMA staging; fp16 32 or 64 with
+                      register staging, 32 with DMA staging (0 = default) */
+  int staging;     /* real-Gram kernel: 1 = global -> VGPR -> LDS, 2 = global -> LDS DMA
+                      (0 = default: DMA for int8 and fp16) */
   int ntimepipe;   /* host-buffer path: samples per host -> device transfer and kernel
                       launch (the time-pipe length; divides the config's ntime, a multiple
                       of 4 for time-swizzled input; 0 = the config's ntimepipe) */

@@ -271,7 +271,7 @@ int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
   e->kernel_pref = 2;
   if (e->family == XG_KERNEL_F16_MFMA) {
     e->stage = XG_DEFAULT_RGH_STAGE;
-    e->staging = 1;
+    e->staging = XG_DEFAULT_RGH_STAGING;
   } else {
     e->stage = XG_DEFAULT_RG_STAGE;
     e->staging = XG_DEFAULT_RG_STAGING;
@@ -355,7 +355,7 @@ int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
   if (e->family == XG_KERNEL_F16_MFMA) {
     /* fp16 real-Gram kernel: register staging (32 or 64 samples per stage)
      * or global -> LDS DMA (32) */
-    staging = t->staging ? t->staging : 1;
+    staging = t->staging ? t->staging : XG_DEFAULT_RGH_STAGING;
     if (staging == 1 && t->stage != 0 && t->stage != 32 && t->stage != 64)
       return xg_fail(XGPU_INVALID_CONFIG, "fp16 stage must be 0, 32 or 64 with register staging");
     if (staging == 2 && t->stage != 0 && t->stage != 32)

@@ -84,6 +84,9 @@ struct XGPUAmdEngineStruct {
 #define XG_DEFAULT_RG_STAGE 32
 /* samples per LDS stage of the fp16 real-Gram kernel (register staging) */
 #define XG_DEFAULT_RGH_STAGE 32
+/* fp16 real-Gram kernel: global -> LDS DMA staging by default (bit-identical to
+ * register staging, 0-8 % faster at NS=256 x 2 pol, NF=128, NT=2048) */
+#define XG_DEFAULT_RGH_STAGING 2
 
 int xg_fail(int code, const char *fmt, ...);
 void xg_plan_launch(XgLaunch *L, int cus, int wg_per_cu);
