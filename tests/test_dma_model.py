@@ -475,3 +475,72 @@ def test_kernel_source_clamps_rows_before_the_range_product():
         assert d >= 0
         line = before[d:before.find("\n", d)]
         assert re.match(r"const int rows = max\(min\([^;]*\), 0\);", line), line
+
+
+def sweep_kernel(case, fmt, call, per):
+    """The kernel the engine launches for call `call` of a sweep case
+    (xg_launch_family, rg_stage_code): None for the complex-plane kernel,
+    which takes nstation % 4 != 0 (int8), input that is not 16-byte aligned
+    and tuning kernel 1."""
+    ns, nf, nt, order, calls, t = case
+    ts = nf * ns * 2 * FMT[fmt]["bpi"]
+    if t["kernel"] == 1 or (fmt == "i8" and ns % 4) or (call * per * ts) % 16:
+        return None
+    if fmt == "i8" and t["kernel"] == 3:
+        return ("kg",)
+    if t["staging"] == 1:
+        return ("rg", t["stage"])
+    if fmt == "f16":
+        return ("dma", 32, 4, model_epilogue)
+    if t["stage"] == 32:
+        return ("dma", 32, 8, model_epilogue)
+    return ("dma", 64, 4, model_epilogue16 if t["mfma"] == 2 else model_epilogue)
+
+
+def _sweep_params():
+    import shape_sweep as S
+    return ([pytest.param(c, "i8", id="i8-" + S.case_id(c)) for c in S.I8_SWEEP] +
+            [pytest.param(c, "f16", id="f16-" + S.case_id(c)) for c in S.F16_SWEEP])
+
+
+@pytest.mark.parametrize("case,fmt", _sweep_params())
+def test_shape_sweep_in_bounds(case, fmt):
+    """Every case of the GPU parity sweep, modelled before it runs: per call,
+    the loads of the kernel the engine picks (every tile of the first and last
+    channel, every stage issued) and, for the real-Gram kernels, the output
+    stores of every tile of every channel, each visibility exactly once."""
+    import shape_sweep as S
+    ns, nf, nt, order, calls, t = case
+    lens = S.call_lengths(nt, calls)
+    nblk = (2 * ns + TILE - 1) // TILE
+    nb = ns * (ns + 1) // 2
+    P = ns // 2
+    plane = nf * P * (P + 1) // 2 * 16 if order == "regtile" else nf * nb * 4
+    for call, n in enumerate(lens):
+        k = sweep_kernel(case, fmt, call, lens[0])
+        if k is None:
+            continue
+        for f in sorted({0, nf - 1}):
+            if k[0] == "kg":  # pair tiles: column blocks 2 cp, 2 cp + 1; rows from 2 cp
+                for cp in range((nblk + 1) // 2):
+                    for rb in range(2 * cp, nblk):
+                        model_kg_dma(ns, nf, n, f, rb, cp)
+                continue
+            for rb in range(nblk):
+                for cb in range(rb + 1):
+                    if k[0] == "rg":
+                        model_rg_loads(ns, nf, n, f, rb, cb, k[1], fmt=fmt)
+                    else:
+                        model_dma(ns, nf, n, f, rb, cb, k[1], k[2], fmt=fmt)
+        if k[0] == "kg":
+            continue  # its epilogue (kg_store_cells) is covered by the GPU tests alone
+        epilogue = k[3] if k[0] == "dma" else model_epilogue
+        hits = np.zeros(2 * plane, dtype=np.int64)
+        for f in range(nf):
+            for rb in range(nblk):
+                for cb in range(rb + 1):
+                    epilogue(ns, nf, f, rb, cb, order, int(call > 0), hits)
+        if order == "regtile" and call > 0:
+            assert hits.max() == 1 and (hits == 1).sum() == 2 * nf * nb * 4
+        else:
+            assert (hits == 1).all()

@@ -18,6 +18,7 @@ import xgpu_amd as X
 from xgpu_amd import workloads as W
 from xgpu_amd import _lib
 from conftest import flush
+import shape_sweep as S
 
 pytestmark = pytest.mark.gpu
 
@@ -1096,3 +1097,34 @@ def test_f16_against_fp32_plain_on_int8_valued_cfg4():
     assert s32["max_norm_err"] <= 2 * nt * u, s32
     assert d["max_norm_err"] <= (nt / 16 + 2 * nt) * u, d
     assert s16["rms_rel_err"] <= 1e-6 and s32["rms_rel_err"] <= 1e-5, (s16, s32)
+
+
+# ------------------------------------------------------------ shape sweep
+# Seeded random shapes over every kernel variant (tests/shape_sweep.py); the
+# loads and stores of each case are modelled on the CPU first
+# (test_dma_model.py::test_shape_sweep_in_bounds).
+
+
+@pytest.mark.parametrize("case", [pytest.param(c, id=S.case_id(c)) for c in S.I8_SWEEP])
+def test_i8_shape_sweep(case):
+    """Bit-exact against the int64 oracle, 32-bit and 64-bit dumps, whichever
+    kernel the engine launches for the case's tuning and each call's input
+    (ragged tiles, time tails, accumulating calls, unaligned later calls)."""
+    ns, nf, nt, order, calls, t = case
+    x = oracle.fill_uniform(nt * nf * ns * 4, 31 * ns + nt, -128, 127)
+    raw, wide = run_i8(ns, nf, nt, x, ORDERS[order], calls, **t)
+    ref = oracle.xengine_i64(x, ns, nf, nt)
+    assert np.array_equal(to_tri(None, raw, ns, nf, ORDERS[order]).astype(np.int64), ref)
+    assert np.array_equal(to_tri(None, wide, ns, nf, ORDERS[order]), ref)
+
+
+@pytest.mark.parametrize("case", [pytest.param(c, id=S.case_id(c)) for c in S.F16_SWEEP])
+def test_f16_shape_sweep(case):
+    """Against the FP64 oracle of the same fp16 input, with the bounds of
+    assert_f16_close; the 64-bit dump equals the 32-bit one."""
+    ns, nf, nt, order, calls, t = case
+    x16 = f16_input(nt, nf, ns, 17 * ns + nt)
+    got, wide = run_f16(ns, nf, nt, x16, ORDERS[order], calls, **t)
+    ref = oracle.xengine_f64_f16(x16, ns, nf, nt)
+    assert_f16_close(got, ref, ns, nf, nt)
+    assert np.array_equal(wide, got.astype(np.float64))
