@@ -619,6 +619,9 @@ struct RgInt8 {
   static constexpr int kChunks = 16;   // 16-byte chunks per row
   static constexpr int kInputs = 8;    // inputs per chunk
   static constexpr int kK = 32;        // samples per MFMA
+  // the DMA kernel's diagonal waves issue all 16 MFMA blocks (skipping the 6
+  // above the diagonal measured no faster for int8, and spilled)
+  static constexpr bool kDiagLower = false;
   __device__ static int slot(int chunk, int srow) { return rg_slot(chunk, srow & 7); }
   // the chunk that belongs in `slot` of sample row `srow` (inverse of slot)
   __device__ static int dma_chunk(int slot, int srow) {
@@ -638,6 +641,9 @@ struct RgHalf {
   static constexpr int kChunks = 32;
   static constexpr int kInputs = 4;
   static constexpr int kK = 16;
+  // the DMA kernel's diagonal waves issue only the 10 MFMA blocks on or below
+  // the diagonal (2 % faster at NS=256, NF=128, NT=2048)
+  static constexpr bool kDiagLower = true;
   __device__ static int slot(int chunk, int srow) { return rgh_slot(chunk, srow & 3); }
   __device__ static int dma_chunk(int slot, int srow) { return slot ^ ((srow & 3) << 2); }
   __device__ static int lane_off(int blk, int lane) { return rgh_lane_off(blk, lane); }
@@ -860,6 +866,14 @@ __device__ __forceinline__ void rg_finish(const XgJob &J, const TileRef &T, Acc 
 // columns run backwards on odd rows, so consecutive MFMAs share an operand
 // (one operand's bits change per MFMA instead of two every fourth).
 __device__ __forceinline__ constexpr int rg_col(int i) { return ((i >> 2) & 1) ? 3 - (i & 3) : (i & 3); }
+
+// Row and column block of the i-th of the 10 MFMA blocks on or below the
+// diagonal (row block >= column block), in the same snake order: (0,0),
+// (1,1), (1,0), (2,0), (2,1), (2,2), (3,3), (3,2), (3,1), (3,0).
+__device__ __forceinline__ constexpr int rg_lrow(int i) { return i < 1 ? 0 : i < 3 ? 1 : i < 6 ? 2 : 3; }
+__device__ __forceinline__ constexpr int rg_lcol(int i) {
+  return i == 0 ? 0 : i == 1 ? 1 : i == 2 ? 0 : i < 6 ? i - 3 : 9 - i;
+}
 
 // scheduling-group masks (llvm.amdgcn.sched.group.barrier)
 #define SG_MFMA 0x008
@@ -1234,25 +1248,52 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
   };
   // One kstep: 16 MFMAs on (A, B), the 16 transpose reads of the next kstep
   // over the first 8, and (when FILL) the DMAs of stage `st` over the last 8.
-  auto kstep = [&](auto fill_c, const v4i *A, const v4i *B, v4i *NA, v4i *NB,
+  // LOWER (a wave on the diagonal of a diagonal tile, E::kDiagLower): only
+  // the 10 MFMA blocks on or below the diagonal, with the reads over the
+  // first 5 and the DMAs over the last 5.  The 6 blocks above it hold
+  // products of station pairs i < j, which no output order stores.
+  auto kstep = [&](auto fill_c, auto lower_c, const v4i *A, const v4i *B, v4i *NA, v4i *NB,
                    const unsigned char *nbase, int st) {
     constexpr bool FILL = decltype(fill_c)::value;
-    constexpr int kTot = 2 * C::kDma;  // DMAs of one stage, spread over MFMAs 8..15
+    constexpr int kTot = 2 * C::kDma;  // DMAs of one stage
+    if constexpr (!decltype(lower_c)::value) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) acc[i >> 2][rg_col(i)] = E::mfma(A[i >> 2], B[rg_col(i)], acc[i >> 2][rg_col(i)]);
-    lread(nbase, NA, NB);
+      for (int i = 0; i < 8; ++i) acc[i >> 2][rg_col(i)] = E::mfma(A[i >> 2], B[rg_col(i)], acc[i >> 2][rg_col(i)]);
+      lread(nbase, NA, NB);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < 8; ++i) {
+        __builtin_amdgcn_sched_group_barrier(SG_MFMA, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(SG_DS_READ, 2, 0);
+      }
+#pragma unroll
+      for (int i = 8; i < 16; ++i) {
+        acc[i >> 2][rg_col(i)] = E::mfma(A[i >> 2], B[rg_col(i)], acc[i >> 2][rg_col(i)]);
+        if constexpr (FILL) {
+          // DMAs [(i-8) kTot / 8, (i-7) kTot / 8): every DMA exactly once
+#pragma unroll
+          for (int d = (i - 8) * kTot / 8; d < (i - 7) * kTot / 8; ++d) dma(st, d / C::kDma, d % C::kDma);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+        acc[rg_lrow(i)][rg_lcol(i)] = E::mfma(A[rg_lrow(i)], B[rg_lcol(i)], acc[rg_lrow(i)][rg_lcol(i)]);
+      lread(nbase, NA, NB);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        __builtin_amdgcn_sched_group_barrier(SG_MFMA, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(SG_DS_READ, 3, 0);
+      }
       __builtin_amdgcn_sched_group_barrier(SG_MFMA, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(SG_DS_READ, 2, 0);
-    }
+      __builtin_amdgcn_sched_group_barrier(SG_DS_READ, 4, 0);
 #pragma unroll
-    for (int i = 8; i < 16; ++i) {
-      acc[i >> 2][rg_col(i)] = E::mfma(A[i >> 2], B[rg_col(i)], acc[i >> 2][rg_col(i)]);
-      if constexpr (FILL) {
-        // DMAs [(i-8) kTot / 8, (i-7) kTot / 8): every DMA exactly once
+      for (int i = 5; i < 10; ++i) {
+        acc[rg_lrow(i)][rg_lcol(i)] = E::mfma(A[rg_lrow(i)], B[rg_lcol(i)], acc[rg_lrow(i)][rg_lcol(i)]);
+        if constexpr (FILL) {
+          // DMAs [(i-5) kTot / 5, (i-4) kTot / 5): every DMA exactly once
 #pragma unroll
-        for (int d = (i - 8) * kTot / 8; d < (i - 7) * kTot / 8; ++d) dma(st, d / C::kDma, d % C::kDma);
+          for (int d = (i - 5) * kTot / 5; d < (i - 4) * kTot / 5; ++d) dma(st, d / C::kDma, d % C::kDma);
+        }
       }
     }
   };
@@ -1279,35 +1320,45 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
   lread(lds, A0, B0);
   // One stage whose kstep-0 operands are in operand set P.  The last kstep
   // reads the next stage's kstep 0 into set (P + kSteps) & 1.
-  auto run_stage = [&](int s, auto p_c) {
+  auto run_stage = [&](int s, auto p_c, auto lower_c) {
     constexpr int P = decltype(p_c)::value;
     const unsigned char *cur = lds + (s & (NBUF - 1)) * C::kBuf;
     const unsigned char *nxt = lds + ((s + 1) & (NBUF - 1)) * C::kBuf;
 #pragma unroll
     for (int ks = 0; ks < C::kSteps - 1; ++ks) {
       if ((P + ks) & 1)
-        kstep(no(), A1, B1, A0, B0, cur + E::kK * (ks + 1) * E::kRow, 0);
+        kstep(no(), lower_c, A1, B1, A0, B0, cur + E::kK * (ks + 1) * E::kRow, 0);
       else
-        kstep(no(), A0, B0, A1, B1, cur + E::kK * (ks + 1) * E::kRow, 0);
+        kstep(no(), lower_c, A0, B0, A1, B1, cur + E::kK * (ks + 1) * E::kRow, 0);
     }
     // stage s+1 has landed (this wave's DMAs; the barrier covers the others'),
     // and every wave is done reading stage s's buffer, which is refilled with
     // stage s+NBUF during the last kstep
     rgd_barrier<(NBUF - 2) * 2 * C::kDma>();
     if constexpr ((P + C::kSteps - 1) & 1)
-      kstep(yes(), A1, B1, A0, B0, nxt, s + NBUF);
+      kstep(yes(), lower_c, A1, B1, A0, B0, nxt, s + NBUF);
     else
-      kstep(yes(), A0, B0, A1, B1, nxt, s + NBUF);
+      kstep(yes(), lower_c, A0, B0, A1, B1, nxt, s + NBUF);
   };
-  if constexpr ((C::kSteps & 1) == 0) {
-    for (int s = 0; s < nstage; ++s) run_stage(s, integral_c<int, 0>());
-  } else {
-    // the operand sets alternate between stages: two stages per iteration (a
-    // block with an odd number of stages runs one more, zero-filled stage)
-    for (int s = 0; s < nstage; s += 2) {
-      run_stage(s, integral_c<int, 0>());
-      run_stage(s + 1, integral_c<int, 1>());
+  auto stage_loop = [&](auto lower_c) {
+    if constexpr ((C::kSteps & 1) == 0) {
+      for (int s = 0; s < nstage; ++s) run_stage(s, integral_c<int, 0>(), lower_c);
+    } else {
+      // the operand sets alternate between stages: two stages per iteration (a
+      // block with an odd number of stages runs one more, zero-filled stage)
+      for (int s = 0; s < nstage; s += 2) {
+        run_stage(s, integral_c<int, 0>(), lower_c);
+        run_stage(s + 1, integral_c<int, 1>(), lower_c);
+      }
     }
+  };
+  if constexpr (E::kDiagLower) {
+    if (diag && wr == wc)
+      stage_loop(yes());
+    else
+      stage_loop(no());
+  } else {
+    stage_loop(no());
   }
   // DMAs of stages past the block (zero-filled) may still be in flight: drain
   // them before the next tile reuses the buffers
