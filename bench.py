@@ -359,6 +359,104 @@ def profile_only(X, which, steps, warmup, kernel=0, stage=0, staging=0):
     buf.free()
 
 
+def smi_json(cmd):
+    """Parsed JSON output of an SMI command, or None."""
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=30)
+        return json.loads(r.stdout)
+    except Exception:
+        return None
+
+
+def power_sample():
+    """Socket power and the shader clock of each XCD now (amd-smi), or None."""
+    def gpu0(d):
+        return d["gpu_data"][0] if isinstance(d, dict) else d[0]
+    try:
+        g = gpu0(smi_json(["amd-smi", "metric", "-p", "-c", "--json"]))
+        if "power" not in g or "clock" not in g:  # older amd-smi: one metric per call
+            g = dict(gpu0(smi_json(["amd-smi", "metric", "-p", "--json"])),
+                     **gpu0(smi_json(["amd-smi", "metric", "-c", "--json"])))
+        clk = [g["clock"][k]["clk"]["value"] for k in sorted(g["clock"]) if k.startswith("gfx_")]
+        clk = [c for c in clk if isinstance(c, (int, float))]
+        maxc = [g["clock"][k]["max_clk"]["value"] for k in sorted(g["clock"]) if k.startswith("gfx_")]
+        return {"socket_w": g["power"]["socket_power"]["value"], "gfx_clk_mhz": clk,
+                "gfx_clk_mean_mhz": round(sum(clk) / len(clk), 1) if clk else None,
+                "max_gfx_clk_mhz": max(c for c in maxc if isinstance(c, (int, float)))}
+    except Exception:
+        return None
+
+
+def rocm_smi_sclk():
+    """The shader clock rocm-smi reports now, in MHz, or None."""
+    d = smi_json(["rocm-smi", "-c", "--json"])
+    try:
+        return int(re.search(r"(\d+)", next(iter(d.values()))["sclk clock speed:"]).group(1))
+    except Exception:
+        return None
+
+
+def power_under_load(X, eng, buf, ns, nf, nt, cmac):
+    """Socket power and shader clocks 1.6 s and 2.4 s into ~3 s of back-to-back
+    launches of the headline kernel on its uniform input `buf` and on constant
+    input (every byte 3), and of the MFMA-only stream on uniform and on
+    constant operands (the SMU's readings lag by about a second).
+    The chip's power management lowers the clock until the socket power is
+    under its cap.  Each rate is also given as a fraction of the dense int8
+    peak scaled to the mean clock measured during it."""
+    import threading
+    cap = smi_json(["rocm-smi", "-M", "--json"])
+    try:
+        cap = float(next(iter(cap.values()))["Max Graphics Package Power (W)"])
+    except Exception:
+        cap = None
+    const = X.DeviceBuffer(nt * nf * ns * 4)
+    const.fill_uniform_int8(2, 3, 3)
+    ops = X.DeviceBuffer(X.PEAK_OPERAND_BYTES)
+    ops.fill_uniform_int8(5, -128, 127)
+    phases = (
+        ("headline_kernel_uniform_input", lambda: eng.time_correlate(buf, nt, warmup=0, reps=3000)),
+        ("headline_kernel_constant_input", lambda: eng.time_correlate(const, nt, warmup=0, reps=4000)),
+        ("mfma_only_uniform_operands", lambda: X.mfma_peak_data(ops, reps=2300) / 1e12),
+        ("mfma_only_constant_operands", lambda: X.mfma_peak(reps=1700) / 1e12),
+    )
+    out = {"what": "socket power (amd-smi, averaged by the SMU) and shader clocks (amd-smi per "
+                   "XCD; rocm-smi sclk) sampled 1.6 s and 2.4 s into ~3 s of back-to-back "
+                   "launches of each; the later sample is reported", "power_cap_w": cap}
+    for key, work in phases:
+        res = {}
+        th = threading.Thread(target=lambda: res.__setitem__("r", work()))
+        t0 = time.perf_counter()
+        th.start()
+        smps = []
+        for at in (1.6, 2.4):
+            time.sleep(max(0.0, at - (time.perf_counter() - t0)))
+            smp = power_sample() or {"error": "no amd-smi reading"}
+            smp["sclk_mhz_rocm_smi"] = rocm_smi_sclk()
+            smp["at_s"] = round(time.perf_counter() - t0, 2)
+            smp["sampled_while_running"] = th.is_alive()
+            smps.append(smp)
+        th.join()
+        smp = dict(smps[-1], earlier_sample={k: smps[0].get(k) for k in
+                                             ("socket_w", "gfx_clk_mean_mhz", "at_s")})
+        r = res.get("r")
+        if r is not None:
+            tcmac = cmac / (r * 1e-3) / 1e12 if key.startswith("headline") else r
+            if key.startswith("headline"):
+                smp["kernel_ms"] = round(r, 4)
+            smp["tcmac_s"] = round(tcmac, 1)
+            smp["pct_int8_mfma_peak"] = round(100.0 * tcmac * 8 / PEAK_INT8_TOPS, 1)
+            clk, top = smp.get("gfx_clk_mean_mhz"), smp.get("max_gfx_clk_mhz")
+            if clk and top:
+                smp["pct_of_peak_at_measured_clock"] = round(
+                    100.0 * tcmac * 8 / (PEAK_INT8_TOPS * clk / top), 1)
+        out[key] = smp
+        log(f"[bench] power, {key}: {smp}")
+    const.free()
+    ops.free()
+    return out
+
+
 FFT_ENTRIES = 8 * 1024 * 1024  # complex values per FFT workload (the reference harness's n_entries)
 
 
@@ -418,6 +516,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-power", action="store_true",
+                    help="skip the socket power and clock readings under load")
     ap.add_argument("--only", choices=["int8", "fp32", "fp16", "copy", "fft", "dump", "cfg3"],
                     default=None,
                     help="run just this path's workload (for profiling); prints no JSON line")
@@ -525,6 +625,9 @@ def main():
                      "pct_int8_mfma_peak": round(100.0 * cmac * 8 / (ms_ref * 1e-3) / 1e12
                                                  / PEAK_INT8_TOPS, 2)}
         log(f"[bench] int8 {kname} on xgpuRandomComplex input: {ms_ref:.3f} ms kernel")
+    # socket power and clocks under this load (the headline kernel is power-bound)
+    eng.set_tuning(kernel=args.kernel, stage=args.stage, staging=args.staging)
+    power = None if args.no_power else power_under_load(X, eng, buf, NS, NF, NT, cmac)
     eng.close()
 
     # FP32 plain kernel on the same workload
@@ -728,6 +831,7 @@ def main():
                                                       / ceil_uniform, 1),
         "pct_of_mfma_only_probe": round(100.0 * cmac / (ms_med * 1e-3) / 1e12 / peak_meas, 1),
         "clocks": clocks,
+        "power": power,
         "fp32_plain": {"tcmac_s": round(t32, 3), "ms": round(ms32, 3),
                        "pct_fp32_peak": round(100 * t32 * 8 / PEAK_FP32_TFLOPS, 1)},
         "int8_over_fp32_plain": round(value / t32, 2),
