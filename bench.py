@@ -291,7 +291,7 @@ def f16_kernel_name(tun):
     if tun["kernel"] == 1:
         return "xg_xengine_mfma<true, false>"
     if tun["staging"] == 2:
-        return "xg_xengine_rgdh<32, 4>"
+        return "xg_xengine_rgdh<32, 4>" if tun["stage"] == 32 else "xg_xengine_rgdh<16, 8>"
     return f"xg_xengine_rgh<{tun['stage']}>"
 
 

@@ -215,7 +215,8 @@ typedef struct XGPUAmdTuningStruct {
                       (pair tiles of 128 x 256 inputs) where the real-Gram one applies */
   int stage;       /* real-Gram kernel: samples per LDS stage: int8 64 or 128 with
                       register staging, 32 or 64 with DMA staging; fp16 32 or 64 with
-                      register staging, 32 with DMA staging (0 = default) */
+                      register staging, 16 or 32 with DMA staging (0 = default: int8
+                      DMA 32, fp16 DMA 16) */
   int staging;     /* real-Gram kernel: 1 = global -> VGPR -> LDS, 2 = global -> LDS DMA
                       (0 = default: DMA for int8 and fp16) */
   int ntimepipe;   /* host-buffer path: samples per host -> device transfer and kernel

@@ -14,11 +14,11 @@ import numpy as np
 
 # (kernel, staging, stage, mfma) of XGPUAmdTuning: the complex-plane kernel;
 # the real-Gram kernel with register staging (int8 64 / 128, fp16 32 / 64
-# samples per stage) and with global -> LDS DMA (int8 32 / 64, fp16 32); the
+# samples per stage) and with global -> LDS DMA (int8 32 / 64, fp16 16 / 32); the
 # int8 DMA kernel with 16x16x64 MFMA tiles; the K-interleaved pair-tile kernel
 I8_TUNINGS = [(1, 0, 0, 0), (2, 1, 64, 0), (2, 1, 128, 0), (2, 2, 32, 0), (2, 2, 64, 0),
               (2, 2, 64, 2), (3, 0, 0, 0)]
-F16_TUNINGS = [(1, 0, 0, 0), (2, 1, 32, 0), (2, 1, 64, 0), (2, 2, 32, 0)]
+F16_TUNINGS = [(1, 0, 0, 0), (2, 1, 32, 0), (2, 1, 64, 0), (2, 2, 16, 0), (2, 2, 32, 0)]
 ORDER_NAMES = ("regtile", "tri", "realimag")
 
 
@@ -43,7 +43,7 @@ def _cases(kind, n, seed):
 
 
 I8_SWEEP = _cases("i8", 35, 20261)
-F16_SWEEP = _cases("f16", 16, 20262)
+F16_SWEEP = _cases("f16", 20, 20262)
 
 
 def call_lengths(nt, calls):

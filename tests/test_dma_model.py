@@ -232,8 +232,8 @@ def test_rg_loads_in_bounds(ns, nf, nt, stg):
 @pytest.mark.parametrize("ns,nf,nt", [(96, 2, 257), (512, 4095, 64), (130, 1, 64), (256, 4, 1000)])
 def test_fp16_loads_in_bounds(ns, nf, nt):
     """The FP16 real-Gram kernels: register-staged rgh<32>, rgh<64> and the DMA
-    kernel rgdh<32,4>, up to the largest FP16 block they take (NF=4095 at
-    NSTATION=512: tstride 16.8 MB)."""
+    kernels rgdh<16,8> and rgdh<32,4>, up to the largest FP16 block they take
+    (NF=4095 at NSTATION=512: tstride 16.8 MB)."""
     assert 128 * nf * ns * 8 < 2 ** 31 - 4096
     nblk = (2 * ns + TILE - 1) // TILE
     for f in sorted({0, nf - 1}):
@@ -242,6 +242,7 @@ def test_fp16_loads_in_bounds(ns, nf, nt):
                 for stg in (32, 64):
                     model_rg_loads(ns, nf, nt, f, rb, cb, stg, fmt="f16")
                 model_dma(ns, nf, nt, f, rb, cb, 32, 4, fmt="f16")
+                model_dma(ns, nf, nt, f, rb, cb, 16, 8, fmt="f16")
 
 
 @pytest.mark.parametrize("ns,nf,nt", [(96, 2, 257), (512, 8191, 64), (132, 2, 64), (256, 4, 1000)])
@@ -490,8 +491,8 @@ def sweep_kernel(case, fmt, call, per):
         return ("kg",)
     if t["staging"] == 1:
         return ("rg", t["stage"])
-    if fmt == "f16":
-        return ("dma", 32, 4, model_epilogue)
+    if fmt == "f16":  # rgdh<16,8> (stage 0 or 16) or rgdh<32,4>
+        return ("dma", 32, 4, model_epilogue) if t["stage"] == 32 else ("dma", 16, 8, model_epilogue)
     if t["stage"] == 32:
         return ("dma", 32, 8, model_epilogue)
     return ("dma", 64, 4, model_epilogue16 if t["mfma"] == 2 else model_epilogue)

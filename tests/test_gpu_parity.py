@@ -337,8 +337,9 @@ def test_f32_cfg1_through_xgpu_h():
 
 # ------------------------------------------------------------------ FP16 MFMA
 # (kernel, staging, stage): complex-plane kernel; real-Gram kernel with
-# register staging (32 / 64 samples per LDS stage) and with global->LDS DMA (32)
-F16_VARIANTS = [(1, 0, 0), (2, 1, 32), (2, 1, 64), (2, 2, 32)]
+# register staging (32 / 64 samples per LDS stage) and with global->LDS DMA
+# (16, the default, / 32)
+F16_VARIANTS = [(1, 0, 0), (2, 1, 32), (2, 1, 64), (2, 2, 16), (2, 2, 32)]
 
 
 def f16_input(nt, nf, ns, seed, sigma=16.0):
@@ -424,6 +425,7 @@ def test_f16_variants_agree_cfg2_shape():
                                                     kernel=kernel, staging=staging, stage=stage)
     assert np.array_equal(outs[(2, 1, 32)], outs[(2, 1, 64)])
     assert np.array_equal(outs[(2, 1, 32)], outs[(2, 2, 32)])
+    assert np.array_equal(outs[(2, 1, 32)], outs[(2, 2, 16)])
     chans = [0, 63]
     ref = oracle.xengine_f64_f16(np.ascontiguousarray(x16[:, chans]), ns, len(chans), nt)
     for v, got in outs.items():
@@ -757,6 +759,29 @@ def test_autotune_keeps_results_exact(tmp_path):
     buf.free()
 
 
+def test_autotune_fp16_candidates():
+    """The FP16 autotuner's candidates (complex-plane kernel; real-Gram kernel
+    with register staging at 32 / 64 and DMA staging at 16 / 32 samples per
+    stage) all run; the winner is kept and its result is within the fp32
+    accumulation bound of the FP64 oracle."""
+    ns, nf, nt = 128, 4, 512
+    x16 = f16_input(nt, nf, ns, 23)
+    ref = oracle.xengine_f64_f16(x16, ns, nf, nt)
+    buf = X.DeviceBuffer.from_array(x16)
+    eng = X.Engine(ns, nf, compute=X.XGPU_FLOAT16, order=X.TRIANGULAR_ORDER)
+    flush("[autotune] fp16 ns=128 nf=4 nt=512")
+    assert eng.get_tuning()["staging"] == 2 and eng.get_tuning()["stage"] == 16
+    best, ms = eng.autotune(buf, nt, warmup=1, reps=2)
+    assert ms > 0 and best["kernel"] in (1, 2)
+    if best["kernel"] == 2:
+        assert (best["staging"], best["stage"]) in ((1, 32), (1, 64), (2, 16), (2, 32))
+    eng.clear()
+    eng.correlate(buf, nt)
+    assert_f16_close(eng.dump(), ref, ns, nf, nt)
+    eng.close()
+    buf.free()
+
+
 def test_autotune_host_path_time_pipe(tmp_path):
     """Host-path tuner over the time-pipe length x graph replay: the winner
     divides NTIME, every setting gives the same exact integration, and the
@@ -1001,9 +1026,9 @@ def test_i8_largest_station_count():
 def test_f16_largest_blocks_at_the_offset_limit():
     """FP16 at NSTATION=512 x 2 pol, NFREQUENCY=4095, NTIME=64: the largest
     block the FP16 real-Gram kernels take (tstride 16.8 MB; 17 GB of float
-    output), for rgh<32> (default), rgh<64> and the DMA kernel rgdh<32,4>:
-    the first, middle and last channel against the FP64 oracle within the
-    fp32 accumulation bound."""
+    output), for the DMA kernels rgdh<16,8> (default) and rgdh<32,4> and the
+    register-staged rgh<32> and rgh<64>: the first, middle and last channel
+    against the FP64 oracle within the fp32 accumulation bound."""
     ns, nf, nt = 512, 4095, 64
     assert 128 * nf * ns * 8 < (1 << 31) - 4096
     flush(f"[f16] offset limit NS={ns} NF={nf} NT={nt}")
@@ -1023,7 +1048,7 @@ def test_f16_largest_blocks_at_the_offset_limit():
     def close(got, r, f):
         assert_f16_close(got, r, ns, 1, nt)
 
-    for kernel, staging, stage in [(0, 0, 0), (2, 1, 64), (2, 2, 32)]:
+    for kernel, staging, stage in [(0, 0, 0), (2, 2, 32), (2, 1, 32), (2, 1, 64)]:
         flush(f"[f16] offset limit kernel={kernel} staging={staging} stage={stage}")
         eng.set_tuning(kernel=kernel, staging=staging, stage=stage)
         eng.clear()

@@ -177,7 +177,7 @@ class Engine:
         K-interleaved pair-tile kernel where the real-Gram one applies; staging
         (real-Gram): 1 registers, 2 global->LDS DMA; stage: samples per LDS
         stage (int8: 64 / 128 with register staging, 32 / 64 with DMA; fp16:
-        32 / 64 with register staging, 32 with DMA); ntimepipe: host-path
+        32 / 64 with register staging, 16 / 32 with DMA); ntimepipe: host-path
         samples per transfer and launch; mfma (int8 DMA kernel): 1 32x32x32,
         2 16x16x64 tiles (0 = default everywhere)."""
         t = _lib.XGPUAmdTuning()

@@ -140,9 +140,9 @@ int xgpuAmdAutotune(XGPUAmdEngine *e, const void *input_d, unsigned ntime, int w
     }
     /* staging, stage */
     static const int v_i8[4][2] = {{1, 64}, {1, 128}, {2, 32}, {2, 64}};
-    static const int v_f16[3][2] = {{1, 32}, {1, 64}, {2, 32}};
+    static const int v_f16[4][2] = {{1, 32}, {1, 64}, {2, 16}, {2, 32}};
     const int fam = xg_launch_family(x, input_d);
-    const int nv = fam == XG_KERNEL_I8_RG ? 4 : fam == XG_KERNEL_F16_RG ? 3 : 0;
+    const int nv = fam == XG_KERNEL_I8_RG ? 4 : fam == XG_KERNEL_F16_RG ? 4 : 0;
     for (int v = 0; v < nv; ++v) {
       XGPUAmdTuning t = keep;
       t.kernel = 2;

@@ -95,8 +95,8 @@ static const char *kernel_label(int family, const XgLaunch *L) {
            : L->stage == 5 ? "xg_xengine_rgd16<4>" : L->stage == 1 ? "xg_xengine_rg<128>"
            : "xg_xengine_rg<64>";
     case XG_KERNEL_F16_RG:
-      return L->stage == 2 ? "xg_xengine_rgdh<32,4>" : L->stage == 1 ? "xg_xengine_rgh<64>"
-           : "xg_xengine_rgh<32>";
+      return L->stage == 3 ? "xg_xengine_rgdh<16,8>" : L->stage == 2 ? "xg_xengine_rgdh<32,4>"
+           : L->stage == 1 ? "xg_xengine_rgh<64>" : "xg_xengine_rgh<32>";
     default: return "unknown";
   }
 }
@@ -270,8 +270,8 @@ int xgpuAmdCreate(XGPUAmdEngine **out, const XGPUAmdConfig *cfg) {
   e->prefetch = XG_DEFAULT_PREFETCH;
   e->kernel_pref = 2;
   if (e->family == XG_KERNEL_F16_MFMA) {
-    e->stage = XG_DEFAULT_RGH_STAGE;
     e->staging = XG_DEFAULT_RGH_STAGING;
+    e->stage = e->staging == 2 ? XG_DEFAULT_RGDH_STAGE : XG_DEFAULT_RGH_STAGE;
   } else {
     e->stage = XG_DEFAULT_RG_STAGE;
     e->staging = XG_DEFAULT_RG_STAGING;
@@ -354,13 +354,13 @@ int xgpuAmdSetTuning(XGPUAmdEngine *e, const XGPUAmdTuning *t) {
   int staging, stage;
   if (e->family == XG_KERNEL_F16_MFMA) {
     /* fp16 real-Gram kernel: register staging (32 or 64 samples per stage)
-     * or global -> LDS DMA (32) */
+     * or global -> LDS DMA (16 or 32) */
     staging = t->staging ? t->staging : XG_DEFAULT_RGH_STAGING;
     if (staging == 1 && t->stage != 0 && t->stage != 32 && t->stage != 64)
       return xg_fail(XGPU_INVALID_CONFIG, "fp16 stage must be 0, 32 or 64 with register staging");
-    if (staging == 2 && t->stage != 0 && t->stage != 32)
-      return xg_fail(XGPU_INVALID_CONFIG, "fp16 stage must be 0 or 32 with DMA staging");
-    stage = t->stage ? t->stage : (staging == 2 ? 32 : XG_DEFAULT_RGH_STAGE);
+    if (staging == 2 && t->stage != 0 && t->stage != 16 && t->stage != 32)
+      return xg_fail(XGPU_INVALID_CONFIG, "fp16 stage must be 0, 16 or 32 with DMA staging");
+    stage = t->stage ? t->stage : (staging == 2 ? XG_DEFAULT_RGDH_STAGE : XG_DEFAULT_RGH_STAGE);
   } else {
     staging = t->staging ? t->staging : XG_DEFAULT_RG_STAGING;
     if (t->stage != 0 && !(staging == 1 && (t->stage == 64 || t->stage == 128)) &&
@@ -441,7 +441,8 @@ int xg_launch_family(const XGPUAmdEngine *e, const void *input_d) {
 
 /* XgLaunch.stage code of the real-Gram kernel variant */
 static char rg_stage_code(const XGPUAmdEngine *e) {
-  if (e->family == XG_KERNEL_F16_MFMA) return (char)(e->staging == 2 ? 2 : e->stage == 64 ? 1 : 0);
+  if (e->family == XG_KERNEL_F16_MFMA)
+    return (char)(e->staging == 2 ? (e->stage == 16 ? 3 : 2) : e->stage == 64 ? 1 : 0);
   if (e->staging == 2) return (char)(e->stage == 32 ? 4 : e->mfma == 2 ? 5 : 3);
   return (char)(e->stage == 128 ? 1 : 0);
 }

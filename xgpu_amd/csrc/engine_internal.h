@@ -82,11 +82,14 @@ struct XGPUAmdEngineStruct {
 /* default staging and samples per LDS stage of the int8 real-Gram kernel */
 #define XG_DEFAULT_RG_STAGING 2
 #define XG_DEFAULT_RG_STAGE 32
-/* samples per LDS stage of the fp16 real-Gram kernel (register staging) */
+/* samples per LDS stage of the fp16 real-Gram kernel with register staging */
 #define XG_DEFAULT_RGH_STAGE 32
 /* fp16 real-Gram kernel: global -> LDS DMA staging by default (bit-identical to
  * register staging, 0-8 % faster at NS=256 x 2 pol, NF=128, NT=2048) */
 #define XG_DEFAULT_RGH_STAGING 2
+/* samples per LDS stage of the fp16 DMA kernel: 16 (8 stage buffers; 2-3 %
+ * faster than 32 with 4 at NS=256 x 2 pol, NF=128, NT=2048, bit-identical) */
+#define XG_DEFAULT_RGDH_STAGE 16
 
 int xg_fail(int code, const char *fmt, ...);
 void xg_plan_launch(XgLaunch *L, int cus, int wg_per_cu);

@@ -2398,7 +2398,9 @@ extern "C" int xgh_launch_xengine(int family, const XgLaunch *L, void *stream) {
       hipLaunchKernelGGL(xg_xengine_kg, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       break;
     case XG_KERNEL_F16_RG:
-      if (L->stage == 2)
+      if (L->stage == 3)
+        hipLaunchKernelGGL((xg_xengine_rgdh<16, 8>), dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
+      else if (L->stage == 2)
         hipLaunchKernelGGL((xg_xengine_rgdh<32, 4>), dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
       else if (L->stage == 1)
         hipLaunchKernelGGL(xg_xengine_rgh<64>, dim3(grid), dim3(XG_WG_THREADS), 0, s, *L);
