@@ -619,9 +619,9 @@ struct RgInt8 {
   static constexpr int kChunks = 16;   // 16-byte chunks per row
   static constexpr int kInputs = 8;    // inputs per chunk
   static constexpr int kK = 32;        // samples per MFMA
-  // the DMA kernel's diagonal waves issue all 16 MFMA blocks (skipping the 6
-  // above the diagonal measured no faster for int8, and spilled)
-  static constexpr bool kDiagLower = false;
+  // the DMA kernel's diagonal waves issue only the 10 MFMA blocks on or below
+  // the diagonal (1.7-2 % faster at NS=512, NF=64, NT=8192)
+  static constexpr bool kDiagLower = true;
   __device__ static int slot(int chunk, int srow) { return rg_slot(chunk, srow & 7); }
   // the chunk that belongs in `slot` of sample row `srow` (inverse of slot)
   __device__ static int dma_chunk(int slot, int srow) {
@@ -1232,11 +1232,7 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
   };
 
   const int wr = wave >> 1, wc = wave & 1;
-  int aoff[4], boff[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) aoff[a] = E::lane_off(4 * wr + a, lane);
-#pragma unroll
-  for (int b = 0; b < 4; ++b) boff[b] = E::lane_off(4 * wc + b, lane) + (diag ? 0 : C::kSide);
+  int aoff[4], boff[4];  // operand read offsets, set after the prologue barrier
   typename E::Acc acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -1322,6 +1318,18 @@ __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsig
     }
     rgd_barrier<0>();
     return;  // nothing to store (rg_finish skips this wave on a diagonal tile)
+  }
+  {
+    // The operand offsets are computed here, from an opaque copy of the lane,
+    // not hoisted out of the tile loop: kept live across it they were spilled,
+    // and the reload's wait (vmcnt(0)) then waited for every prologue DMA
+    // instead of stage 0's.
+    int lane_o = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane_o));
+#pragma unroll
+    for (int a = 0; a < 4; ++a) aoff[a] = E::lane_off(4 * wr + a, lane_o);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) boff[b] = E::lane_off(4 * wc + b, lane_o) + (diag ? 0 : C::kSide);
   }
   lread(lds, A0, B0);
   // One stage whose kstep-0 operands are in operand set P.  The last kstep
