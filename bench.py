@@ -359,6 +359,51 @@ def profile_only(X, which, steps, warmup, kernel=0, stage=0, staging=0):
     buf.free()
 
 
+def converters_bench(X, ns=512, nf=64, nt=8192, reps=5):
+    """The device converters on the headline shape, wall clock around each
+    call (the library synchronises the device after it), median of `reps`
+    after one warm-up: the input corner turn of xgpuSwizzleInput (1 GiB read,
+    1 GiB written) and its inverse, and xgpuExtractMatrix of the int32
+    triangular integration into full Hermitian matrices (the packed values
+    read, NF (2 NS)^2 complex values written)."""
+    def timed(fn):
+        fn()
+        dts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            dts.append(time.perf_counter() - t0)
+        return sorted(dts)[reps // 2] * 1e3
+
+    def rate(ms, nbytes):
+        return {"ms": round(ms, 4), "bytes": nbytes,
+                "gb_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                "pct_hbm_8tb_s": round(100 * nbytes / (ms * 1e-3) / 8e12, 1)}
+
+    out = {"workload": f"NSTATION={ns} x 2 pol, NFREQUENCY={nf}, NTIME={nt}, buffers in HBM",
+           "what": "wall clock around the call, median of 5"}
+    nin = nt * nf * ns * 4
+    src = X.DeviceBuffer(nin)
+    src.fill_uniform_int8(7, -128, 127)
+    dst = X.DeviceBuffer(nin)
+    out["swizzle"] = rate(timed(lambda: X.swizzle_input(dst, src, ns, nf, nt)), 2 * nin)
+    out["unswizzle"] = rate(timed(lambda: X.swizzle_input(src, dst, ns, nf, nt, unswizzle=True)),
+                            2 * nin)
+    src.free()
+    dst.free()
+    tri = X.info(ns, nf, nt).triLength
+    packed = X.DeviceBuffer(2 * tri * 4)
+    packed.fill_uniform_int8(8, -128, 127)
+    full_bytes = nf * (2 * ns) ** 2 * 8
+    full = X.DeviceBuffer(full_bytes)
+    out["extract_int32"] = rate(timed(lambda: X.extract_matrix(full, packed, ns, nf, X.ELEM_INT32)),
+                                2 * tri * 4 + full_bytes)
+    packed.free()
+    full.free()
+    log(f"[bench] converters: {out}")
+    return out
+
+
 def smi_json(cmd):
     """Parsed JSON output of an SMI command, or None."""
     try:
@@ -709,6 +754,11 @@ def main():
     except Exception as exc:
         cfg3 = {"error": str(exc)}
 
+    try:
+        converters = converters_bench(X)
+    except Exception as exc:
+        converters = {"error": str(exc)}
+
     # FP16 MFMA path on its own workload (NS=256, NF=128, NT=2048, fp16 input
     # from the workload generator: complex Gaussian sigma 16 plus a delayed tone)
     import numpy as np
@@ -861,6 +911,7 @@ def main():
         "int8_kernel_variants": variants,
         "int8_on_reference_input_distribution": ref_input,
         "cfg3": cfg3,
+        "converters": converters,
         "host_path": host,
         "cpu_baseline": cpu,
         "p99_dump_latency_ms": p99_summary(serving),

@@ -142,13 +142,15 @@ def test_swizzled_input_engine_equals_plain():
     buf.free()
 
 
+@pytest.mark.parametrize("ns,nf", [(24, 3), (1, 1), (16, 2), (37, 2), (130, 1)])
 @pytest.mark.parametrize("kind,dtype", [(X.ELEM_INT32, np.int32), (X.ELEM_INT64, np.int64),
                                         (X.ELEM_FLOAT32, np.float32),
                                         (X.ELEM_FLOAT64, np.float64)])
-def test_device_extract_matches_oracle(kind, dtype):
+def test_device_extract_matches_oracle(kind, dtype, ns, nf):
     """Device xgpuExtractMatrix against the oracle's (pinned to the
-    reference's golden vectors), on values every element type holds exactly."""
-    ns, nf = 24, 3
+    reference's golden vectors), on values every element type holds exactly:
+    tiles of 16 x 16 stations, whole (16), partial (24, 37, 130) and a single
+    station."""
     n = 2 * oracle.tri_len(ns, nf)
     vals = oracle.fill_uniform(n, 420 + kind, -128, 127).astype(np.int64) * 4099
     packed = vals.astype(dtype)

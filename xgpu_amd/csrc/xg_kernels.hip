@@ -2298,39 +2298,77 @@ __global__ void xg_swizzle_k(unsigned char *dst, const unsigned char *src, long 
 // polarization pair by the reference loop, and its last writes leave
 // [a][b] = packed[a][b] for a > b and conj(packed[b][a]) for a <= b; the
 // same values are produced here (for Hermitian input both readings agree).
-// One thread per output cell (i, j): 8 values written contiguously, read from
-// one packed cell.  Negation as the reference does it on its element type
-// (integer negation wraps for the most negative value; floating point flips
-// the sign).
+// Negation as the reference does it on its element type (integer negation
+// wraps for the most negative value; floating point flips the sign).
+//
+// One workgroup per tile of 16 x 16 stations on or below
+// the diagonal (row tile a >= column tile b) of one channel: thread (il, jl)
+// reads packed cell (i, j) = (16 a + il, 16 b + jl), i >= j, once (the 16
+// cells of a row are contiguous), writes out[f][i][j] (the rule above for a
+// diagonal cell), and keeps the cell in LDS; after the barrier thread (il, jl)
+// writes the mirror cell out[f][16 b + il][16 a + jl] (row < column) from the
+// packed cell thread (jl, il) read.  Both writes store 16 contiguous cells per
+// row.  Rows of the LDS tile are padded by one element, so the transposed
+// reads of a wave fall on different banks.
+#define XG_XT 16
 template <typename T>
-__global__ void xg_extract_k(T *out, const T *packed, int ns, long long ncell) {
-  const long long stride = (long long)gridDim.x * blockDim.x;
+__global__ __launch_bounds__(256) void xg_extract_tile_k(T *out, const T *packed, int ns, int nf) {
+  __shared__ T cell[XG_XT][XG_XT * 8 + 1];
+  const int nt = (ns + XG_XT - 1) / XG_XT;
+  const long long npair = (long long)nt * (nt + 1) / 2;
   const long long nb = (long long)ns * (ns + 1) / 2;
-  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += stride) {
-    const long long f = c / ((long long)ns * ns);
-    const long long r = c - f * ns * ns;
-    const int i = (int)(r / ns), j = (int)(r - (long long)i * ns);
-    const bool low = i >= j;
-    const int a = low ? i : j, b = low ? j : i;
-    const T *src = packed + (f * nb + (long long)a * (a + 1) / 2 + b) * 8;
-    T v[8];
+  const int il = threadIdx.x >> 4, jl = threadIdx.x & 15;
+  for (long long w = blockIdx.x; w < npair * nf; w += gridDim.x) {
+    const int f = (int)(w / npair);
+    const long long p = w - (long long)f * npair;
+    // pair p -> (a, b), b <= a: p = a (a + 1) / 2 + b
+    int a = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while ((long long)a * (a + 1) / 2 > p) --a;
+    while ((long long)(a + 1) * (a + 2) / 2 <= p) ++a;
+    const int b = (int)(p - (long long)a * (a + 1) / 2);
+    const int i = a * XG_XT + il, j = b * XG_XT + jl;
+    if (i < ns && j <= i) {
+      const T *src = packed + ((long long)f * nb + (long long)i * (i + 1) / 2 + j) * 8;
+      T v[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = src[q];
-    T o[8];
+      for (int q = 0; q < 8; ++q) v[q] = src[q];
+      T o[8];
 #pragma unroll
-    for (int p1 = 0; p1 < 2; ++p1)
+      for (int p1 = 0; p1 < 2; ++p1)
 #pragma unroll
-      for (int p2 = 0; p2 < 2; ++p2) {
-        // straight copy below the diagonal (and in a diagonal cell below its
-        // own diagonal), conjugate of the transposed entry elsewhere
-        const bool direct = (i > j) || (i == j && p1 > p2);
-        const int d = (p1 * 2 + p2) * 2, s2 = direct ? d : (p2 * 2 + p1) * 2;
-        o[d] = v[s2];
-        o[d + 1] = direct ? v[s2 + 1] : (T)(T(0) - v[s2 + 1]);
-      }
-    T *dst = out + c * 8;
+        for (int p2 = 0; p2 < 2; ++p2) {
+          const bool direct = (i > j) || (p1 > p2);
+          const int d = (p1 * 2 + p2) * 2, s2 = direct ? d : (p2 * 2 + p1) * 2;
+          o[d] = v[s2];
+          o[d + 1] = direct ? v[s2 + 1] : (T)(T(0) - v[s2 + 1]);
+        }
+      T *dst = out + (((long long)f * ns + i) * ns + j) * 8;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) dst[q] = o[q];
+      for (int q = 0; q < 8; ++q) dst[q] = o[q];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) cell[il][jl * 8 + q] = v[q];
+    }
+    __syncthreads();
+    // mirror cell (row jm < column im) from packed cell (im, jm), read by thread (jl, il)
+    const int jm = b * XG_XT + il, im = a * XG_XT + jl;
+    if (im < ns && jm < im) {
+      T v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = cell[jl][il * 8 + q];
+      T o[8];
+#pragma unroll
+      for (int p1 = 0; p1 < 2; ++p1)
+#pragma unroll
+        for (int p2 = 0; p2 < 2; ++p2) {
+          const int d = (p1 * 2 + p2) * 2, s2 = (p2 * 2 + p1) * 2;
+          o[d] = v[s2];
+          o[d + 1] = (T)(T(0) - v[s2 + 1]);
+        }
+      T *dst = out + (((long long)f * ns + jm) * ns + im) * 8;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) dst[q] = o[q];
+    }
+    __syncthreads();  // the tile's LDS is reused by the next pair
   }
 }
 
@@ -2593,25 +2631,27 @@ extern "C" int xgh_launch_swizzle(int unswizzle, void *dst, const void *src, int
 
 extern "C" int xgh_launch_extract(int kind, void *dst, const void *packed, int nstation, int nfreq,
                                   void *stream) {
-  const long long ncell = (long long)nfreq * nstation * nstation;
-  if (ncell == 0) return 0;
-  const dim3 g(grid_for((size_t)ncell, 256)), b(256);
+  const long long nt = (nstation + XG_XT - 1) / XG_XT;
+  const long long nwork = nt * (nt + 1) / 2 * nfreq;  // tiles on or below the diagonal
+  if (nwork == 0) return 0;
+  const dim3 g((unsigned)(nwork < 65536 ? nwork : 65536)), b(256);
   hipStream_t s = (hipStream_t)stream;
   switch (kind) {
     case XG_OUT_I32:
-      hipLaunchKernelGGL(xg_extract_k<int>, g, b, 0, s, (int *)dst, (const int *)packed, nstation, ncell);
+      hipLaunchKernelGGL(xg_extract_tile_k<int>, g, b, 0, s, (int *)dst, (const int *)packed,
+                         nstation, nfreq);
       break;
     case XG_OUT_I64:
-      hipLaunchKernelGGL(xg_extract_k<long long>, g, b, 0, s, (long long *)dst,
-                         (const long long *)packed, nstation, ncell);
+      hipLaunchKernelGGL(xg_extract_tile_k<long long>, g, b, 0, s, (long long *)dst,
+                         (const long long *)packed, nstation, nfreq);
       break;
     case XG_OUT_F32:
-      hipLaunchKernelGGL(xg_extract_k<float>, g, b, 0, s, (float *)dst, (const float *)packed,
-                         nstation, ncell);
+      hipLaunchKernelGGL(xg_extract_tile_k<float>, g, b, 0, s, (float *)dst, (const float *)packed,
+                         nstation, nfreq);
       break;
     case XG_OUT_F64:
-      hipLaunchKernelGGL(xg_extract_k<double>, g, b, 0, s, (double *)dst, (const double *)packed,
-                         nstation, ncell);
+      hipLaunchKernelGGL(xg_extract_tile_k<double>, g, b, 0, s, (double *)dst,
+                         (const double *)packed, nstation, nfreq);
       break;
     default:
       return xgh_report(hipErrorInvalidValue, "xgh_launch_extract(kind)");
