@@ -2301,15 +2301,15 @@ __global__ void xg_swizzle_k(unsigned char *dst, const unsigned char *src, long 
 // Negation as the reference does it on its element type (integer negation
 // wraps for the most negative value; floating point flips the sign).
 //
-// One workgroup per tile of 16 x 16 stations on or below
-// the diagonal (row tile a >= column tile b) of one channel: thread (il, jl)
-// reads packed cell (i, j) = (16 a + il, 16 b + jl), i >= j, once (the 16
-// cells of a row are contiguous), writes out[f][i][j] (the rule above for a
-// diagonal cell), and keeps the cell in LDS; after the barrier thread (il, jl)
-// writes the mirror cell out[f][16 b + il][16 a + jl] (row < column) from the
-// packed cell thread (jl, il) read.  Both writes store 16 contiguous cells per
-// row.  Rows of the LDS tile are padded by one element, so the transposed
-// reads of a wave fall on different banks.
+// One workgroup per tile of 16 x 16 stations on or below the diagonal (row
+// tile a >= column tile b) of one channel: thread (il, jl) reads packed cell
+// (i, j) = (16 a + il, 16 b + jl), i >= j, once (the 16 cells of a row are
+// contiguous), writes out[f][i][j] (the rule above for a diagonal cell), and
+// keeps the cell in LDS; after the barrier thread (il, jl) writes the mirror
+// cell out[f][16 b + il][16 a + jl] (row < column) from the packed cell
+// thread (jl, il) read.  Both writes store 16 contiguous cells per row.  Rows
+// of the LDS tile are padded by one element, so the transposed reads of a
+// wave fall on different banks.
 #define XG_XT 16
 template <typename T>
 __global__ __launch_bounds__(256) void xg_extract_tile_k(T *out, const T *packed, int ns, int nf) {
