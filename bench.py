@@ -299,7 +299,38 @@ def profile_only(X, which, steps, warmup, kernel=0, stage=0, staging=0):
     """Run one path's workload warmup+steps times (one kernel per step).
     "copy": a plain 1 GiB device-to-device copy per step (counter calibration);
     "dump": one NS=2048, NF=16 integration, then one device triangular dump per step;
-    "cfg3": int8 integrations of NS=2048, NF=16, NT=4096."""
+    "cfg3": int8 integrations of NS=2048, NF=16, NT=4096;
+    "serve": the serving mix with blocks and dump rings in HBM, everything
+    submitted at once, warmup + steps passes (4 dumps per stream each)."""
+    if which == "serve":
+        from xgpu_amd import workloads as W
+        streams = W.serving_mix(5)
+        rng = __import__("numpy").random.default_rng(5)
+        rings = []
+        for ns, nf, nt, k in streams:
+            d = X.DeviceBuffer(k * nt * nf * ns * 4)
+            d.fill_uniform_int8(11, -127, 127)
+            rings.append(d)
+        for rep in range(warmup + steps):
+            srv = X.Server(staging_bytes=8 << 30)
+            opened = [srv.open_stream(ns, nf, k, dump_slots=1, dump_device=True)
+                      for ns, nf, nt, k in streams]
+            order = [i for i, (ns, nf, nt, k) in enumerate(streams) for _ in range(4 * k)]
+            count = [0] * len(streams)
+            t0 = time.perf_counter()
+            for j in rng.permutation(len(order)):
+                i = order[j]
+                ns, nf, nt, k = streams[i]
+                srv.submit_device(opened[i][0], rings[i].ptr + (count[i] % k) * nt * nf * ns * 4, nt)
+                count[i] += 1
+            srv.drain()
+            st = srv.stats()
+            log(f"[bench] serve pass {rep}: {time.perf_counter() - t0:.4f} s, "
+                f"{st['tcmac_per_s']:.1f} TCMAC/s, {st['launches']} launches")
+            srv.close()
+        for d in rings:
+            d.free()
+        return
     if which == "fft":
         import numpy as np
         buf = X.DeviceBuffer(FFT_ENTRIES * 8)
@@ -563,7 +594,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-power", action="store_true",
                     help="skip the socket power and clock readings under load")
-    ap.add_argument("--only", choices=["int8", "fp32", "fp16", "copy", "fft", "dump", "cfg3"],
+    ap.add_argument("--only", choices=["int8", "fp32", "fp16", "copy", "fft", "dump", "cfg3", "serve"],
                     default=None,
                     help="run just this path's workload (for profiling); prints no JSON line")
     ap.add_argument("--kernel", type=int, default=0, choices=[0, 1, 2, 3],

@@ -737,30 +737,45 @@ static int dump_stage(XGPUAmdEngine *e, int flags, void *direct, const void **ou
     *bytes = e->info.triLength * 2 * esz;
     return XGPU_OK;
   }
+  /* Without a reorder, a device destination takes the combined, widened or
+   * zeroed values straight from the kernel that makes them (no copy after). */
+  void *stage = (!tri && direct) ? direct : NULL;
   if (e->cleared && !e->spilled) {
-    if ((rc = ensure_scratch(e, &e->scratch, n * 8))) return rc;
-    HIPCALL(xgh_launch_fill_zero(e->scratch, n * esz, e->stream), "dump zero");
-    val = e->scratch;
+    if (!stage) {
+      if ((rc = ensure_scratch(e, &e->scratch, n * 8))) return rc;
+      stage = e->scratch;
+    }
+    HIPCALL(xgh_launch_fill_zero(stage, n * esz, e->stream), "dump zero");
+    val = stage;
   } else if (!wide && !e->spilled) {
     val = e->integ;
+    stage = NULL;
   } else {
-    if ((rc = ensure_scratch(e, &e->scratch, n * 8))) return rc;
+    if (!stage) {
+      if ((rc = ensure_scratch(e, &e->scratch, n * 8))) return rc;
+      stage = e->scratch;
+    }
     if (e->spilled && !e->cleared) {
-      HIPCALL(xgh_launch_combine(e->wide_kind, !wide, e->scratch, e->wide, e->integ, n, e->stream),
+      HIPCALL(xgh_launch_combine(e->wide_kind, !wide, stage, e->wide, e->integ, n, e->stream),
               "dump combine");
     } else if (e->spilled) { /* only the spilled part is live */
       if (wide) {
-        HIPCALL(xgh_memcpy_d2d_async(e->scratch, e->wide, n * 8, e->stream), "dump copy");
+        HIPCALL(xgh_memcpy_d2d_async(stage, e->wide, n * 8, e->stream), "dump copy");
       } else {
         if ((rc = ensure_scratch(e, &e->scratch2, n * 8))) return rc;
         HIPCALL(xgh_launch_fill_zero(e->scratch2, n * 4, e->stream), "dump zero");
-        HIPCALL(xgh_launch_combine(e->wide_kind, 1, e->scratch, e->wide, e->scratch2, n, e->stream),
+        HIPCALL(xgh_launch_combine(e->wide_kind, 1, stage, e->wide, e->scratch2, n, e->stream),
                 "dump combine");
       }
     } else { /* wide requested, nothing spilled: widen integ */
-      HIPCALL(xgh_launch_spill(e->wide_kind, 0, e->scratch, e->integ, n, e->stream), "dump widen");
+      HIPCALL(xgh_launch_spill(e->wide_kind, 0, stage, e->integ, n, e->stream), "dump widen");
     }
-    val = e->scratch;
+    val = stage;
+  }
+  if (stage && stage == direct) { /* written in place */
+    *out = NULL;
+    *bytes = n * esz;
+    return XGPU_OK;
   }
   if (tri) {
     void *target = direct;
@@ -782,12 +797,13 @@ static int dump_stage(XGPUAmdEngine *e, int flags, void *direct, const void **ou
 int xg_dump_async(XGPUAmdEngine *e, void *dst, int flags) {
   const void *val = NULL;
   size_t bytes = 0;
-  const int tri = (flags & XGPU_DUMP_TRIANGULAR) && e->cfg.matrix_order != TRIANGULAR_ORDER;
   trace_event(e, "dump", "dst=%p%s%s%s samples=%llu", dst, (flags & XGPU_DUMP_DEVICE) ? " device" : " host",
               (flags & XGPU_DUMP_WIDE) ? " wide" : "", (flags & XGPU_DUMP_TRIANGULAR) ? " triangular" : "",
               (unsigned long long)e->samples);
-  /* A triangular dump to device memory is reordered straight into dst. */
-  int rc = dump_stage(e, flags, (tri && (flags & XGPU_DUMP_DEVICE)) ? dst : NULL, &val, &bytes);
+  /* A dump to device memory is written straight into dst by the kernel that
+   * makes it (reorder, combine, widen or zero); only a plain copy of the
+   * integration goes through a memcpy. */
+  int rc = dump_stage(e, flags, (flags & XGPU_DUMP_DEVICE) ? dst : NULL, &val, &bytes);
   if (rc) return rc;
   if (!val) return XGPU_OK;
   if (flags & XGPU_DUMP_DEVICE)
