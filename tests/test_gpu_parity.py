@@ -495,6 +495,64 @@ def test_f16_spilled_triangular_dump(order):
     assert np.array_equal(outs[1], outs[4]) and np.array_equal(outs[1], outs[5])
 
 
+@pytest.mark.parametrize("compute", ["i8", "f16"])
+def test_triangular_device_dumps_every_state(compute):
+    """A triangular-order engine dumps into device memory through the kernel
+    that forms the dump, writing straight into the destination: a zero fill
+    after a clear, the widening before any spill, the spill + current
+    combine after one (3 x 40000 samples), wide and narrow.  Destinations
+    16-byte aligned (4 elements per thread) and off by one element (one per
+    thread), filled with garbage first.  int8 against k x the int64 oracle
+    exactly (narrow: modulo 2^32); fp16 device dumps equal to the host dumps
+    of the same state, which are within (k NT / 16 + 4) 2^-24 of the FP64
+    oracle, normalized."""
+    ns, nf, nt = 8, 2, 40000
+    i8 = compute == "i8"
+    if i8:
+        x = oracle.fill_uniform(nt * nf * ns * 4, 31, -128, 127)
+        ref1 = oracle.xengine_i64(x, ns, nf, nt)
+    else:
+        x = f16_input(nt, nf, ns, 32, sigma=4.0)
+        ref1 = oracle.xengine_f64_f16(x, ns, nf, nt)
+    eng = X.Engine(ns, nf, compute=X.XGPU_INT8 if i8 else X.XGPU_FLOAT16,
+                   order=X.TRIANGULAR_ORDER)
+    buf = X.DeviceBuffer.from_array(x)
+    n = ref1.size
+    out = X.DeviceBuffer(n * 8 + 16)
+    wdt, ndt = (np.int64, np.int32) if i8 else (np.float64, np.float32)
+    done = 0
+    for k in (0, 1, 3):
+        flush(f"[{compute}] triangular device dumps after {k} x {nt} samples")
+        if k == 0:
+            eng.clear()
+        while done < k:
+            eng.correlate(buf, nt)
+            done += 1
+        host = {True: eng.dump(wide=True), False: eng.dump()}
+        for wide, dt, offs in ((True, wdt, (0, 8)), (False, ndt, (0, 4))):
+            flags = X.XGPU_DUMP_DEVICE | X.XGPU_DUMP_TRIANGULAR | (X.XGPU_DUMP_WIDE if wide else 0)
+            for off in offs:
+                out.fill_uniform_int8(40 + off, -127, 127)
+                X.check(X.core().xgpuAmdDump(eng.handle, out.ptr + off, flags))
+                got = out.download(dt, count=n, offset=off)
+                assert np.array_equal(got, host[wide]), (k, wide, off)
+        if i8:
+            exact = k * ref1
+            assert np.array_equal(host[True], exact), k
+            assert np.array_equal(host[False].astype(np.int64),
+                                  ((exact + 2**31) % 2**32) - 2**31), k
+        elif k == 0:
+            assert not host[True].any() and not host[False].any()
+        else:
+            bound = (k * nt / 16 + 4) * 2.0 ** -24
+            for wide in (True, False):
+                st = W.vis_error_stats(host[wide], k * ref1, ns)
+                assert st["max_norm_err"] <= bound, (k, wide, st)
+    out.free()
+    buf.free()
+    eng.close()
+
+
 # ------------------------------------------------------------------ hipGraph steps
 def test_device_path_graph_replays():
     """graph knob on: a device-path call (layout conversion + launch) is

@@ -2096,19 +2096,71 @@ __global__ void xg_copy_k(uint4 *dst, const uint4 *src, size_t n16, unsigned *dt
   if (t < ntail) dtail[t] = stail[t];
 }
 
-// dst += src (ADD) or dst = src, widening each element.
-template <typename D, typename S, bool ADD>
-__global__ void xg_spill_k(D *dst, const S *src, size_t n) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    dst[i] = ADD ? dst[i] + (D)src[i] : (D)src[i];
+// Four consecutive 4- or 8-byte elements through 16-byte accesses (p 16-byte aligned).
+template <typename T>
+__device__ __forceinline__ void xg_ld4(const T *p, T (&o)[4]) {
+  if constexpr (sizeof(T) == 4) {
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    const T4 v = *(const T4 *)p;
+    o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
+  } else {
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    const T2 a = ((const T2 *)p)[0], b = ((const T2 *)p)[1];
+    o[0] = a[0]; o[1] = a[1]; o[2] = b[0]; o[3] = b[1];
+  }
+}
+template <typename T>
+__device__ __forceinline__ void xg_st4(T *p, const T (&v)[4]) {
+  if constexpr (sizeof(T) == 4) {
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    *(T4 *)p = T4{v[0], v[1], v[2], v[3]};
+  } else {
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    ((T2 *)p)[0] = T2{v[0], v[1]};
+    ((T2 *)p)[1] = T2{v[2], v[3]};
+  }
 }
 
-template <typename W, typename S, typename O>
-__global__ void xg_combine_k(O *dst, const W *spill, const S *cur, size_t n) {
+// dst += src (ADD) or dst = src, widening each element.  V = 4: each thread
+// takes 4 consecutive elements with 16-byte accesses (nv = n / 4, every
+// pointer 16-byte aligned); V = 1: one element per thread (nv = n).
+template <typename D, typename S, bool ADD, int V>
+__global__ void xg_spill_k(D *dst, const S *src, size_t nv) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    dst[i] = (O)(spill[i] + (W)cur[i]);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
+    if constexpr (V == 1) {
+      dst[i] = ADD ? dst[i] + (D)src[i] : (D)src[i];
+    } else {
+      S s[4];
+      D d[4];
+      xg_ld4(src + 4 * i, s);
+      if (ADD) xg_ld4(dst + 4 * i, d);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) d[k] = ADD ? d[k] + (D)s[k] : (D)s[k];
+      xg_st4(dst + 4 * i, d);
+    }
+  }
+}
+
+// dst = spill + cur, in the wide type W, stored as O (W, or its low 32 bits /
+// float rounding for a narrow dump).  V as for xg_spill_k.
+template <typename W, typename S, typename O, int V>
+__global__ void xg_combine_k(O *dst, const W *spill, const S *cur, size_t nv) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
+    if constexpr (V == 1) {
+      dst[i] = (O)(spill[i] + (W)cur[i]);
+    } else {
+      W w[4];
+      S s[4];
+      O o[4];
+      xg_ld4(spill + 4 * i, w);
+      xg_ld4(cur + 4 * i, s);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) o[k] = (O)(w[k] + (W)s[k]);
+      xg_st4(dst + 4 * i, o);
+    }
+  }
 }
 
 // Counter-based generator: byte i = lo + ((hi32(splitmix64(key + i)) * span) >> 32)
@@ -2481,22 +2533,38 @@ extern "C" int xgh_launch_copy(void *dst, const void *src, size_t bytes, void *s
   return xgh_report(hipGetLastError(), "xg_copy launch");
 }
 
+template <typename D, typename S, bool ADD>
+static void launch_spill(D *dst, const S *src, size_t n, hipStream_t s) {
+  if ((((uintptr_t)dst | (uintptr_t)src) & 15) == 0 && (n & 3) == 0)
+    hipLaunchKernelGGL((xg_spill_k<D, S, ADD, 4>), dim3(grid_for(n / 4, 256)), dim3(256), 0, s,
+                       dst, src, n / 4);
+  else
+    hipLaunchKernelGGL((xg_spill_k<D, S, ADD, 1>), dim3(grid_for(n, 256)), dim3(256), 0, s, dst,
+                       src, n);
+}
+
+template <typename W, typename S, typename O>
+static void launch_combine(O *dst, const W *spill, const S *cur, size_t n, hipStream_t s) {
+  if ((((uintptr_t)dst | (uintptr_t)spill | (uintptr_t)cur) & 15) == 0 && (n & 3) == 0)
+    hipLaunchKernelGGL((xg_combine_k<W, S, O, 4>), dim3(grid_for(n / 4, 256)), dim3(256), 0, s,
+                       dst, spill, cur, n / 4);
+  else
+    hipLaunchKernelGGL((xg_combine_k<W, S, O, 1>), dim3(grid_for(n, 256)), dim3(256), 0, s, dst,
+                       spill, cur, n);
+}
+
 extern "C" int xgh_launch_spill(int kind, int add, void *dst, const void *src, size_t n,
                                 void *stream) {
   if (!n) return 0;
-  const dim3 g(grid_for(n, 256)), b(256);
+  hipStream_t s = (hipStream_t)stream;
   if (kind == XG_OUT_I64 && add)
-    hipLaunchKernelGGL((xg_spill_k<long long, int, true>), g, b, 0, (hipStream_t)stream,
-                       (long long *)dst, (const int *)src, n);
+    launch_spill<long long, int, true>((long long *)dst, (const int *)src, n, s);
   else if (kind == XG_OUT_I64)
-    hipLaunchKernelGGL((xg_spill_k<long long, int, false>), g, b, 0, (hipStream_t)stream,
-                       (long long *)dst, (const int *)src, n);
+    launch_spill<long long, int, false>((long long *)dst, (const int *)src, n, s);
   else if (kind == XG_OUT_F64 && add)
-    hipLaunchKernelGGL((xg_spill_k<double, float, true>), g, b, 0, (hipStream_t)stream,
-                       (double *)dst, (const float *)src, n);
+    launch_spill<double, float, true>((double *)dst, (const float *)src, n, s);
   else if (kind == XG_OUT_F64)
-    hipLaunchKernelGGL((xg_spill_k<double, float, false>), g, b, 0, (hipStream_t)stream,
-                       (double *)dst, (const float *)src, n);
+    launch_spill<double, float, false>((double *)dst, (const float *)src, n, s);
   else
     return xgh_report(hipErrorInvalidValue, "xgh_launch_spill(kind)");
   return xgh_report(hipGetLastError(), "xg_spill launch");
@@ -2506,21 +2574,16 @@ extern "C" int xgh_launch_combine(int kind, int narrow, void *dst, const void *s
                                   const void *cur, size_t n, void *stream) {
   if (!n) return 0;
   hipStream_t s = (hipStream_t)stream;
-  const dim3 g(grid_for(n, 256)), b(256);
   if (kind == XG_OUT_I64) {
     if (narrow)  // low 32 bits: what wrapping int32 accumulation would hold
-      hipLaunchKernelGGL((xg_combine_k<unsigned long long, int, unsigned>), g, b, 0, s,
-                         (unsigned *)dst, (const unsigned long long *)spill, (const int *)cur, n);
+      launch_combine((unsigned *)dst, (const unsigned long long *)spill, (const int *)cur, n, s);
     else
-      hipLaunchKernelGGL((xg_combine_k<long long, int, long long>), g, b, 0, s, (long long *)dst,
-                         (const long long *)spill, (const int *)cur, n);
+      launch_combine((long long *)dst, (const long long *)spill, (const int *)cur, n, s);
   } else if (kind == XG_OUT_F64) {
     if (narrow)
-      hipLaunchKernelGGL((xg_combine_k<double, float, float>), g, b, 0, s, (float *)dst,
-                         (const double *)spill, (const float *)cur, n);
+      launch_combine((float *)dst, (const double *)spill, (const float *)cur, n, s);
     else
-      hipLaunchKernelGGL((xg_combine_k<double, float, double>), g, b, 0, s, (double *)dst,
-                         (const double *)spill, (const float *)cur, n);
+      launch_combine((double *)dst, (const double *)spill, (const float *)cur, n, s);
   } else {
     return xgh_report(hipErrorInvalidValue, "xgh_launch_combine(kind)");
   }
