@@ -736,7 +736,8 @@ __device__ __forceinline__ void rg_combine(const Acc &blk, int q, bool odd, cons
 // (register tile) or 32 cells instead of 4-byte pieces.  Index formulas as in vis_index (reference
 // src/kernel.cuh:49-117).
 #define RG_CELL_GROUP 144                     // LDS bytes per quad group: 2 x 64 + 16 padding
-#define RG_CELL_SCRATCH (16 * RG_CELL_GROUP)  // per wave (16 quad groups)
+#define RG_CELL_SCRATCH (16 * RG_CELL_GROUP)  // per wave and step (16 quad groups)
+#define RG_EPI_LDS (4 * 2 * RG_CELL_SCRATCH)  // 4 waves x 2 halves: the epilogue's LDS
 
 // ORDER: an XG_ORDER_* constant, or 0 to read the order from the job.
 template <int ORDER, class Acc, class SW>
@@ -753,8 +754,8 @@ __device__ __forceinline__ void rg_store_cells(const XgJob &J, const TileRef &T,
   const int e = 2 * (lane & 1) + ((lane >> 1) & 1);  // this lane's product: 2 p1 + p2
   const int bq = lane & 3;                           // the cell (column block) this lane stores
   // quad group (hi, mj); the 144-byte group stride spreads the writes over the banks
-  unsigned char *scr =
-      lds + (wr * 2 + wc) * RG_CELL_SCRATCH + (hi * 8 + mj) * RG_CELL_GROUP;
+  unsigned char *scr0 =
+      lds + (wr * 2 + wc) * 2 * RG_CELL_SCRATCH + (hi * 8 + mj) * RG_CELL_GROUP;
   const int ns = J.nstation;
   const int accum = J.accumulate;
   const long long plane = J.plane;
@@ -769,57 +770,73 @@ __device__ __forceinline__ void rg_store_cells(const XgJob &J, const TileRef &T,
     const long long NB = (long long)ns * (ns + 1) / 2;
     base = (S *)J.out + (tri ? 8 : 4) * ((long long)T.f * NB + j);
   }
+  // Cells of step n = 4 a + q (row block a, row pair q): combined, sent
+  // through scratch half n & 1 and read back whole.
+  auto gather = [&](int n, S4 &cre, S4 &cim) {
+    const int a = n >> 2, q = n & 3;
+    unsigned char *scr = scr0 + (n & 1) * RG_CELL_SCRATCH;
+    E re[4], im[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) rg_combine(acc[a][b], q, odd, swap, re[b], im[b]);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      __builtin_amdgcn_sched_barrier(0);
-      E re[4], im[4];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) rg_combine(acc[a][b], q, odd, swap, re[b], im[b]);
-      // the cell's re and im planes through LDS
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        *(S *)(scr + 16 * b + 4 * e) = (S)re[b];
-        *(S *)(scr + 64 + 16 * b + 4 * e) = (S)im[b];
-      }
-      const S4 cre = *(const S4 *)(scr + 16 * bq);
-      const S4 cim = *(const S4 *)(scr + 64 + 16 * bq);
-      S4 v0 = cre, v1 = cim;
-      if (tri) {  // triangular order interleaves (re, im) by product
-        v0 = S4{cre[0], cim[0], cre[1], cim[1]};
-        v1 = S4{cre[2], cim[2], cre[3], cim[3]};
-      }
-      const int i = T.rb * 64 + 32 * wr + hi + 8 * a + 2 * q;
-      if (!full) {
-        if (i >= ns || j >= ns) continue;
-        if (i < j) {
-          // quadrant "12" of a diagonal register-tile cell is never written by
-          // the reference (kernel.cuh:65-71): zero after a clear
-          if (!regt || (i >> 1) != (j >> 1) || accum) continue;
-          v0 = (S4)(S)0;
-          v1 = (S4)(S)0;
-        }
-      }
-      S *p, *p1;
-      if (regt) {
-        const long long R = i >> 1;
-        p = base + 4 * (R * (R + 1) / 2);
-        p1 = p + plane;
-      } else {
-        const long long rowi = (long long)i * (i + 1) / 2;
-        p = base + (tri ? 8 : 4) * rowi;
-        p1 = tri ? p + 4 : p + plane;
-      }
-      if (accum) {
-        v0 += *(const S4 *)p;
-        v1 += *(const S4 *)p1;
-      }
-      // streaming stores: the tile's output is not read again by this
-      // launch and should not evict the input the next round reuses
-      __builtin_nontemporal_store(v0, (S4 *)p);
-      __builtin_nontemporal_store(v1, (S4 *)p1);
+    for (int b = 0; b < 4; ++b) {
+      *(S *)(scr + 16 * b + 4 * e) = (S)re[b];
+      *(S *)(scr + 64 + 16 * b + 4 * e) = (S)im[b];
     }
+    cre = *(const S4 *)(scr + 16 * bq);
+    cim = *(const S4 *)(scr + 64 + 16 * bq);
+  };
+  auto put = [&](int n, const S4 &cre, const S4 &cim) {
+    const int a = n >> 2, q = n & 3;
+    S4 v0 = cre, v1 = cim;
+    if (tri) {  // triangular order interleaves (re, im) by product
+      v0 = S4{cre[0], cim[0], cre[1], cim[1]};
+      v1 = S4{cre[2], cim[2], cre[3], cim[3]};
+    }
+    const int i = T.rb * 64 + 32 * wr + hi + 8 * a + 2 * q;
+    if (!full) {
+      if (i >= ns || j >= ns) return;
+      if (i < j) {
+        // quadrant "12" of a diagonal register-tile cell is never written by
+        // the reference (kernel.cuh:65-71): zero after a clear
+        if (!regt || (i >> 1) != (j >> 1) || accum) return;
+        v0 = (S4)(S)0;
+        v1 = (S4)(S)0;
+      }
+    }
+    S *p, *p1;
+    if (regt) {
+      const long long R = i >> 1;
+      p = base + 4 * (R * (R + 1) / 2);
+      p1 = p + plane;
+    } else {
+      const long long rowi = (long long)i * (i + 1) / 2;
+      p = base + (tri ? 8 : 4) * rowi;
+      p1 = tri ? p + 4 : p + plane;
+    }
+    if (accum) {
+      v0 += *(const S4 *)p;
+      v1 += *(const S4 *)p1;
+    }
+    // streaming stores: the tile's output is not read again by this
+    // launch and should not evict the input the next round reuses
+    __builtin_nontemporal_store(v0, (S4 *)p);
+    __builtin_nontemporal_store(v1, (S4 *)p1);
+  };
+  // Software-pipelined by one step: step n + 1's lane exchange and LDS
+  // round trip are issued before step n's bounds checks and stores, whose
+  // branches would otherwise keep the two steps apart.
+  S4 c0, c1;
+  gather(0, c0, c1);
+#pragma unroll
+  for (int n = 0; n < 15; ++n) {
+    S4 n0, n1;
+    gather(n + 1, n0, n1);
+    put(n, c0, c1);
+    c0 = n0;
+    c1 = n1;
+  }
+  put(15, c0, c1);
 }
 
 // int32 output: one code path for the three orders (no per-order copies of
@@ -849,8 +866,8 @@ __device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, v1
 // waits for every store and load issued before it, which would serialize the
 // epilogue's stores.  On a diagonal tile the wave above the diagonal
 // (wr < wc) duplicates its mirror and stores nothing.
-// The 32-bit stores use the first RG_CELL_SCRATCH x 4 bytes of LDS: every
-// wave must be done with the stage buffers (the caller's barrier).
+// The 32-bit stores use the first RG_EPI_LDS bytes of LDS: every wave must
+// be done with the stage buffers (the caller's barrier).
 template <class Acc>
 __device__ __forceinline__ void rg_finish(const XgJob &J, const TileRef &T, Acc (&acc)[4][4],
                                           unsigned char *lds) {
@@ -1089,6 +1106,7 @@ __device__ __forceinline__ void rg_tile(const XgJob &J, const TileRef &T, const 
 template <class E, int STG>
 __device__ __forceinline__ void rg_run(const XgLaunch &L, unsigned char *lds) {
   typedef RgCfg<E, STG> C;
+  static_assert(C::kLds >= RG_EPI_LDS, "the epilogue's scratch fits in the stage buffers");
   const int xcd = blockIdx.x % XG_NXCD;
   const int wl = blockIdx.x / XG_NXCD;
   TileRef T;
@@ -1183,6 +1201,7 @@ template <class E, int STG, int NBUF>
 __device__ __forceinline__ void rgd_tile(const XgJob &J, const TileRef &T, unsigned char *lds) {
   typedef RgdCfg<E, STG, NBUF> C;
   static_assert((NBUF & (NBUF - 1)) == 0 && NBUF >= 2, "NBUF: power of two");
+  static_assert(C::kLds >= RG_EPI_LDS, "the epilogue's scratch fits in the stage buffers");
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ninput = 2 * J.nstation;
