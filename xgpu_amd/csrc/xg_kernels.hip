@@ -690,8 +690,20 @@ __device__ __forceinline__ int rg_sub(int a, int b) { return (int)((unsigned)a -
 __device__ __forceinline__ float rg_add(float a, float b) { return a + b; }
 __device__ __forceinline__ float rg_sub(float a, float b) { return a - b; }
 
-// Exchange of a value between lanes l and l ^ 1 through ds_bpermute, with an
-// address the caller computed from its own lane index.
+// Exchange of a value between lanes l and l ^ 1 with DPP quad_perm(1, 0, 3, 2):
+// no LDS-queue round trip, and the compiler folds it into the add or subtract
+// that uses it.
+struct RgDppSwap {
+  __device__ __forceinline__ int operator()(int x) const {
+    return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
+  }
+  __device__ __forceinline__ float operator()(float x) const {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+  }
+};
+
+// The same exchange through ds_bpermute, with an address the caller computed
+// from its own lane index.
 struct RgBperm {
   int addr;  // 4 * (lane ^ 1)
   __device__ __forceinline__ int operator()(int x) const {
@@ -708,12 +720,16 @@ __device__ __forceinline__ void rg_combine(const Acc &blk, int q, bool odd, cons
   const T x0 = blk[4 * q + 0], x1 = blk[4 * q + 1];
   const T x2 = blk[4 * q + 2], x3 = blk[4 * q + 3];
   // each lane sends what its partner lacks: odd lanes their e%4 = 0,1
-  // (column n_im of row m), even lanes their e%4 = 2,3 (column n_re of m+1)
-  const T y0 = swap(odd ? x0 : x2);
+  // (column n_im of row m; e%4 = 0 negated), even lanes their e%4 = 2,3
+  // (column n_re of m+1).  The negation on the sending side makes every
+  // received value an addend or a minuend: with the DPP swap, a received
+  // subtrahend compiled to v_subrev_u32_dpp, whose int32 results were wrong
+  // on gfx950 (the float form was not affected).
+  const T y0 = swap(odd ? rg_sub(T(0), x0) : x2);
   const T y1 = swap(odd ? x1 : x3);
-  if (!odd) {  // x0 = G[mr][nr], x1 = G[mi][nr], y0 = G[mr][ni], y1 = G[mi][ni]
+  if (!odd) {  // x0 = G[mr][nr], x1 = G[mi][nr], y0 = -G[mr][ni], y1 = G[mi][ni]
     re = rg_add(x0, y1);
-    im = rg_sub(x1, y0);
+    im = rg_add(x1, y0);
   } else {     // x2 = G[m'r][ni], x3 = G[m'i][ni], y0 = G[m'r][nr], y1 = G[m'i][nr]
     re = rg_add(y0, x3);
     im = rg_sub(y1, x2);
@@ -876,7 +892,7 @@ __device__ __forceinline__ void rg_finish(const XgJob &J, const TileRef &T, Acc 
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   if (!(T.rb == T.cb && wr < wc))
-    rg_epilogue(J, T, acc, wr, wc, lane, RgBperm{4 * (lane ^ 1)}, lds);
+    rg_epilogue(J, T, acc, wr, wc, lane, RgDppSwap{}, lds);
 }
 
 // Column block of the i-th of a kstep's 16 MFMAs (row block i / 4): the
