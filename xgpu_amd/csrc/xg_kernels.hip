@@ -855,14 +855,21 @@ __device__ __forceinline__ void rg_store_cells(const XgJob &J, const TileRef &T,
   put(15, c0, c1);
 }
 
-// int32 output: one code path for the three orders (no per-order copies of
-// the accumulator reads for the compiler to hoist and spill); float output:
-// one copy per order, which measures faster for the fp16 kernels.
+// One copy of the stores per output order, int32 and float.  With the
+// epilogue pipelined (each step's accumulator reads feed the next step's
+// exchange), the per-order copies no longer make the compiler hoist those
+// reads and spill them; they leave the int8 DMA kernels with no spill at all
+// and measure faster than one code path that reads the order at run time.
 template <class SW>
 __device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, v16i (&acc)[4][4],
                                             int wr, int wc, int lane, const SW &swap,
                                             unsigned char *lds) {
-  rg_store_cells<0>(J, T, acc, wr, wc, lane, swap, lds);
+  if (J.order == XG_ORDER_REGTILE)
+    rg_store_cells<XG_ORDER_REGTILE>(J, T, acc, wr, wc, lane, swap, lds);
+  else if (J.order == XG_ORDER_TRI)
+    rg_store_cells<XG_ORDER_TRI>(J, T, acc, wr, wc, lane, swap, lds);
+  else
+    rg_store_cells<XG_ORDER_REALIMAG>(J, T, acc, wr, wc, lane, swap, lds);
 }
 template <class SW>
 __device__ __forceinline__ void rg_epilogue(const XgJob &J, const TileRef &T, v16f (&acc)[4][4],
