@@ -678,11 +678,18 @@ template <>
 struct RgElem<v16i> {
   typedef int T;       // accumulator element
   typedef unsigned S;  // 32-bit output element
+  // Non-temporal output stores (the int8 kernels: plain stores measured
+  // 0.15 % slower at NS=512 and 0.45 % at NS=2048, where the input is
+  // re-read from beyond L2 and the output would displace it).
+  static constexpr bool kStreamOut = true;
 };
 template <>
 struct RgElem<v16f> {
   typedef float T;
   typedef float S;
+  // Plain output stores (the fp16 kernels: 0.6-1.2 % faster than
+  // non-temporal ones at NS=256, NF=128, NT=2048).
+  static constexpr bool kStreamOut = false;
 };
 
 __device__ __forceinline__ int rg_add(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
@@ -834,10 +841,15 @@ __device__ __forceinline__ void rg_store_cells(const XgJob &J, const TileRef &T,
       v0 += *(const S4 *)p;
       v1 += *(const S4 *)p1;
     }
-    // streaming stores: the tile's output is not read again by this
-    // launch and should not evict the input the next round reuses
-    __builtin_nontemporal_store(v0, (S4 *)p);
-    __builtin_nontemporal_store(v1, (S4 *)p1);
+    if constexpr (RgElem<Acc>::kStreamOut) {
+      // streaming stores: the tile's output is not read again by this
+      // launch and should not evict the input the next round reuses
+      __builtin_nontemporal_store(v0, (S4 *)p);
+      __builtin_nontemporal_store(v1, (S4 *)p1);
+    } else {
+      *(S4 *)p = v0;
+      *(S4 *)p1 = v1;
+    }
   };
   // Software-pipelined by one step: step n + 1's lane exchange and LDS
   // round trip are issued before step n's bounds checks and stores, whose
